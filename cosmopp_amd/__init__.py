@@ -1,0 +1,319 @@
+"""cosmopp_amd - MI355X-native map-level CMB likelihood (HIP, gfx950, FP64).
+
+Python binding of the C ABI in include/cmbl.h (library
+cosmopp_amd/libcosmopp_amd.so).  The compute path is the HIP library only:
+if the library is missing or no GPU is usable, every entry point raises.
+
+Reference interface mirrored (aslanyan/cosmopp 3.1):
+  MapLikelihood.evaluate(x)            Math::RealFunctionMultiDim::evaluate   include/function.hpp:138
+  MapLikelihood.evaluate_grad(x, res)  Math::RealFunctionMultiToMulti::evaluate include/function.hpp:168
+  MapLikelihood.calculate / calculate_deriv
+                                       Math::LikelihoodWithDerivs            include/likelihood_function.hpp:39,46
+  simulate_alm / simulate_white_noise  Simulate::simulateAlm / simulateWhiteNoise source/simulate.cpp:162-203
+  beam                                 Utils::beamFunction                   source/utils.cpp:54-64
+  mask_regions                         Utils::maskRegions                    source/utils.cpp:235-253
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcosmopp_amd.so")
+
+CMBL_OK = 0
+CMBL_ERR_ARG = 1
+CMBL_ERR_HIP = 2
+CMBL_ERR_ALLOC = 3
+CMBL_ERR_STATE = 4
+CMBL_ERR_NONFINITE = 5
+
+
+class CmblError(RuntimeError):
+    """Raised for a non-zero status of the C ABI (StandardException analogue)."""
+
+    def __init__(self, code, msg):
+        super().__init__("cmbl error %d: %s" % (code, msg))
+        self.code = code
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_lib = None
+
+# name -> (argtypes, restype)
+_SIGS = {
+    "cmbl_last_error": ([], ctypes.c_char_p),
+    "cmbl_version": ([], ctypes.c_char_p),
+    "cmbl_map_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "cmbl_map_destroy": ([_vp], None),
+    "cmbl_map_nside": ([_vp], ctypes.c_int),
+    "cmbl_map_lmax": ([_vp], ctypes.c_int),
+    "cmbl_map_npix": ([_vp], ctypes.c_long),
+    "cmbl_map_nparams": ([_vp], ctypes.c_long),
+    "cmbl_map_set_beam": ([_vp, _vp], ctypes.c_int),
+    "cmbl_map_set_cl": ([_vp, _vp], ctypes.c_int),
+    "cmbl_map_set_data": ([_vp, _vp, _vp], ctypes.c_int),
+    "cmbl_map_set_data_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_value_and_grad": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_value_and_grad_device": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_forward": ([_vp, _vp, _vp], ctypes.c_int),
+    "cmbl_forward_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_backward": ([_vp, _vp, _vp], ctypes.c_int),
+    "cmbl_backward_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_batch_create": ([ctypes.POINTER(_vp), _vp, ctypes.c_int], ctypes.c_int),
+    "cmbl_batch_destroy": ([_vp], None),
+    "cmbl_batch_value_and_grad_device": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_simulate_alm": ([_vp, ctypes.c_int, ctypes.c_long, _vp], ctypes.c_int),
+    "cmbl_simulate_white_noise": ([ctypes.c_int, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
+    "cmbl_beam": ([ctypes.c_int, ctypes.c_double, _vp], ctypes.c_int),
+    "cmbl_mask_latitude": ([ctypes.c_int, ctypes.c_double, _vp], ctypes.c_int),
+    "cmbl_mask_regions": ([ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_weights_from_mask": ([ctypes.c_long, _vp, ctypes.c_double, _vp], ctypes.c_int),
+    "cmbl_pix2ang_ring": ([ctypes.c_int, ctypes.c_long, _dp, _dp], ctypes.c_int),
+    "cmbl_vec_copy": ([_vp, _vp, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
+    "cmbl_vec_set": ([_vp, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
+    "cmbl_vec_axpy": ([_vp, _vp, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
+    "cmbl_vec_mul": ([_vp, _vp, ctypes.c_long, _vp], ctypes.c_int),
+    "cmbl_vec_div": ([_vp, _vp, ctypes.c_long, _vp], ctypes.c_int),
+    "cmbl_vec_pow": ([_vp, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
+    "cmbl_vec_dot": ([_vp, _vp, ctypes.c_long, _dp, _vp], ctypes.c_int),
+    "cmbl_vec_dot_device": ([_vp, _vp, ctypes.c_long, _vp, _vp], ctypes.c_int),
+}
+
+
+def declared_symbols():
+    """Every entry point bound here (the declarations of include/cmbl.h)."""
+    return sorted(_SIGS)
+
+
+def lib():
+    """The loaded HIP library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("cosmopp_amd: %s is missing; run `python -m cosmopp_amd.build` "
+                               "(there is no CPU fallback)" % LIB_PATH)
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(l, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = l
+    return _lib
+
+
+def _check(rc):
+    if rc != CMBL_OK:
+        raise CmblError(rc, lib().cmbl_last_error().decode())
+
+
+def _arr(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError("expected %d values, got %d" % (n, a.size))
+    return a
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp)
+
+
+# ------------------------------------------------------------------ inputs (host)
+
+def simulate_alm(cl, lmax, seed):
+    """Packed a_lm drawn as Simulate::simulateAlm (mt19937 + std::normal_distribution)."""
+    cl = _arr(cl)
+    out = np.zeros((lmax + 1) ** 2)
+    _check(lib().cmbl_simulate_alm(_p(cl), lmax, seed, _p(out)))
+    return out
+
+
+def simulate_white_noise(nside, sigma, seed):
+    out = np.zeros(12 * nside * nside)
+    _check(lib().cmbl_simulate_white_noise(nside, sigma, seed, _p(out)))
+    return out
+
+
+def beam(lmax, fwhm_deg):
+    out = np.zeros(lmax + 1)
+    _check(lib().cmbl_beam(lmax, fwhm_deg, _p(out)))
+    return out
+
+
+def mask_latitude(nside, cut_rad, mask=None):
+    mask = np.ones(12 * nside * nside) if mask is None else _arr(mask).copy()
+    _check(lib().cmbl_mask_latitude(nside, cut_rad, _p(mask)))
+    return mask
+
+
+def mask_regions(nside, theta, phi, angle, mask=None):
+    mask = np.ones(12 * nside * nside) if mask is None else _arr(mask).copy()
+    th, ph, an = _arr(theta), _arr(phi), _arr(angle)
+    _check(lib().cmbl_mask_regions(nside, th.size, _p(th), _p(ph), _p(an), _p(mask)))
+    return mask
+
+
+def weights_from_mask(mask, sigma):
+    mask = _arr(mask)
+    w = np.zeros_like(mask)
+    _check(lib().cmbl_weights_from_mask(mask.size, _p(mask), sigma, _p(w)))
+    return w
+
+
+def pix2ang_ring(nside, pix):
+    t, p = ctypes.c_double(), ctypes.c_double()
+    _check(lib().cmbl_pix2ang_ring(nside, pix, ctypes.byref(t), ctypes.byref(p)))
+    return t.value, p.value
+
+
+def cl_shape(lmax, amp=1e3):
+    """C_l = amp / (l (l+1)) for l >= 2, C_0 = C_1 = 0."""
+    cl = np.zeros(lmax + 1)
+    l = np.arange(2, lmax + 1)
+    cl[2:] = amp / (l * (l + 1.0))
+    return cl
+
+
+# ------------------------------------------------------------------ map context
+
+class Map:
+    """One HEALPix map with its likelihood tables on the current GPU (cmbl_map)."""
+
+    def __init__(self, nside, lmax):
+        h = _vp()
+        _check(lib().cmbl_map_create(ctypes.byref(h), nside, lmax))
+        self._h = h
+        self.nside = nside
+        self.lmax = lmax
+        self.npix = 12 * nside * nside
+        self.nparams = (lmax + 1) ** 2
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().cmbl_map_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_beam(self, b):
+        b = _arr(b, self.lmax + 1)
+        _check(lib().cmbl_map_set_beam(self._h, _p(b)))
+
+    def set_cl(self, cl):
+        cl = _arr(cl, self.lmax + 1)
+        _check(lib().cmbl_map_set_cl(self._h, _p(cl)))
+
+    def set_data(self, d, w):
+        d, w = _arr(d, self.npix), _arr(w, self.npix)
+        _check(lib().cmbl_map_set_data(self._h, _p(d), _p(w)))
+
+    def set_data_device(self, d_ptr, w_ptr, stream=0):
+        _check(lib().cmbl_map_set_data_device(self._h, d_ptr, w_ptr, stream))
+
+    def value_and_grad(self, x, grad=True):
+        x = _arr(x, self.nparams)
+        f = np.zeros(1)
+        g = np.zeros(self.nparams) if grad else None
+        _check(lib().cmbl_value_and_grad(self._h, _p(x), _p(f), None if g is None else _p(g)))
+        return (f[0], g) if grad else f[0]
+
+    def value_and_grad_device(self, x_ptr, f_ptr, g_ptr, stream=0):
+        _check(lib().cmbl_value_and_grad_device(self._h, x_ptr, f_ptr, g_ptr, stream))
+
+    def forward(self, x):
+        x = _arr(x, self.nparams)
+        out = np.zeros(self.npix)
+        _check(lib().cmbl_forward(self._h, _p(x), _p(out)))
+        return out
+
+    def forward_device(self, x_ptr, map_ptr, stream=0):
+        _check(lib().cmbl_forward_device(self._h, x_ptr, map_ptr, stream))
+
+    def backward(self, v):
+        v = _arr(v, self.npix)
+        out = np.zeros(self.nparams)
+        _check(lib().cmbl_backward(self._h, _p(v), _p(out)))
+        return out
+
+    def backward_device(self, v_ptr, g_ptr, stream=0):
+        _check(lib().cmbl_backward_device(self._h, v_ptr, g_ptr, stream))
+
+
+class Batch:
+    """Many maps evaluated with one launch per stage (cmbl_batch)."""
+
+    def __init__(self, maps):
+        self.maps = list(maps)
+        arr = (_vp * len(self.maps))(*[m.handle for m in self.maps])
+        h = _vp()
+        _check(lib().cmbl_batch_create(ctypes.byref(h), ctypes.cast(arr, _vp), len(self.maps)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().cmbl_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def value_and_grad_device(self, x_ptrs, f_ptr, g_ptrs, stream=0):
+        n = len(self.maps)
+        xs = (_vp * n)(*x_ptrs)
+        gs = (_vp * n)(*g_ptrs)
+        _check(lib().cmbl_batch_value_and_grad_device(self._h, ctypes.cast(xs, _vp), f_ptr,
+                                                      ctypes.cast(gs, _vp), stream))
+
+
+class MapLikelihood:
+    """-2lnL(a_lm) with gradient behind the reference's function interfaces.
+
+    evaluate(x) -> -2lnL                      (RealFunctionMultiDim::evaluate)
+    evaluate_grad(x, res) -> fills res        (RealFunctionMultiToMulti::evaluate; res is resized)
+    calculate(params) / calculate_deriv(params, i)  (LikelihoodWithDerivs)
+    The last (x, value, gradient) is cached by contents, so value + derivative
+    at one point cost one device evaluation.
+    """
+
+    def __init__(self, mp):
+        self.map = mp
+        self._x = None
+        self._f = None
+        self._g = None
+        self.evaluations = 0
+
+    def _eval(self, x):
+        x = _arr(x, self.map.nparams)
+        if self._x is None or not np.array_equal(self._x, x):
+            self._f, self._g = self.map.value_and_grad(x)
+            self._x = x.copy()
+            self.evaluations += 1
+        return self._f, self._g
+
+    def evaluate(self, x):
+        return self._eval(x)[0]
+
+    def evaluate_grad(self, x, res):
+        g = self._eval(x)[1]
+        res[:] = g
+        return res
+
+    def calculate(self, params):
+        return self.evaluate(params)
+
+    def calculate_deriv(self, params, i):
+        if not 0 <= i < self.map.nparams:
+            raise IndexError("invalid parameter index %d" % i)
+        return self._eval(params)[1][i]

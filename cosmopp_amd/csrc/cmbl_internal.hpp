@@ -1,0 +1,163 @@
+// Internal definitions shared by the HIP kernels (cmbl_kernels.hip) and the
+// host orchestration (cmbl_map.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "cmbl.h"
+
+namespace cmbl {
+
+// ------------------------------------------------------------------ layout
+//
+// a_lm tables are m-major: entry (l, m) lives at moff(L, m) + (l - m).
+// Ring pairs: pair p = 0 .. npair-1 holds north ring p and its mirror south
+// ring nring-1-p (the equator, p = 2*nside-1, has no mirror).  Pair tables
+// are padded to npair_pad (a multiple of 64 * max K) with zero seeds.
+// Stage buffers, four doubles per (pair, m), pair-major:
+//   P = (E, O): the even / odd parity sums of the synthesis (zeroed once at
+//       set-up; entries of (m, ring block) items that can never be live are
+//       never written and stay zero);
+//   Q = (PsiE, PsiO) = (Psi_N + Psi_S, Psi_N - Psi_S): ring transforms of the
+//       weighted residual, input of the adjoint Legendre stage.
+
+__host__ __device__ inline long moff(int L, int m) { return (long)m * (L + 1) - (long)m * (m - 1) / 2; }
+
+// Legendre scaling: true value = v * 2^(-1000 k).  A lane whose scale count k
+// is > 0 has |true| <= 2^-100 and contributes nothing; it is rescaled when
+// |v| exceeds 2^900.
+constexpr double kScaleHi = 0x1p+900;
+constexpr double kScaleDn = 0x1p-1000;
+constexpr double kScaleUp = 0x1p+1000;
+constexpr double kScaleLo = 0x1p-100;
+
+constexpr int kWave = 64;
+constexpr int kRingThreads = 256;
+constexpr int kMaxFFT = 8192;          // largest LDS FFT (complex doubles)
+constexpr int kRedChunk = 8;           // l-steps per cross-lane reduction in the adjoint
+constexpr int kRedStride = 17;         // doubles per lane row in the reduction buffer
+
+struct MapDesc {
+    int nside, L, npair, npair_pad, nring;
+    int nrb5;                 // ring blocks of the adjoint Legendre stage
+    long npix, nalm;
+    // ring-pair tables [npair_pad]
+    const double* ring_x;     // cos(theta) of the north ring
+    const int* ring_nph;      // pixels per ring
+    const int* ring_shift;    // 1 if phi0 = pi/nph, 0 if phi0 = 0
+    const long* ring_ofsN;    // first pixel of the north ring
+    const long* ring_ofsS;    // first pixel of the south ring, -1 for the equator
+    const int* ring_M;        // FFT length used (nph if a power of two, else Bluestein length)
+    const long* ring_bsofs;   // offset of the Bluestein filter in bs_filt (-1 if none)
+    const double2* bs_filt;   // FFT_M of the conjugate chirp, per non-power-of-two ring
+    // Legendre tables
+    const double* seed_v;     // [(L+1) * npair_pad] scaled lambda_mm
+    const int* seed_k;        // [(L+1) * npair_pad] scale count
+    const double* A;          // [nalm] recurrence coefficient of mu_l = A x mu_{l-1} - mu_{l-2}
+    const double* Nrm;        // [nalm] lambda_lm = Nrm * mu_lm
+    const double* beam;       // [L+1]
+    const double* invcl;      // [L+1] 1/C_l or 0
+    // data
+    const double* d;          // [npix]
+    const double* w;          // [npix]
+    // per-evaluation workspace
+    double2* alm;             // [nalm] B_l N_lm a_lm, m-major
+    double* P;                // [npair * (L+1) * 4] synthesis parity sums
+    double* Q;                // [npair * (L+1) * 4] adjoint ring transforms
+    double* upart;            // [nrb5 * nalm * 2]
+    double* chi;              // [npair] per-pair chi^2
+    double* prior;            // [nprior_blocks]
+    int nprior;
+};
+
+struct WorkItem {
+    int map;
+    int m0;
+    int mcnt;
+    int rb;
+};
+
+struct RingItem {
+    int map;
+    int pair;
+};
+
+enum RingMode { kRingLike = 0, kRingSyn = 1, kRingAna = 2 };
+
+// Thread-local error string.
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define CMBL_HIP(call)                                  \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) return ::cmbl::hip_fail(e_, #call); \
+    } while (0)
+
+// ------------------------------------------------------------------ launchers
+// (defined in cmbl_kernels.hip)
+int launch_pre(const MapDesc& md, const MapDesc* d_md, const double* d_x, hipStream_t s);
+int launch_synth(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, hipStream_t s);
+int launch_adj(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, hipStream_t s);
+int launch_ring(int mode, const RingItem* items, int nitems, const MapDesc* d_maps, int lds_bytes,
+                double* d_map_out, const double* d_map_in, const double2* d_tw, hipStream_t s);
+int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, double* d_g, int like,
+                hipStream_t s);
+int launch_final(const MapDesc* d_md, double* d_f, hipStream_t s);
+int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_bs, double2* d_out, int lds_bytes,
+                           const double2* d_tw, hipStream_t s);
+int launch_scan(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, int* d_lstart,
+                hipStream_t s);
+
+// BLAS-1
+int launch_vec(int op, double* y, const double* x, double c, long n, hipStream_t s);
+int launch_dot(const double* x, const double* y, long n, double* d_out, double* d_work, hipStream_t s);
+
+}  // namespace cmbl
+
+// Opaque handles of the C ABI.
+struct cmbl_map {
+    int nside = 0, L = 0, npair = 0, npair_pad = 0, nring = 0;
+    int K1 = 1, K5 = 1, nrb1 = 0, nrb5 = 0;
+    long npix = 0, nalm = 0, nparam = 0;
+    int device = 0;
+    int lds_ring = 0;
+    bool has_beam = false, has_cl = false, has_data = false;
+    std::vector<void*> allocs;
+    cmbl::MapDesc desc{};
+    cmbl::MapDesc* d_desc = nullptr;
+    cmbl::WorkItem* d_items1 = nullptr;
+    cmbl::WorkItem* d_items5 = nullptr;
+    cmbl::RingItem* d_ring_items = nullptr;
+    int nitems1 = 0, nitems5 = 0, nring_items = 0;
+    std::vector<cmbl::WorkItem> items1, items5;
+    std::vector<cmbl::RingItem> ring_items;
+    double2* d_tw = nullptr;
+    double* d_f = nullptr;
+    double* d_x = nullptr;  // staging for the host entry points
+    double* d_g = nullptr;
+    double* d_map = nullptr;
+    // mutable views of the workspace / data owned by this map
+    double* d_beam = nullptr;
+    double* d_invcl = nullptr;
+    double* d_data = nullptr;
+    double* d_w = nullptr;
+};
+
+struct cmbl_batch {
+    std::vector<cmbl_map*> maps;
+    cmbl::MapDesc* d_descs = nullptr;
+    // work lists grouped by K (index = log2 K)
+    cmbl::WorkItem* d_items1[4] = {nullptr, nullptr, nullptr, nullptr};
+    cmbl::WorkItem* d_items5[4] = {nullptr, nullptr, nullptr, nullptr};
+    int n1[4] = {0, 0, 0, 0}, n5[4] = {0, 0, 0, 0};
+    cmbl::RingItem* d_ring_items = nullptr;
+    int nring_items = 0;
+    int lds_ring = 0;
+    double2* d_tw = nullptr;
+    std::vector<void*> allocs;
+};

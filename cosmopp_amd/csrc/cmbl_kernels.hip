@@ -1,0 +1,1041 @@
+// HIP kernels of the map-level likelihood for gfx950 (CDNA4, wave64, FP64).
+//
+// Stages of one likelihood + gradient evaluation (all FP64):
+//   k_pre     packed a_lm (almToVector layout, source/cmb_gibbs.cpp:320-357)
+//             -> m-major B_l N_lm a_lm, and the prior partial sums
+//   k_synth   Legendre synthesis (the a_lm -> ring Fourier part of HEALPix
+//             alm2map): per (m, ring pair) parity sums E_m, O_m
+//   k_ring    per ring pair: phase fold + inverse ring DFT -> map s,
+//             r = d - s, w r, chi^2 partial (source/cmb_gibbs.cpp:381-387),
+//             forward ring DFT + unfold -> Psi_m (map2alm's ring part with
+//             ring weight Npix/4pi, source/cmb_gibbs.cpp:390-391)
+//   k_adj     adjoint Legendre: u_lm = sum_rings lambda_lm Psi_m
+//   k_post    gradient assembly into the packed layout
+//   k_final   deterministic -2lnL reduction
+#include "cmbl_internal.hpp"
+
+namespace cmbl {
+
+// ================================================================== helpers
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b)  // a * conj(b)
+{
+    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// Wave-uniform read-only tables are read through the constant address space
+// so that the compiler emits scalar loads (s_load) and feeds SGPR operands.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T* p)
+{
+    return (cptr<T>)p;
+}
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// e^{i pi q / n} for integer q (argument reduced exactly modulo 2n)
+__device__ __forceinline__ double2 cis_pi_frac(long q, long n)
+{
+    const long two_n = 2 * n;
+    long r = q % two_n;
+    if (r < 0) r += two_n;
+    double s, c;
+    sincospi((double)r / (double)n, &s, &c);
+    return make_double2(c, s);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Block-wide sum for blockDim.x == nthreads (multiple of 64); result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < NT / 64; ++i) t += sh[i];
+    }
+    __syncthreads();
+    return t;
+}
+
+// ================================================================== k_pre
+// grid: (ceil((L+1)/256), L+1), block 256.  m = blockIdx.y, l = m + j.
+__global__ __launch_bounds__(256) void k_pre(const MapDesc* __restrict__ md_, const double* __restrict__ x)
+{
+    __shared__ double sh[4];
+    const MapDesc& md = *md_;
+    const int L = md.L;
+    const int m = blockIdx.y;
+    const int l = m + blockIdx.x * 256 + threadIdx.x;
+    double pr = 0.0;
+    if (l <= L) {
+        const long li = (long)l * l;
+        const double re = (m == 0) ? x[li] : x[li + 2 * m - 1];
+        const double im = (m == 0) ? 0.0 : x[li + 2 * m];
+        const long k = moff(L, m) + (l - m);
+        const double s = md.beam[l] * md.Nrm[k];
+        md.alm[k] = make_double2(s * re, s * im);
+        pr = md.invcl[l] * ((m == 0) ? re * re : 2.0 * (re * re + im * im));
+    }
+    const double t = block_sum<256>(pr, sh);
+    if (threadIdx.x == 0) md.prior[blockIdx.y * gridDim.x + blockIdx.x] = t;
+}
+
+// ================================================================== Legendre stages
+//
+// One wavefront per work item (map, m0..m0+mcnt-1, ring block rb).  Lane
+// `lane` handles ring pairs p_k = rb*64*K + k*64 + lane, k < K.  The
+// recurrence mu_l = A_lm x mu_{l-1} - mu_{l-2} runs in registers; A_lm and
+// the a_lm coefficients are wave-uniform and come through scalar loads.
+// Lanes whose scale count is > 0 are negligible (|lambda| < 2^-100): the
+// wave runs a "scaled" loop with rescale checks until every lane is live,
+// then a check-free loop.
+
+template <int K>
+__device__ __forceinline__ bool all_live(const int (&sc)[K])
+{
+    int ok = 1;
+#pragma unroll
+    for (int k = 0; k < K; ++k) ok &= (sc[k] == 0);
+    return __all(ok);
+}
+template <int K>
+__device__ __forceinline__ bool any_live(const int (&sc)[K])
+{
+    int ok = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) ok |= (sc[k] == 0);
+    return __any(ok);
+}
+
+template <int K>
+__device__ __forceinline__ bool rescale(double (&mu1)[K], double (&mu2)[K], int (&sc)[K])
+{
+    int need = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) need |= (sc[k] > 0) & (fabs(mu1[k]) > kScaleHi);
+    if (!__any(need)) return false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (sc[k] > 0 && fabs(mu1[k]) > kScaleHi) {
+            mu1[k] *= kScaleDn;
+            mu2[k] *= kScaleDn;
+            sc[k] -= 1;
+        }
+    }
+    return true;
+}
+
+// -------------------------------------------------------------- synthesis
+template <int K, bool ODD>
+__device__ __forceinline__ void syn_step(double Al, double ar, double ai, const double (&x)[K], double (&mu1)[K],
+                                         double (&mu2)[K], double (&er)[K], double (&ei)[K],
+                                         double (&orr)[K], double (&oi)[K])
+{
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double nu = fma(Al * x[k], mu1[k], -mu2[k]);
+        mu2[k] = mu1[k];
+        mu1[k] = nu;
+        if (ODD) {
+            orr[k] = fma(nu, ar, orr[k]);
+            oi[k] = fma(nu, ai, oi[k]);
+        } else {
+            er[k] = fma(nu, ar, er[k]);
+            ei[k] = fma(nu, ai, ei[k]);
+        }
+    }
+}
+
+// One step of the scaled region: recurrence for every lane, accumulation of
+// the live lanes only (gate = 1 for live lanes, 0 otherwise).
+template <int K, bool ODD>
+__device__ __forceinline__ void syn_scaled_step(double Al, double ar, double ai, bool live, const double (&gate)[K],
+                                                const double (&x)[K], double (&mu1)[K], double (&mu2)[K],
+                                                double (&er)[K], double (&ei)[K], double (&orr)[K], double (&oi)[K])
+{
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double nu = fma(Al * x[k], mu1[k], -mu2[k]);
+        mu2[k] = mu1[k];
+        mu1[k] = nu;
+    }
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const double f = gate[k] * mu1[k];
+            if (ODD) {
+                orr[k] = fma(f, ar, orr[k]);
+                oi[k] = fma(f, ai, oi[k]);
+            } else {
+                er[k] = fma(f, ar, er[k]);
+                ei[k] = fma(f, ai, ei[k]);
+            }
+        }
+    }
+}
+
+// Four live steps l+1 .. l+4 with parity pattern starting at ODD0, with the
+// coefficients of the next four steps prefetched into SGPRs.
+template <int K, bool ODD0>
+__device__ __forceinline__ void syn_run(int& l, int L, cptr<double> A, cptr<double> a, const double (&x)[K],
+                                        double (&mu1)[K], double (&mu2)[K], double (&er)[K], double (&ei)[K],
+                                        double (&orr)[K], double (&oi)[K])
+{
+    if (l + 4 <= L) {
+        double c0 = A[l + 1], c1 = A[l + 2], c2 = A[l + 3], c3 = A[l + 4];
+        double r0 = a[2 * (l + 1)], i0 = a[2 * (l + 1) + 1], r1 = a[2 * (l + 2)], i1 = a[2 * (l + 2) + 1];
+        double r2 = a[2 * (l + 3)], i2 = a[2 * (l + 3) + 1], r3 = a[2 * (l + 4)], i3 = a[2 * (l + 4) + 1];
+        for (; l + 8 <= L; l += 4) {
+            const double n0 = A[l + 5], n1 = A[l + 6], n2 = A[l + 7], n3 = A[l + 8];
+            const double s0 = a[2 * (l + 5)], t0 = a[2 * (l + 5) + 1], s1 = a[2 * (l + 6)], t1 = a[2 * (l + 6) + 1];
+            const double s2 = a[2 * (l + 7)], t2 = a[2 * (l + 7) + 1], s3 = a[2 * (l + 8)], t3 = a[2 * (l + 8) + 1];
+            syn_step<K, ODD0>(c0, r0, i0, x, mu1, mu2, er, ei, orr, oi);
+            syn_step<K, !ODD0>(c1, r1, i1, x, mu1, mu2, er, ei, orr, oi);
+            syn_step<K, ODD0>(c2, r2, i2, x, mu1, mu2, er, ei, orr, oi);
+            syn_step<K, !ODD0>(c3, r3, i3, x, mu1, mu2, er, ei, orr, oi);
+            c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+            r0 = s0; i0 = t0; r1 = s1; i1 = t1; r2 = s2; i2 = t2; r3 = s3; i3 = t3;
+        }
+        syn_step<K, ODD0>(c0, r0, i0, x, mu1, mu2, er, ei, orr, oi);
+        syn_step<K, !ODD0>(c1, r1, i1, x, mu1, mu2, er, ei, orr, oi);
+        syn_step<K, ODD0>(c2, r2, i2, x, mu1, mu2, er, ei, orr, oi);
+        syn_step<K, !ODD0>(c3, r3, i3, x, mu1, mu2, er, ei, orr, oi);
+        l += 4;
+    }
+    // tail (< 4 steps); the parity of step l+1 is ODD0 again since 4 is even
+    bool odd = ODD0;
+    while (l < L) {
+        ++l;
+        if (odd)
+            syn_step<K, true>(A[l], a[2 * l], a[2 * l + 1], x, mu1, mu2, er, ei, orr, oi);
+        else
+            syn_step<K, false>(A[l], a[2 * l], a[2 * l + 1], x, mu1, mu2, er, ei, orr, oi);
+        odd = !odd;
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
+{
+    const WorkItem it = items[blockIdx.x];
+    const MapDesc& md = maps[it.map];
+    const int L = md.L;
+    const int npad = md.npair_pad;
+    const int lane = threadIdx.x;
+    double x[K];
+    int p[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        p[k] = it.rb * 64 * K + k * 64 + lane;
+        x[k] = md.ring_x[p[k]];
+    }
+    for (int mi = 0; mi < it.mcnt; ++mi) {
+        const int m = it.m0 + mi;
+        const long base = moff(L, m) - m;
+        cptr<double> A = as_const(md.A + base);
+        cptr<double> a = as_const(reinterpret_cast<const double*>(md.alm + base));
+        double mu1[K], mu2[K], er[K], ei[K], orr[K], oi[K];
+        int sc[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            mu1[k] = md.seed_v[(long)m * npad + p[k]];
+            sc[k] = md.seed_k[(long)m * npad + p[k]];
+            mu2[k] = 0.0;
+            er[k] = ei[k] = orr[k] = oi[k] = 0.0;
+        }
+        bool live = any_live<K>(sc);
+        bool all = all_live<K>(sc);
+        double gate[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) gate[k] = sc[k] == 0 ? 1.0 : 0.0;
+        if (live) {
+            const double ar = a[2 * m], ai = a[2 * m + 1];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const double f = gate[k] * mu1[k];
+                er[k] = fma(f, ar, er[k]);
+                ei[k] = fma(f, ai, ei[k]);
+            }
+        }
+        int l = m;
+        // scaled region: some lanes still below 2^-100; steps alternate odd, even
+        while (!all && l < L) {
+            ++l;
+            syn_scaled_step<K, true>(A[l], a[2 * l], a[2 * l + 1], live, gate, x, mu1, mu2, er, ei, orr, oi);
+            if (rescale<K>(mu1, mu2, sc)) {
+                live = any_live<K>(sc);
+                all = all_live<K>(sc);
+#pragma unroll
+                for (int k = 0; k < K; ++k) gate[k] = sc[k] == 0 ? 1.0 : 0.0;
+            }
+            if (all || l >= L) break;
+            ++l;
+            syn_scaled_step<K, false>(A[l], a[2 * l], a[2 * l + 1], live, gate, x, mu1, mu2, er, ei, orr, oi);
+            if (rescale<K>(mu1, mu2, sc)) {
+                live = any_live<K>(sc);
+                all = all_live<K>(sc);
+#pragma unroll
+                for (int k = 0; k < K; ++k) gate[k] = sc[k] == 0 ? 1.0 : 0.0;
+            }
+        }
+        // live region: no checks
+        if (((l - m) & 1) == 0)
+            syn_run<K, true>(l, L, A, a, x, mu1, mu2, er, ei, orr, oi);
+        else
+            syn_run<K, false>(l, L, A, a, x, mu1, mu2, er, ei, orr, oi);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (p[k] < md.npair) {
+                double2* dst = reinterpret_cast<double2*>(md.P + ((long)p[k] * (L + 1) + m) * 4);
+                dst[0] = make_double2(er[k], ei[k]);
+                dst[1] = make_double2(orr[k], oi[k]);
+            }
+        }
+    }
+}
+
+// -------------------------------------------------------------- adjoint
+// Per l-step each lane forms sum_k mu_l(p_k) Psi^{parity}(p_k) and parks it
+// in LDS; every kRedChunk steps the wave reduces the 64 lanes and writes the
+// ring-block partial u_lm (re, im) with one coalesced store.
+
+__device__ __forceinline__ void adj_flush(double* red, int lane, int cnt, double* dst)
+{
+    __syncthreads();
+    const int c = lane & 15;
+    const int q = lane >> 4;
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s += red[(16 * q + i) * kRedStride + c];
+    s += __shfl_xor(s, 16);
+    s += __shfl_xor(s, 32);
+    if (lane < 2 * cnt) dst[lane] = s;
+    __syncthreads();
+}
+
+template <int K, bool ODD>
+__device__ __forceinline__ void adj_step(double Al, const double (&x)[K], double (&mu1)[K], double (&mu2)[K],
+                                         const double (&per)[K], const double (&pei)[K],
+                                         const double (&por)[K], const double (&poi)[K], double& sr,
+                                         double& si)
+{
+    sr = 0.0;
+    si = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double nu = fma(Al * x[k], mu1[k], -mu2[k]);
+        mu2[k] = mu1[k];
+        mu1[k] = nu;
+        if (ODD) {
+            sr = fma(nu, por[k], sr);
+            si = fma(nu, poi[k], si);
+        } else {
+            sr = fma(nu, per[k], sr);
+            si = fma(nu, pei[k], si);
+        }
+    }
+}
+
+template <int K, bool ODD>
+__device__ __forceinline__ void adj_scaled_step(double Al, bool live, const double (&gate)[K], const double (&x)[K],
+                                                double (&mu1)[K], double (&mu2)[K], const double (&per)[K],
+                                                const double (&pei)[K], const double (&por)[K],
+                                                const double (&poi)[K], double* out)
+{
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double nu = fma(Al * x[k], mu1[k], -mu2[k]);
+        mu2[k] = mu1[k];
+        mu1[k] = nu;
+    }
+    double sr = 0.0, si = 0.0;
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const double f = gate[k] * mu1[k];
+            sr = fma(f, ODD ? por[k] : per[k], sr);
+            si = fma(f, ODD ? poi[k] : pei[k], si);
+        }
+    }
+    out[0] = sr;
+    out[1] = si;
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void k_adj(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
+{
+    __shared__ double red[64 * kRedStride];
+    const WorkItem it = items[blockIdx.x];
+    const MapDesc& md = maps[it.map];
+    const int L = md.L;
+    const int npad = md.npair_pad;
+    const int lane = threadIdx.x;
+    double x[K];
+    int p[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        p[k] = it.rb * 64 * K + k * 64 + lane;
+        x[k] = md.ring_x[p[k]];
+    }
+    double* urow = md.upart + 2 * (long)it.rb * md.nalm;
+    double* myred = red + lane * kRedStride;
+    for (int mi = 0; mi < it.mcnt; ++mi) {
+        const int m = it.m0 + mi;
+        const long base = moff(L, m) - m;
+        cptr<double> A = as_const(md.A + base);
+        double* dst = urow + 2 * base;  // dst[2l], dst[2l+1]
+        double mu1[K], mu2[K], per[K], pei[K], por[K], poi[K];
+        int sc[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            mu1[k] = md.seed_v[(long)m * npad + p[k]];
+            sc[k] = md.seed_k[(long)m * npad + p[k]];
+            mu2[k] = 0.0;
+            if (p[k] < md.npair) {
+                const double2* src = reinterpret_cast<const double2*>(md.Q + ((long)p[k] * (L + 1) + m) * 4);
+                const double2 e = src[0], o = src[1];
+                per[k] = e.x;
+                pei[k] = e.y;
+                por[k] = o.x;
+                poi[k] = o.y;
+            } else {
+                per[k] = pei[k] = por[k] = poi[k] = 0.0;
+            }
+        }
+        bool live = any_live<K>(sc);
+        bool all = all_live<K>(sc);
+        double gate[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) gate[k] = sc[k] == 0 ? 1.0 : 0.0;
+        int cnt = 0;
+        int lbase = m;
+        {
+            double sr = 0.0, si = 0.0;
+            if (live) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const double f = gate[k] * mu1[k];
+                    sr = fma(f, per[k], sr);
+                    si = fma(f, pei[k], si);
+                }
+            }
+            myred[0] = sr;
+            myred[1] = si;
+            cnt = 1;
+            if (m == L) {
+                adj_flush(red, lane, cnt, dst + 2 * lbase);
+                lbase += cnt;
+                cnt = 0;
+            }
+        }
+        int l = m;
+        // scaled region: steps alternate odd, even
+        while (!all && l < L) {
+            ++l;
+            adj_scaled_step<K, true>(A[l], live, gate, x, mu1, mu2, per, pei, por, poi, myred + 2 * cnt);
+            ++cnt;
+            if (cnt == kRedChunk || l == L) {
+                adj_flush(red, lane, cnt, dst + 2 * lbase);
+                lbase += cnt;
+                cnt = 0;
+            }
+            if (rescale<K>(mu1, mu2, sc)) {
+                live = any_live<K>(sc);
+                all = all_live<K>(sc);
+#pragma unroll
+                for (int k = 0; k < K; ++k) gate[k] = sc[k] == 0 ? 1.0 : 0.0;
+            }
+            if (all || l >= L) break;
+            ++l;
+            adj_scaled_step<K, false>(A[l], live, gate, x, mu1, mu2, per, pei, por, poi, myred + 2 * cnt);
+            ++cnt;
+            if (cnt == kRedChunk || l == L) {
+                adj_flush(red, lane, cnt, dst + 2 * lbase);
+                lbase += cnt;
+                cnt = 0;
+            }
+            if (rescale<K>(mu1, mu2, sc)) {
+                live = any_live<K>(sc);
+                all = all_live<K>(sc);
+#pragma unroll
+                for (int k = 0; k < K; ++k) gate[k] = sc[k] == 0 ? 1.0 : 0.0;
+            }
+        }
+        // live region: two steps (one of each parity) per iteration
+        bool odd_next = ((l - m) & 1) == 0;
+        if (l < L && !odd_next) {
+            // align so that each iteration starts with an odd step
+            ++l;
+            double sr, si;
+            adj_step<K, false>(A[l], x, mu1, mu2, per, pei, por, poi, sr, si);
+            myred[2 * cnt] = sr;
+            myred[2 * cnt + 1] = si;
+            ++cnt;
+            if (cnt == kRedChunk || l == L) {
+                adj_flush(red, lane, cnt, dst + 2 * lbase);
+                lbase += cnt;
+                cnt = 0;
+            }
+        }
+        if (l + 2 <= L) {
+            double c0 = A[l + 1], c1 = A[l + 2];
+            for (; l + 2 <= L; l += 2) {
+                double n0 = 0.0, n1 = 0.0;
+                if (l + 4 <= L) {
+                    n0 = A[l + 3];
+                    n1 = A[l + 4];
+                }
+                if (cnt > kRedChunk - 2) {  // keep room for two entries in the LDS row
+                    adj_flush(red, lane, cnt, dst + 2 * lbase);
+                    lbase += cnt;
+                    cnt = 0;
+                }
+                double sr, si;
+                adj_step<K, true>(c0, x, mu1, mu2, per, pei, por, poi, sr, si);
+                myred[2 * cnt] = sr;
+                myred[2 * cnt + 1] = si;
+                adj_step<K, false>(c1, x, mu1, mu2, per, pei, por, poi, sr, si);
+                myred[2 * cnt + 2] = sr;
+                myred[2 * cnt + 3] = si;
+                cnt += 2;
+                if (cnt == kRedChunk || l + 2 == L) {
+                    adj_flush(red, lane, cnt, dst + 2 * lbase);
+                    lbase += cnt;
+                    cnt = 0;
+                }
+                c0 = n0;
+                c1 = n1;
+            }
+        }
+        if (l < L) {
+            ++l;
+            double sr, si;
+            adj_step<K, true>(A[l], x, mu1, mu2, per, pei, por, poi, sr, si);
+            myred[2 * cnt] = sr;
+            myred[2 * cnt + 1] = si;
+            ++cnt;
+            adj_flush(red, lane, cnt, dst + 2 * lbase);
+            lbase += cnt;
+            cnt = 0;
+        }
+    }
+}
+
+// -------------------------------------------------------------- scan (set-up)
+// Runs the scaled recurrence once per (m, pair) and records the first l at
+// which the pair becomes live (L+1 if never).  Used to drop dead work items.
+template <int K>
+__global__ __launch_bounds__(64) void k_scan(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps,
+                                             int* __restrict__ lstart)
+{
+    const WorkItem it = items[blockIdx.x];
+    const MapDesc& md = maps[it.map];
+    const int L = md.L;
+    const int npad = md.npair_pad;
+    const int lane = threadIdx.x;
+    double x[K];
+    int p[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        p[k] = it.rb * 64 * K + k * 64 + lane;
+        x[k] = md.ring_x[p[k]];
+    }
+    for (int mi = 0; mi < it.mcnt; ++mi) {
+        const int m = it.m0 + mi;
+        const double* __restrict__ A = md.A + moff(L, m) - m;
+        double mu1[K], mu2[K];
+        int sc[K], ls[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            mu1[k] = md.seed_v[(long)m * npad + p[k]];
+            sc[k] = md.seed_k[(long)m * npad + p[k]];
+            mu2[k] = 0.0;
+            ls[k] = sc[k] == 0 ? m : L + 1;
+        }
+        bool all = all_live<K>(sc);
+        int l = m;
+        while (!all && l < L) {
+            ++l;
+            const double Al = A[l];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const double nu = fma(Al * x[k], mu1[k], -mu2[k]);
+                mu2[k] = mu1[k];
+                mu1[k] = nu;
+            }
+            if (rescale<K>(mu1, mu2, sc)) {
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    if (sc[k] == 0 && ls[k] > L) ls[k] = l + 1;
+                all = all_live<K>(sc);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) lstart[(long)m * npad + p[k]] = ls[k];
+    }
+}
+
+// ================================================================== ring stage
+
+// In-place power-of-two FFT in LDS, radix-2 DIT with bit reversal.
+// tw[k] = e^{-2 pi i k / kMaxFFT}, k < kMaxFFT/2.  sign = -1 forward, +1 inverse.
+__device__ void lds_fft(double2* z, int M, int sign, const double2* __restrict__ tw)
+{
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    int logM = 0;
+    while ((1 << logM) < M) ++logM;
+    if (logM == 0) return;
+    for (int i = tid; i < M; i += nt) {
+        const int j = (int)(__brev((unsigned)i) >> (32 - logM));
+        if (i < j) {
+            const double2 t = z[i];
+            z[i] = z[j];
+            z[j] = t;
+        }
+    }
+    __syncthreads();
+    for (int s = 1; s <= logM; ++s) {
+        const int half = 1 << (s - 1);
+        const int tstride = kMaxFFT >> s;
+        for (int b = tid; b < (M >> 1); b += nt) {
+            const int k = b & (half - 1);
+            const int i0 = ((b >> (s - 1)) << s) + k;
+            const int i1 = i0 + half;
+            double2 w = tw[k * tstride];
+            if (sign > 0) w.y = -w.y;
+            const double2 u = z[i0];
+            const double2 v = cmul(z[i1], w);
+            z[i0] = cadd(u, v);
+            z[i1] = csub(u, v);
+        }
+        __syncthreads();
+    }
+}
+
+// DFT of length n held in z[0..n-1] (capacity M), result in z[0..n-1].
+// Power-of-two n: direct.  Otherwise Bluestein with the pair's filter
+// bf = FFT_M(e^{+i pi j^2/n}) (wrapped), which serves both signs.
+__device__ void ring_dft(double2* z, int n, int M, int sign, const double2* __restrict__ bf,
+                         const double2* __restrict__ tw)
+{
+    if (M == n) {
+        lds_fft(z, n, sign, tw);
+        return;
+    }
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    // a_j = z_j c_j, c_j = e^{sign i pi j^2 / n}
+    for (int j = tid; j < M; j += nt) {
+        if (j < n) {
+            double2 c = cis_pi_frac(((long)j * j) % (2L * n), n);
+            if (sign < 0) c.y = -c.y;
+            z[j] = cmul(z[j], c);
+        } else {
+            z[j] = make_double2(0.0, 0.0);
+        }
+    }
+    __syncthreads();
+    lds_fft(z, M, -1, tw);
+    for (int k = tid; k < M; k += nt) {
+        double2 f;
+        if (sign < 0) {
+            f = bf[k];
+        } else {
+            const double2 g = bf[(M - k) & (M - 1)];
+            f = make_double2(g.x, -g.y);
+        }
+        z[k] = cmul(z[k], f);
+    }
+    __syncthreads();
+    lds_fft(z, M, +1, tw);
+    const double inv = 1.0 / (double)M;
+    for (int k = tid; k < n; k += nt) {
+        double2 c = cis_pi_frac(((long)k * k) % (2L * n), n);
+        if (sign < 0) c.y = -c.y;
+        const double2 v = z[k];
+        z[k] = cmul(make_double2(v.x * inv, v.y * inv), c);
+    }
+    __syncthreads();
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restrict__ items,
+                                                       const MapDesc* __restrict__ maps,
+                                                       double* __restrict__ map_out,
+                                                       const double* __restrict__ map_in,
+                                                       const double2* __restrict__ tw)
+{
+    extern __shared__ double2 Z[];
+    __shared__ double sh[kRingThreads / 64];
+    const RingItem it = items[blockIdx.x];
+    const MapDesc& md = maps[it.map];
+    const int L = md.L;
+    const int p = it.pair;
+    const int n = md.ring_nph[p];
+    const int M = md.ring_M[p];
+    const bool shifted = md.ring_shift[p] != 0;
+    const long oN = md.ring_ofsN[p];
+    const long oS = md.ring_ofsS[p];
+    const double2* bf = md.ring_bsofs[p] >= 0 ? md.bs_filt + md.ring_bsofs[p] : nullptr;
+    const double* row = md.P + (long)p * (L + 1) * 4;
+    double* qrow = md.Q + (long)p * (L + 1) * 4;
+    const int tid = threadIdx.x;
+
+    if (MODE != kRingAna) {
+        // fold: Z_k = G_N,k + i G_S,k
+        for (int k = tid; k < n; k += kRingThreads) {
+            double2 gN = make_double2(0.0, 0.0), gS = make_double2(0.0, 0.0);
+            for (int mm = k; mm <= L; mm += n) {
+                const double2* src = reinterpret_cast<const double2*>(row + 4L * mm);
+                const double2 e = src[0], o = src[1];
+                double2 fN = cadd(e, o), fS = csub(e, o);
+                if (shifted) {
+                    const double2 ph = cis_pi_frac(mm, n);
+                    fN = cmul(fN, ph);
+                    fS = cmul(fS, ph);
+                }
+                gN = cadd(gN, fN);
+                gS = cadd(gS, fS);
+            }
+            for (int mm = (k == 0 ? n : n - k); mm <= L; mm += n) {
+                const double2* src = reinterpret_cast<const double2*>(row + 4L * mm);
+                const double2 e = src[0], o = src[1];
+                double2 fN = cadd(e, o), fS = csub(e, o);
+                if (shifted) {
+                    const double2 ph = cis_pi_frac(mm, n);
+                    fN = cmul(fN, ph);
+                    fS = cmul(fS, ph);
+                }
+                gN = make_double2(gN.x + fN.x, gN.y - fN.y);
+                gS = make_double2(gS.x + fS.x, gS.y - fS.y);
+            }
+            if (oS < 0) gS = make_double2(0.0, 0.0);
+            Z[k] = make_double2(gN.x - gS.y, gN.y + gS.x);
+        }
+        __syncthreads();
+        ring_dft(Z, n, M, +1, bf, tw);
+    }
+
+    if (MODE == kRingSyn) {
+        for (int j = tid; j < n; j += kRingThreads) {
+            const double2 v = Z[j];
+            map_out[oN + j] = v.x;
+            if (oS >= 0) map_out[oS + j] = v.y;
+        }
+        return;
+    }
+
+    if (MODE == kRingLike) {
+        double chi = 0.0;
+        for (int j = tid; j < n; j += kRingThreads) {
+            const double2 v = Z[j];
+            const double rN = md.d[oN + j] - v.x;
+            const double fN = md.w[oN + j] * rN;
+            chi = fma(fN, rN, chi);
+            double fS = 0.0;
+            if (oS >= 0) {
+                const double rS = md.d[oS + j] - v.y;
+                fS = md.w[oS + j] * rS;
+                chi = fma(fS, rS, chi);
+            }
+            Z[j] = make_double2(fN, fS);
+        }
+        const double t = block_sum<kRingThreads>(chi, sh);
+        if (tid == 0) md.chi[p] = t;
+    } else {  // kRingAna: load the map to analyse
+        for (int j = tid; j < n; j += kRingThreads)
+            Z[j] = make_double2(map_in[oN + j], oS >= 0 ? map_in[oS + j] : 0.0);
+    }
+    __syncthreads();
+    ring_dft(Z, n, M, -1, bf, tw);
+
+    // unfold: Psi_X,m = e^{-i m phi0} H_X,(m mod n)
+    for (int mm = tid; mm <= L; mm += kRingThreads) {
+        const int b = mm % n;
+        const int nb = (b == 0) ? 0 : n - b;
+        const double2 y = Z[b];
+        const double2 yc = make_double2(Z[nb].x, -Z[nb].y);
+        double2 hN = make_double2(0.5 * (y.x + yc.x), 0.5 * (y.y + yc.y));
+        double2 hS = make_double2(0.5 * (y.y - yc.y), -0.5 * (y.x - yc.x));
+        if (shifted) {
+            double2 ph = cis_pi_frac(mm, n);
+            ph.y = -ph.y;
+            hN = cmul(hN, ph);
+            hS = cmul(hS, ph);
+        }
+        double2* dst = reinterpret_cast<double2*>(qrow + 4L * mm);
+        dst[0] = cadd(hN, hS);
+        dst[1] = csub(hN, hS);
+    }
+}
+
+// Bluestein filter: bf = FFT_M(b), b_j = e^{+i pi j^2/n} for |j| < n, wrapped mod M.
+__global__ __launch_bounds__(kRingThreads) void k_bluestein_setup(const int* __restrict__ pairs,
+                                                                  const MapDesc* __restrict__ md_,
+                                                                  double2* __restrict__ out,
+                                                                  const double2* __restrict__ tw)
+{
+    extern __shared__ double2 Z[];
+    const MapDesc& md = *md_;
+    const int p = pairs[blockIdx.x];
+    const int n = md.ring_nph[p];
+    const int M = md.ring_M[p];
+    for (int j = threadIdx.x; j < M; j += blockDim.x) {
+        int jj = -1;
+        if (j < n) jj = j;
+        else if (j > M - n) jj = M - j;
+        Z[j] = jj >= 0 ? cis_pi_frac(((long)jj * jj) % (2L * n), n) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    lds_fft(Z, M, -1, tw);
+    double2* dst = out + md.ring_bsofs[p];
+    for (int j = threadIdx.x; j < M; j += blockDim.x) dst[j] = Z[j];
+}
+
+// ================================================================== k_post
+// grid (ceil((L+1)/256), L+1).  u = sum_rb upart (fixed order), lambda = N mu.
+// like = 1: g = -2 J^T (w r) + prior gradient;  like = 0: g = J^T v.
+__global__ __launch_bounds__(256) void k_post(const MapDesc* __restrict__ md_, const double* __restrict__ x,
+                                              double* __restrict__ g, int like)
+{
+    const MapDesc& md = *md_;
+    const int L = md.L;
+    const int m = blockIdx.y;
+    const int l = m + blockIdx.x * 256 + threadIdx.x;
+    if (l > L) return;
+    const long k = moff(L, m) + (l - m);
+    double ur = 0.0, ui = 0.0;
+    for (int rb = 0; rb < md.nrb5; ++rb) {
+        const double2 v = reinterpret_cast<const double2*>(md.upart + 2 * (long)rb * md.nalm)[k];
+        ur += v.x;
+        ui += v.y;
+    }
+    const double s = md.Nrm[k] * md.beam[l];
+    ur *= s;
+    ui *= s;
+    const long li = (long)l * l;
+    if (m == 0) {
+        if (like)
+            g[li] = -2.0 * ur + 2.0 * md.invcl[l] * x[li];
+        else
+            g[li] = ur;
+    } else {
+        if (like) {
+            g[li + 2 * m - 1] = -4.0 * ur + 4.0 * md.invcl[l] * x[li + 2 * m - 1];
+            g[li + 2 * m] = -4.0 * ui + 4.0 * md.invcl[l] * x[li + 2 * m];
+        } else {
+            g[li + 2 * m - 1] = 2.0 * ur;
+            g[li + 2 * m] = 2.0 * ui;
+        }
+    }
+}
+
+// ================================================================== k_final
+// One block: -2lnL = sum_pairs chi + sum_blocks prior, fixed order.
+__global__ __launch_bounds__(1024) void k_final(const MapDesc* __restrict__ md_, double* __restrict__ f)
+{
+    __shared__ double sh[16];
+    const MapDesc& md = *md_;
+    double a = 0.0;
+    for (int i = threadIdx.x; i < md.npair; i += 1024) a += md.chi[i];
+    for (int i = threadIdx.x; i < md.nprior; i += 1024) a += md.prior[i];
+    const double t = block_sum<1024>(a, sh);
+    if (threadIdx.x == 0) *f = t;
+}
+
+// ================================================================== BLAS-1
+enum { kVecCopy = 0, kVecSet, kVecAxpy, kVecMul, kVecDiv, kVecPow };
+
+__global__ void k_vec(int op, double* __restrict__ y, const double* __restrict__ x, double c, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        switch (op) {
+        case kVecCopy: y[i] = c * x[i]; break;
+        case kVecSet: y[i] = c; break;
+        case kVecAxpy: y[i] = fma(c, x[i], y[i]); break;
+        case kVecMul: y[i] *= x[i]; break;
+        case kVecDiv: y[i] /= x[i]; break;
+        case kVecPow: y[i] = pow(y[i], c); break;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dot_part(const double* __restrict__ x, const double* __restrict__ y, long n,
+                                                  double* __restrict__ part)
+{
+    __shared__ double sh[4];
+    double a = 0.0;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) a = fma(x[i], y[i], a);
+    const double t = block_sum<256>(a, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(1024) void k_dot_final(const double* __restrict__ part, int np, double* __restrict__ out)
+{
+    __shared__ double sh[16];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < np; i += 1024) a += part[i];
+    const double t = block_sum<1024>(a, sh);
+    if (threadIdx.x == 0) *out = t;
+}
+
+// ================================================================== launchers
+
+int launch_pre(const MapDesc& md, const MapDesc* d_md, const double* d_x, hipStream_t s)
+{
+    dim3 grid((md.L + 1 + 255) / 256, md.L + 1);
+    hipLaunchKernelGGL(k_pre, grid, dim3(256), 0, s, d_md, d_x);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+template <int K>
+static int synth_k(const WorkItem* items, int n, const MapDesc* d_maps, hipStream_t s)
+{
+    if (n <= 0) return CMBL_OK;
+    hipLaunchKernelGGL(k_synth<K>, dim3(n), dim3(64), 0, s, items, d_maps);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+template <int K>
+static int adj_k(const WorkItem* items, int n, const MapDesc* d_maps, hipStream_t s)
+{
+    if (n <= 0) return CMBL_OK;
+    hipLaunchKernelGGL(k_adj<K>, dim3(n), dim3(64), 0, s, items, d_maps);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+template <int K>
+static int scan_k(const WorkItem* items, int n, const MapDesc* d_maps, int* d_lstart, hipStream_t s)
+{
+    if (n <= 0) return CMBL_OK;
+    hipLaunchKernelGGL(k_scan<K>, dim3(n), dim3(64), 0, s, items, d_maps, d_lstart);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_synth(int K, const WorkItem* items, int n, const MapDesc* d_maps, hipStream_t s)
+{
+    switch (K) {
+    case 1: return synth_k<1>(items, n, d_maps, s);
+    case 2: return synth_k<2>(items, n, d_maps, s);
+    case 4: return synth_k<4>(items, n, d_maps, s);
+    case 8: return synth_k<8>(items, n, d_maps, s);
+    }
+    return fail(CMBL_ERR_ARG, "unsupported rings-per-lane K");
+}
+int launch_adj(int K, const WorkItem* items, int n, const MapDesc* d_maps, hipStream_t s)
+{
+    switch (K) {
+    case 1: return adj_k<1>(items, n, d_maps, s);
+    case 2: return adj_k<2>(items, n, d_maps, s);
+    case 4: return adj_k<4>(items, n, d_maps, s);
+    case 8: return adj_k<8>(items, n, d_maps, s);
+    }
+    return fail(CMBL_ERR_ARG, "unsupported rings-per-lane K");
+}
+int launch_scan(int K, const WorkItem* items, int n, const MapDesc* d_maps, int* d_lstart, hipStream_t s)
+{
+    switch (K) {
+    case 1: return scan_k<1>(items, n, d_maps, d_lstart, s);
+    case 2: return scan_k<2>(items, n, d_maps, d_lstart, s);
+    case 4: return scan_k<4>(items, n, d_maps, d_lstart, s);
+    case 8: return scan_k<8>(items, n, d_maps, d_lstart, s);
+    }
+    return fail(CMBL_ERR_ARG, "unsupported rings-per-lane K");
+}
+
+template <int MODE>
+static int ring_m(const RingItem* items, int n, const MapDesc* d_maps, int lds, double* out, const double* in,
+                  const double2* tw, hipStream_t s)
+{
+    if (n <= 0) return CMBL_OK;
+    static bool attr_set = false;
+    if (!attr_set) {
+        CMBL_HIP(hipFuncSetAttribute((const void*)k_ring<MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     kMaxFFT * (int)sizeof(double2)));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_ring<MODE>, dim3(n), dim3(kRingThreads), lds, s, items, d_maps, out, in, tw);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_ring(int mode, const RingItem* items, int n, const MapDesc* d_maps, int lds, double* out,
+                const double* in, const double2* tw, hipStream_t s)
+{
+    switch (mode) {
+    case kRingLike: return ring_m<kRingLike>(items, n, d_maps, lds, out, in, tw, s);
+    case kRingSyn: return ring_m<kRingSyn>(items, n, d_maps, lds, out, in, tw, s);
+    case kRingAna: return ring_m<kRingAna>(items, n, d_maps, lds, out, in, tw, s);
+    }
+    return fail(CMBL_ERR_ARG, "bad ring mode");
+}
+
+int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_bs, double2* d_out, int lds,
+                           const double2* tw, hipStream_t s)
+{
+    if (npairs_bs <= 0) return CMBL_OK;
+    static bool attr_set = false;
+    if (!attr_set) {
+        CMBL_HIP(hipFuncSetAttribute((const void*)k_bluestein_setup, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     kMaxFFT * (int)sizeof(double2)));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_bluestein_setup, dim3(npairs_bs), dim3(kRingThreads), lds, s, d_pairs, d_md, d_out, tw);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, double* d_g, int like, hipStream_t s)
+{
+    dim3 grid((md.L + 1 + 255) / 256, md.L + 1);
+    hipLaunchKernelGGL(k_post, grid, dim3(256), 0, s, d_md, d_x, d_g, like);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_final(const MapDesc* d_md, double* d_f, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(1024), 0, s, d_md, d_f);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_vec(int op, double* y, const double* x, double c, long n, hipStream_t s)
+{
+    if (n <= 0) return CMBL_OK;
+    long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_vec, dim3((unsigned)blocks), dim3(256), 0, s, op, y, x, c, n);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_dot(const double* x, const double* y, long n, double* d_out, double* d_work, hipStream_t s)
+{
+    const int np = 1024;
+    hipLaunchKernelGGL(k_dot_part, dim3(np), dim3(256), 0, s, x, y, n, d_work);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(1024), 0, s, d_work, np, d_out);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+}  // namespace cmbl

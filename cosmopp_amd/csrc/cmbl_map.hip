@@ -1,0 +1,859 @@
+// Host side of the C ABI (include/cmbl.h): map contexts, set-up tables,
+// evaluation orchestration, batches, inputs and device-vector BLAS-1.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "cmbl_internal.hpp"
+
+namespace cmbl {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+int hip_fail(hipError_t e, const char* what)
+{
+    g_err = std::string("HIP error ") + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ") in " + what;
+    return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? CMBL_ERR_ALLOC : CMBL_ERR_HIP;
+}
+
+static const long double kPiL = 3.141592653589793238462643383279502884L;
+
+// ------------------------------------------------------------------ geometry
+// HEALPix RING scheme (Gorski et al. 2005); pair p = north ring p+1.
+struct RingGeom {
+    double z, sint;
+    int nph;
+    int shift;  // phi0 = pi/nph
+    long ofs;
+};
+
+static RingGeom ring_geom(int nside, long i /* 1-based ring */)
+{
+    const long N = nside;
+    const long npix = 12 * N * N;
+    const long in = (i <= 2 * N) ? i : 4 * N - i;
+    RingGeom g{};
+    if (in < N) {
+        const double t = (double)(in * in) / (3.0 * (double)N * (double)N);
+        g.z = 1.0 - t;
+        g.sint = std::sqrt(t * (2.0 - t));
+        g.nph = (int)(4 * in);
+        g.shift = 1;
+        g.ofs = 2 * in * (in - 1);
+    } else {
+        g.z = (double)(2 * N - in) * 2.0 / (3.0 * (double)N);
+        g.sint = std::sqrt((1.0 - g.z) * (1.0 + g.z));
+        g.nph = (int)(4 * N);
+        g.shift = ((in - N) % 2 == 0) ? 1 : 0;
+        g.ofs = 2 * N * (N - 1) + (in - N) * 4 * N;
+    }
+    if (i > 2 * N) {
+        g.z = -g.z;
+        g.ofs = npix - g.ofs - g.nph;
+    }
+    return g;
+}
+
+static long isqrt_l(long v)
+{
+    long r = (long)std::sqrt((double)v);
+    while (r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return r;
+}
+
+static void pix2ang(int nside, long pix, double* theta, double* phi)
+{
+    const long N = nside;
+    const long npix = 12 * N * N;
+    const long ncap = 2 * N * (N - 1);
+    const double halfpi = 0.5 * 3.141592653589793;
+    if (pix < ncap) {
+        const long iring = (1 + isqrt_l(1 + 2 * pix)) >> 1;
+        const long iphi = (pix + 1) - 2 * iring * (iring - 1);
+        const double z = 1.0 - (double)(iring * iring) / (3.0 * (double)N * (double)N);
+        *theta = std::acos(z);
+        *phi = ((double)iphi - 0.5) * halfpi / (double)iring;
+    } else if (pix < npix - ncap) {
+        const long ip = pix - ncap;
+        const long iring = ip / (4 * N) + N;
+        const long iphi = ip % (4 * N) + 1;
+        const double fodd = ((iring + N) & 1) ? 1.0 : 0.5;
+        const double z = (double)(2 * N - iring) * 2.0 / (3.0 * (double)N);
+        *theta = std::acos(z);
+        *phi = ((double)iphi - fodd) * halfpi / (double)N;
+    } else {
+        const long ip = npix - pix;
+        const long iring = (1 + isqrt_l(2 * ip - 1)) >> 1;
+        const long iphi = 4 * iring + 1 - (ip - 2 * iring * (iring - 1));
+        const double z = -1.0 + (double)(iring * iring) / (3.0 * (double)N * (double)N);
+        *theta = std::acos(z);
+        *phi = ((double)iphi - 0.5) * halfpi / (double)iring;
+    }
+}
+
+static bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// ------------------------------------------------------------------ allocation helpers
+template <typename T>
+static int dalloc(cmbl_map* mp, T** p, size_t count)
+{
+    void* q = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    CMBL_HIP(hipMalloc(&q, bytes));
+    mp->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return CMBL_OK;
+}
+template <typename T>
+static int dupload(cmbl_map* mp, T** p, const std::vector<T>& h)
+{
+    int rc = dalloc(mp, p, h.size());
+    if (rc) return rc;
+    CMBL_HIP(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return CMBL_OK;
+}
+
+#define CMBL_TRY(expr)                \
+    do {                              \
+        int rc_ = (expr);             \
+        if (rc_ != CMBL_OK) return rc_; \
+    } while (0)
+
+// Work lists for a Legendre stage with K rings per lane.  Items are chunks
+// of up to kMChunk consecutive live m for one ring block, ordered by
+// estimated cost, largest first.
+static const int kMChunk = 4;
+
+static void build_items(const cmbl_map* mp, int K, const std::vector<int>& lstart, int map_index,
+                        std::vector<WorkItem>* out, std::vector<double>* cost)
+{
+    const int L = mp->L;
+    const int npad = mp->npair_pad;
+    const int nrb = npad / (64 * K);
+    for (int rb = 0; rb < nrb; ++rb) {
+        int m = 0;
+        while (m <= L) {
+            WorkItem it{map_index, m, 0, rb};
+            double c = 0.0;
+            while (m <= L && it.mcnt < kMChunk) {
+                int lmin = L + 1;
+                for (int j = 0; j < 64 * K; ++j) lmin = std::min(lmin, lstart[(long)m * npad + rb * 64 * K + j]);
+                if (lmin > L) {
+                    if (it.mcnt > 0) break;  // close the chunk at a dead m
+                    ++m;
+                    it.m0 = m;
+                    continue;
+                }
+                c += K * (2.0 * (lmin - m) + 4.0 * (L - lmin + 1));
+                ++it.mcnt;
+                ++m;
+            }
+            if (it.mcnt > 0) {
+                out->push_back(it);
+                cost->push_back(c);
+            }
+        }
+    }
+}
+
+static void sort_by_cost(std::vector<WorkItem>* items, std::vector<double>* cost)
+{
+    std::vector<size_t> idx(items->size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return (*cost)[a] > (*cost)[b]; });
+    std::vector<WorkItem> s(items->size());
+    std::vector<double> c(items->size());
+    for (size_t i = 0; i < idx.size(); ++i) {
+        s[i] = (*items)[idx[i]];
+        c[i] = (*cost)[idx[i]];
+    }
+    items->swap(s);
+    cost->swap(c);
+}
+
+static int choose_k(int npair, int kmax)
+{
+    int k = 1;
+    while (k < kmax && npair >= 64 * k * 2 * 4) k *= 2;  // keep >= 4 ring blocks per m
+    return k;
+}
+
+static int map_create(cmbl_map* mp, int nside, int lmax)
+{
+    const int N = nside;
+    const int L = lmax;
+    mp->nside = N;
+    mp->L = L;
+    mp->npix = 12L * N * N;
+    mp->nring = 4 * N - 1;
+    mp->npair = 2 * N;
+    mp->nalm = (long)(L + 1) * (L + 2) / 2;
+    mp->nparam = (long)(L + 1) * (L + 1);
+    CMBL_HIP(hipGetDevice(&mp->device));
+    mp->K1 = choose_k(mp->npair, 4);
+    mp->K5 = choose_k(mp->npair, 8);
+    const int kmax = std::max(mp->K1, mp->K5);
+    mp->npair_pad = ((mp->npair + 64 * kmax - 1) / (64 * kmax)) * (64 * kmax);
+    mp->nrb1 = mp->npair_pad / (64 * mp->K1);
+    mp->nrb5 = mp->npair_pad / (64 * mp->K5);
+    const int npad = mp->npair_pad;
+
+    // ---- ring-pair tables
+    std::vector<double> rx(npad, 0.0), rs(npad, 0.0);
+    std::vector<int> nph(npad, 0), shift(npad, 0), Mfft(npad, 0);
+    std::vector<long> ofsN(npad, 0), ofsS(npad, -1), bsofs(npad, -1);
+    std::vector<int> bs_pairs;
+    long bs_total = 0;
+    int maxM = 1;
+    for (int p = 0; p < mp->npair; ++p) {
+        const RingGeom g = ring_geom(N, p + 1);
+        rx[p] = g.z;
+        rs[p] = g.sint;
+        nph[p] = g.nph;
+        shift[p] = g.shift;
+        ofsN[p] = g.ofs;
+        ofsS[p] = (p + 1 == 2 * N) ? -1 : ring_geom(N, 4L * N - (p + 1)).ofs;
+        if (is_pow2(g.nph)) {
+            Mfft[p] = g.nph;
+        } else {
+            int M = 1;
+            while (M < 2 * g.nph - 1) M <<= 1;
+            Mfft[p] = M;
+            bsofs[p] = bs_total;
+            bs_total += M;
+            bs_pairs.push_back(p);
+        }
+        maxM = std::max(maxM, Mfft[p]);
+    }
+    if (maxM > kMaxFFT)
+        return fail(CMBL_ERR_ARG, "ring transform length " + std::to_string(maxM) + " exceeds the LDS limit " +
+                                      std::to_string(kMaxFFT) + " (nside > 1024 is not supported yet)");
+    mp->lds_ring = maxM * (int)sizeof(double2);
+
+    // ---- recurrence tables (long double on the host, rounded once)
+    std::vector<double> A(mp->nalm, 0.0), Nrm(mp->nalm, 1.0);
+    for (int m = 0; m <= L; ++m) {
+        const long b = moff(L, m) - m;
+        long double Nm2 = 1.0L, Nm1 = 1.0L;  // N_{l-2}, N_{l-1}
+        long double alpha_prev = 0.0L;
+        for (int l = m; l <= L; ++l) {
+            if (l == m) {
+                Nrm[b + l] = 1.0;
+                continue;
+            }
+            const long double ll = l, mm = m;
+            const long double alpha = std::sqrt((4.0L * ll * ll - 1.0L) / (ll * ll - mm * mm));
+            if (l == m + 1) {
+                A[b + l] = (double)alpha;
+                Nrm[b + l] = 1.0;
+                Nm2 = 1.0L;
+                Nm1 = 1.0L;
+            } else {
+                const long double Nl = alpha / alpha_prev * Nm2;
+                A[b + l] = (double)(alpha * Nm1 / Nl);
+                Nrm[b + l] = (double)Nl;
+                Nm2 = Nm1;
+                Nm1 = Nl;
+            }
+            alpha_prev = alpha;
+        }
+    }
+
+    // ---- seeds lambda_mm(theta_p) in scaled form, m-major [m][p]
+    std::vector<double> seed_v((size_t)(L + 1) * npad, 0.0);
+    std::vector<int> seed_k((size_t)(L + 1) * npad, 0);
+    const long double up = ldexpl(1.0L, 1000), lo = ldexpl(1.0L, -100);
+    for (int p = 0; p < mp->npair; ++p) {
+        long double r = 1.0L / std::sqrt(4.0L * kPiL);
+        int k = 0;
+        const long double s = rs[p];
+        for (int m = 0; m <= L; ++m) {
+            if (m > 0) {
+                r *= -std::sqrt((2.0L * m - 1.0L) / (2.0L * m)) * s;
+                while (r != 0.0L && fabsl(r) <= lo) {
+                    r *= up;
+                    ++k;
+                }
+            }
+            seed_v[(size_t)m * npad + p] = (double)(r * std::sqrt(2.0L * m + 1.0L));
+            seed_k[(size_t)m * npad + p] = k;
+        }
+    }
+
+    // ---- twiddles e^{-2 pi i k / kMaxFFT}
+    std::vector<double2> tw(kMaxFFT / 2);
+    for (int k = 0; k < kMaxFFT / 2; ++k) {
+        const long double a = 2.0L * kPiL * k / kMaxFFT;
+        tw[k] = make_double2((double)cosl(a), (double)-sinl(a));
+    }
+
+    // ---- device uploads
+    MapDesc& d = mp->desc;
+    std::memset(&d, 0, sizeof(d));
+    d.nside = N;
+    d.L = L;
+    d.npair = mp->npair;
+    d.npair_pad = npad;
+    d.nring = mp->nring;
+    d.nrb5 = mp->nrb5;
+    d.npix = mp->npix;
+    d.nalm = mp->nalm;
+    double *p_rx, *p_A, *p_N, *p_sv;
+    int *p_nph, *p_sh, *p_M, *p_sk;
+    long *p_oN, *p_oS, *p_bo;
+    CMBL_TRY(dupload(mp, &p_rx, rx));
+    CMBL_TRY(dupload(mp, &p_nph, nph));
+    CMBL_TRY(dupload(mp, &p_sh, shift));
+    CMBL_TRY(dupload(mp, &p_oN, ofsN));
+    CMBL_TRY(dupload(mp, &p_oS, ofsS));
+    CMBL_TRY(dupload(mp, &p_M, Mfft));
+    CMBL_TRY(dupload(mp, &p_bo, bsofs));
+    CMBL_TRY(dupload(mp, &p_A, A));
+    CMBL_TRY(dupload(mp, &p_N, Nrm));
+    CMBL_TRY(dupload(mp, &p_sv, seed_v));
+    CMBL_TRY(dupload(mp, &p_sk, seed_k));
+    CMBL_TRY(dupload(mp, &mp->d_tw, tw));
+    d.ring_x = p_rx;
+    d.ring_nph = p_nph;
+    d.ring_shift = p_sh;
+    d.ring_ofsN = p_oN;
+    d.ring_ofsS = p_oS;
+    d.ring_M = p_M;
+    d.ring_bsofs = p_bo;
+    d.A = p_A;
+    d.Nrm = p_N;
+    d.seed_v = p_sv;
+    d.seed_k = p_sk;
+    double2* p_bs;
+    CMBL_TRY(dalloc(mp, &p_bs, (size_t)std::max<long>(bs_total, 1)));
+    d.bs_filt = p_bs;
+    // beam = 1, no prior until set
+    CMBL_TRY(dupload(mp, &mp->d_beam, std::vector<double>(L + 1, 1.0)));
+    CMBL_TRY(dupload(mp, &mp->d_invcl, std::vector<double>(L + 1, 0.0)));
+    d.beam = mp->d_beam;
+    d.invcl = mp->d_invcl;
+    CMBL_TRY(dalloc(mp, &mp->d_data, mp->npix));
+    CMBL_TRY(dalloc(mp, &mp->d_w, mp->npix));
+    CMBL_HIP(hipMemset(mp->d_data, 0, sizeof(double) * mp->npix));
+    CMBL_HIP(hipMemset(mp->d_w, 0, sizeof(double) * mp->npix));
+    d.d = mp->d_data;
+    d.w = mp->d_w;
+    // workspace
+    CMBL_TRY(dalloc(mp, &d.alm, mp->nalm));
+    CMBL_TRY(dalloc(mp, &d.P, (size_t)mp->npair * (L + 1) * 4));
+    CMBL_HIP(hipMemset(d.P, 0, sizeof(double) * (size_t)mp->npair * (L + 1) * 4));
+    CMBL_TRY(dalloc(mp, &d.Q, (size_t)mp->npair * (L + 1) * 4));
+    CMBL_TRY(dalloc(mp, &d.upart, (size_t)mp->nrb5 * mp->nalm * 2));
+    CMBL_HIP(hipMemset(d.upart, 0, sizeof(double) * (size_t)mp->nrb5 * mp->nalm * 2));
+    CMBL_TRY(dalloc(mp, &d.chi, mp->npair));
+    d.nprior = ((L + 1 + 255) / 256) * (L + 1);
+    CMBL_TRY(dalloc(mp, &d.prior, d.nprior));
+    CMBL_TRY(dalloc(mp, &mp->d_f, 1));
+    CMBL_TRY(dalloc(mp, &mp->d_x, mp->nparam));
+    CMBL_TRY(dalloc(mp, &mp->d_g, mp->nparam));
+    CMBL_TRY(dalloc(mp, &mp->d_map, mp->npix));
+    CMBL_TRY(dalloc(mp, &mp->d_desc, 1));
+    CMBL_HIP(hipMemcpy(mp->d_desc, &d, sizeof(d), hipMemcpyHostToDevice));
+
+    // ---- Bluestein filters
+    if (!bs_pairs.empty()) {
+        int* d_pairs;
+        CMBL_TRY(dupload(mp, &d_pairs, bs_pairs));
+        CMBL_TRY(launch_bluestein_setup(mp->d_desc, d_pairs, (int)bs_pairs.size(), p_bs, mp->lds_ring, mp->d_tw,
+                                        nullptr));
+        CMBL_HIP(hipDeviceSynchronize());
+    }
+
+    // ---- scan: first live l per (m, pair), then the work lists
+    std::vector<WorkItem> all;
+    for (int rb = 0; rb < mp->nrb1; ++rb)
+        for (int m = 0; m <= L; m += kMChunk) all.push_back(WorkItem{0, m, std::min(kMChunk, L + 1 - m), rb});
+    WorkItem* d_all;
+    int* d_ls;
+    CMBL_TRY(dupload(mp, &d_all, all));
+    CMBL_TRY(dalloc(mp, &d_ls, (size_t)(L + 1) * npad));
+    CMBL_TRY(launch_scan(mp->K1, d_all, (int)all.size(), mp->d_desc, d_ls, nullptr));
+    std::vector<int> lstart((size_t)(L + 1) * npad);
+    CMBL_HIP(hipMemcpy(lstart.data(), d_ls, sizeof(int) * lstart.size(), hipMemcpyDeviceToHost));
+    // padding pairs never constrain liveness
+    for (int m = 0; m <= L; ++m)
+        for (int p = mp->npair; p < npad; ++p) lstart[(size_t)m * npad + p] = L + 1;
+
+    std::vector<double> c1, c5;
+    build_items(mp, mp->K1, lstart, 0, &mp->items1, &c1);
+    build_items(mp, mp->K5, lstart, 0, &mp->items5, &c5);
+    sort_by_cost(&mp->items1, &c1);
+    sort_by_cost(&mp->items5, &c5);
+    mp->nitems1 = (int)mp->items1.size();
+    mp->nitems5 = (int)mp->items5.size();
+    CMBL_TRY(dupload(mp, &mp->d_items1, mp->items1));
+    CMBL_TRY(dupload(mp, &mp->d_items5, mp->items5));
+    // ring items: largest transform first
+    for (int p = 0; p < mp->npair; ++p) mp->ring_items.push_back(RingItem{0, p});
+    std::stable_sort(mp->ring_items.begin(), mp->ring_items.end(),
+                     [&](const RingItem& a, const RingItem& b) { return Mfft[a.pair] > Mfft[b.pair]; });
+    mp->nring_items = (int)mp->ring_items.size();
+    CMBL_TRY(dupload(mp, &mp->d_ring_items, mp->ring_items));
+    CMBL_HIP(hipDeviceSynchronize());
+    return CMBL_OK;
+}
+
+static void map_free(cmbl_map* mp)
+{
+    for (void* p : mp->allocs) (void)hipFree(p);
+    mp->allocs.clear();
+}
+
+static int check_map(const cmbl_map* mp)
+{
+    if (!mp) return fail(CMBL_ERR_ARG, "null map");
+    return CMBL_OK;
+}
+
+// One evaluation on stream s with device x; f and g are device pointers.
+static int eval_device(cmbl_map* mp, const double* x, double* f, double* g, hipStream_t s)
+{
+    const MapDesc& d = mp->desc;
+    CMBL_TRY(launch_pre(d, mp->d_desc, x, s));
+    CMBL_TRY(launch_synth(mp->K1, mp->d_items1, mp->nitems1, mp->d_desc, s));
+    CMBL_TRY(launch_ring(kRingLike, mp->d_ring_items, mp->nring_items, mp->d_desc, mp->lds_ring, nullptr, nullptr,
+                         mp->d_tw, s));
+    CMBL_TRY(launch_adj(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, s));
+    CMBL_TRY(launch_post(d, mp->d_desc, x, g, 1, s));
+    CMBL_TRY(launch_final(mp->d_desc, f, s));
+    return CMBL_OK;
+}
+
+}  // namespace cmbl
+
+using namespace cmbl;
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* cmbl_last_error(void) { return g_err.c_str(); }
+const char* cmbl_version(void) { return "cosmopp_amd 0.1 (gfx950)"; }
+
+int cmbl_map_create(cmbl_map** out, int nside, int lmax)
+{
+    if (!out) return fail(CMBL_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (nside < 1 || !is_pow2(nside)) return fail(CMBL_ERR_ARG, "nside must be a positive power of two");
+    if (lmax < 2 || lmax > 4 * nside) return fail(CMBL_ERR_ARG, "lmax must be in [2, 4*nside]");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CMBL_ERR_HIP, "no HIP device available");
+    cmbl_map* mp = new cmbl_map();
+    const int rc = map_create(mp, nside, lmax);
+    if (rc != CMBL_OK) {
+        const std::string msg = g_err;
+        map_free(mp);
+        delete mp;
+        g_err = msg;
+        return rc;
+    }
+    *out = mp;
+    return CMBL_OK;
+}
+
+void cmbl_map_destroy(cmbl_map* mp)
+{
+    if (!mp) return;
+    (void)hipDeviceSynchronize();
+    map_free(mp);
+    delete mp;
+}
+
+int cmbl_map_nside(const cmbl_map* mp) { return mp ? mp->nside : -1; }
+int cmbl_map_lmax(const cmbl_map* mp) { return mp ? mp->L : -1; }
+long cmbl_map_npix(const cmbl_map* mp) { return mp ? mp->npix : -1; }
+long cmbl_map_nparams(const cmbl_map* mp) { return mp ? mp->nparam : -1; }
+
+int cmbl_map_set_beam(cmbl_map* mp, const double* beam)
+{
+    CMBL_TRY(check_map(mp));
+    if (!beam) return fail(CMBL_ERR_ARG, "beam is null");
+    CMBL_HIP(hipMemcpy(mp->d_beam, beam, sizeof(double) * (mp->L + 1), hipMemcpyHostToDevice));
+    mp->has_beam = true;
+    return CMBL_OK;
+}
+
+int cmbl_map_set_cl(cmbl_map* mp, const double* cl)
+{
+    CMBL_TRY(check_map(mp));
+    if (!cl) return fail(CMBL_ERR_ARG, "cl is null");
+    std::vector<double> inv(mp->L + 1, 0.0);
+    for (int l = 0; l <= mp->L; ++l) inv[l] = cl[l] > 0 ? 1.0 / cl[l] : 0.0;
+    CMBL_HIP(hipMemcpy(mp->d_invcl, inv.data(), sizeof(double) * (mp->L + 1), hipMemcpyHostToDevice));
+    mp->has_cl = true;
+    return CMBL_OK;
+}
+
+int cmbl_map_set_data(cmbl_map* mp, const double* d, const double* w)
+{
+    CMBL_TRY(check_map(mp));
+    if (!d || !w) return fail(CMBL_ERR_ARG, "data or weight is null");
+    CMBL_HIP(hipMemcpy(mp->d_data, d, sizeof(double) * mp->npix, hipMemcpyHostToDevice));
+    CMBL_HIP(hipMemcpy(mp->d_w, w, sizeof(double) * mp->npix, hipMemcpyHostToDevice));
+    mp->has_data = true;
+    return CMBL_OK;
+}
+
+int cmbl_map_set_data_device(cmbl_map* mp, const double* d, const double* w, void* stream)
+{
+    CMBL_TRY(check_map(mp));
+    if (!d || !w) return fail(CMBL_ERR_ARG, "data or weight is null");
+    hipStream_t s = (hipStream_t)stream;
+    CMBL_HIP(hipMemcpyAsync(mp->d_data, d, sizeof(double) * mp->npix, hipMemcpyDeviceToDevice, s));
+    CMBL_HIP(hipMemcpyAsync(mp->d_w, w, sizeof(double) * mp->npix, hipMemcpyDeviceToDevice, s));
+    mp->has_data = true;
+    return CMBL_OK;
+}
+
+int cmbl_value_and_grad_device(cmbl_map* mp, const double* x, double* f, double* g, void* stream)
+{
+    CMBL_TRY(check_map(mp));
+    if (!x || !f) return fail(CMBL_ERR_ARG, "x or f is null");
+    if (!mp->has_data) return fail(CMBL_ERR_STATE, "cmbl_map_set_data has not been called");
+    hipStream_t s = (hipStream_t)stream;
+    return eval_device(mp, x, f, g ? g : mp->d_g, s);
+}
+
+int cmbl_value_and_grad(cmbl_map* mp, const double* x, double* f, double* g)
+{
+    CMBL_TRY(check_map(mp));
+    if (!x || !f) return fail(CMBL_ERR_ARG, "x or f is null");
+    if (!mp->has_data) return fail(CMBL_ERR_STATE, "cmbl_map_set_data has not been called");
+    CMBL_HIP(hipMemcpy(mp->d_x, x, sizeof(double) * mp->nparam, hipMemcpyHostToDevice));
+    CMBL_TRY(eval_device(mp, mp->d_x, mp->d_f, mp->d_g, nullptr));
+    CMBL_HIP(hipMemcpy(f, mp->d_f, sizeof(double), hipMemcpyDeviceToHost));
+    if (g) CMBL_HIP(hipMemcpy(g, mp->d_g, sizeof(double) * mp->nparam, hipMemcpyDeviceToHost));
+    if (!std::isfinite(*f)) return fail(CMBL_ERR_NONFINITE, "-2lnL is not finite (NaN/Inf in a_lm or data)");
+    return CMBL_OK;
+}
+
+int cmbl_forward_device(cmbl_map* mp, const double* x, double* map, void* stream)
+{
+    CMBL_TRY(check_map(mp));
+    if (!x || !map) return fail(CMBL_ERR_ARG, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    CMBL_TRY(launch_pre(mp->desc, mp->d_desc, x, s));
+    CMBL_TRY(launch_synth(mp->K1, mp->d_items1, mp->nitems1, mp->d_desc, s));
+    CMBL_TRY(launch_ring(kRingSyn, mp->d_ring_items, mp->nring_items, mp->d_desc, mp->lds_ring, map, nullptr,
+                         mp->d_tw, s));
+    return CMBL_OK;
+}
+
+int cmbl_forward(cmbl_map* mp, const double* x, double* map)
+{
+    CMBL_TRY(check_map(mp));
+    if (!x || !map) return fail(CMBL_ERR_ARG, "null pointer");
+    CMBL_HIP(hipMemcpy(mp->d_x, x, sizeof(double) * mp->nparam, hipMemcpyHostToDevice));
+    CMBL_TRY(cmbl_forward_device(mp, mp->d_x, mp->d_map, nullptr));
+    CMBL_HIP(hipMemcpy(map, mp->d_map, sizeof(double) * mp->npix, hipMemcpyDeviceToHost));
+    return CMBL_OK;
+}
+
+int cmbl_backward_device(cmbl_map* mp, const double* v, double* g, void* stream)
+{
+    CMBL_TRY(check_map(mp));
+    if (!v || !g) return fail(CMBL_ERR_ARG, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    CMBL_TRY(launch_ring(kRingAna, mp->d_ring_items, mp->nring_items, mp->d_desc, mp->lds_ring, nullptr, v,
+                         mp->d_tw, s));
+    CMBL_TRY(launch_adj(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, s));
+    CMBL_TRY(launch_post(mp->desc, mp->d_desc, nullptr, g, 0, s));
+    return CMBL_OK;
+}
+
+int cmbl_backward(cmbl_map* mp, const double* v, double* g)
+{
+    CMBL_TRY(check_map(mp));
+    if (!v || !g) return fail(CMBL_ERR_ARG, "null pointer");
+    CMBL_HIP(hipMemcpy(mp->d_map, v, sizeof(double) * mp->npix, hipMemcpyHostToDevice));
+    CMBL_TRY(cmbl_backward_device(mp, mp->d_map, mp->d_g, nullptr));
+    CMBL_HIP(hipMemcpy(g, mp->d_g, sizeof(double) * mp->nparam, hipMemcpyDeviceToHost));
+    return CMBL_OK;
+}
+
+// ---------------------------------------------------------------- batch
+int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
+{
+    if (!out || !maps || nmaps < 1) return fail(CMBL_ERR_ARG, "bad batch arguments");
+    *out = nullptr;
+    cmbl_batch* b = new cmbl_batch();
+    auto cleanup = [&](int rc) {
+        for (void* p : b->allocs) (void)hipFree(p);
+        delete b;
+        return rc;
+    };
+    std::vector<MapDesc> descs;
+    std::vector<WorkItem> w1[4], w5[4];
+    std::vector<double> c1[4], c5[4];
+    std::vector<RingItem> rings;
+    std::vector<int> ringM;
+    for (int i = 0; i < nmaps; ++i) {
+        cmbl_map* mp = maps[i];
+        if (!mp) return cleanup(fail(CMBL_ERR_ARG, "null map in batch"));
+        b->maps.push_back(mp);
+        descs.push_back(mp->desc);
+        int k1 = 0, k5 = 0;
+        while ((1 << k1) < mp->K1) ++k1;
+        while ((1 << k5) < mp->K5) ++k5;
+        for (WorkItem it : mp->items1) {
+            it.map = i;
+            w1[k1].push_back(it);
+        }
+        for (WorkItem it : mp->items5) {
+            it.map = i;
+            w5[k5].push_back(it);
+        }
+        // costs are recomputed approximately from the m range
+        for (const WorkItem& it : mp->items1) c1[k1].push_back(mp->K1 * (double)it.mcnt * (mp->L + 1 - it.m0));
+        for (const WorkItem& it : mp->items5) c5[k5].push_back(mp->K5 * (double)it.mcnt * (mp->L + 1 - it.m0));
+        for (RingItem r : mp->ring_items) {
+            r.map = i;
+            rings.push_back(r);
+            ringM.push_back(mp->lds_ring);
+        }
+        b->lds_ring = std::max(b->lds_ring, mp->lds_ring);
+    }
+    b->d_tw = maps[0]->d_tw;
+    void* q = nullptr;
+    if (hipMalloc(&q, sizeof(MapDesc) * descs.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
+    b->allocs.push_back(q);
+    b->d_descs = (MapDesc*)q;
+    if (hipMemcpy(b->d_descs, descs.data(), sizeof(MapDesc) * descs.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(CMBL_ERR_HIP, "batch upload"));
+    for (int k = 0; k < 4; ++k) {
+        for (int stage = 0; stage < 2; ++stage) {
+            std::vector<WorkItem>& w = stage ? w5[k] : w1[k];
+            std::vector<double>& c = stage ? c5[k] : c1[k];
+            if (w.empty()) continue;
+            sort_by_cost(&w, &c);
+            if (hipMalloc(&q, sizeof(WorkItem) * w.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
+            b->allocs.push_back(q);
+            if (hipMemcpy(q, w.data(), sizeof(WorkItem) * w.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return cleanup(fail(CMBL_ERR_HIP, "batch upload"));
+            if (stage) {
+                b->d_items5[k] = (WorkItem*)q;
+                b->n5[k] = (int)w.size();
+            } else {
+                b->d_items1[k] = (WorkItem*)q;
+                b->n1[k] = (int)w.size();
+            }
+        }
+    }
+    {
+        std::vector<size_t> idx(rings.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t c) {
+            const cmbl_map* ma = maps[rings[a].map];
+            const cmbl_map* mc = maps[rings[c].map];
+            return ma->nside > mc->nside;
+        });
+        std::vector<RingItem> sorted;
+        for (size_t i : idx) sorted.push_back(rings[i]);
+        rings.swap(sorted);
+    }
+    if (hipMalloc(&q, sizeof(RingItem) * rings.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
+    b->allocs.push_back(q);
+    b->d_ring_items = (RingItem*)q;
+    b->nring_items = (int)rings.size();
+    if (hipMemcpy(q, rings.data(), sizeof(RingItem) * rings.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(CMBL_ERR_HIP, "batch upload"));
+    *out = b;
+    return CMBL_OK;
+}
+
+void cmbl_batch_destroy(cmbl_batch* b)
+{
+    if (!b) return;
+    (void)hipDeviceSynchronize();
+    for (void* p : b->allocs) (void)hipFree(p);
+    delete b;
+}
+
+int cmbl_batch_value_and_grad_device(cmbl_batch* b, const double* const* x, double* f, double* const* g,
+                                     void* stream)
+{
+    if (!b || !x || !f || !g) return fail(CMBL_ERR_ARG, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const int n = (int)b->maps.size();
+    for (int i = 0; i < n; ++i) {
+        if (!b->maps[i]->has_data) return fail(CMBL_ERR_STATE, "map without data in batch");
+        CMBL_TRY(launch_pre(b->maps[i]->desc, b->d_descs + i, x[i], s));
+    }
+    for (int k = 0; k < 4; ++k) CMBL_TRY(launch_synth(1 << k, b->d_items1[k], b->n1[k], b->d_descs, s));
+    CMBL_TRY(launch_ring(kRingLike, b->d_ring_items, b->nring_items, b->d_descs, b->lds_ring, nullptr, nullptr,
+                         b->d_tw, s));
+    for (int k = 0; k < 4; ++k) CMBL_TRY(launch_adj(1 << k, b->d_items5[k], b->n5[k], b->d_descs, s));
+    for (int i = 0; i < n; ++i) {
+        CMBL_TRY(launch_post(b->maps[i]->desc, b->d_descs + i, x[i], g[i], 1, s));
+        CMBL_TRY(launch_final(b->d_descs + i, f + i, s));
+    }
+    return CMBL_OK;
+}
+
+// ---------------------------------------------------------------- inputs
+int cmbl_simulate_alm(const double* cl, int lmax, long seed, double* x)
+{
+    if (!cl || !x || lmax < 0) return fail(CMBL_ERR_ARG, "bad simulate_alm arguments");
+    if (seed == 0) return fail(CMBL_ERR_ARG, "seed 0 means time(0) in the reference; pass a nonzero seed");
+    std::mt19937 gen((int)seed);
+    std::normal_distribution<> dist(0, 1);
+    for (int l = 0; l <= lmax; ++l) {
+        x[(long)l * l] = dist(gen) * std::sqrt(cl[l]);
+        for (int m = 1; m <= l; ++m) {
+            const double re = dist(gen) * std::sqrt(cl[l] / 2);
+            const double im = dist(gen) * std::sqrt(cl[l] / 2);
+            x[(long)l * l + 2 * m - 1] = re;
+            x[(long)l * l + 2 * m] = im;
+        }
+    }
+    return CMBL_OK;
+}
+
+int cmbl_simulate_white_noise(int nside, double sigma, long seed, double* map)
+{
+    if (!map || nside < 1) return fail(CMBL_ERR_ARG, "bad white-noise arguments");
+    if (seed == 0) return fail(CMBL_ERR_ARG, "seed 0 means time(0) in the reference; pass a nonzero seed");
+    std::mt19937 gen((int)seed);
+    std::normal_distribution<> dist(0, sigma);
+    const long npix = 12L * nside * nside;
+    for (long i = 0; i < npix; ++i) map[i] = dist(gen);
+    return CMBL_OK;
+}
+
+int cmbl_beam(int lmax, double fwhm, double* beam)
+{
+    if (!beam || lmax < 0) return fail(CMBL_ERR_ARG, "bad beam arguments");
+    if (fwhm < 0) return fail(CMBL_ERR_ARG, "invalid fwhm");
+    for (int l = 0; l <= lmax; ++l) {
+        if (fwhm == 0) {
+            beam[l] = 1.0;
+            continue;
+        }
+        const double sigma = std::sqrt(8 * std::log(2.0)) / (fwhm * 3.141592653589793 / 180);
+        beam[l] = std::exp(-l * (l + 1) / (2 * sigma * sigma));
+    }
+    return CMBL_OK;
+}
+
+int cmbl_pix2ang_ring(int nside, long pix, double* theta, double* phi)
+{
+    if (nside < 1 || pix < 0 || pix >= 12L * nside * nside || !theta || !phi)
+        return fail(CMBL_ERR_ARG, "bad pix2ang arguments");
+    pix2ang(nside, pix, theta, phi);
+    return CMBL_OK;
+}
+
+int cmbl_mask_latitude(int nside, double cut, double* mask)
+{
+    if (!mask || nside < 1) return fail(CMBL_ERR_ARG, "bad mask arguments");
+    const long npix = 12L * nside * nside;
+    const double halfpi = 3.141592653589793 / 2;
+    for (long p = 0; p < npix; ++p) {
+        double th, ph;
+        pix2ang(nside, p, &th, &ph);
+        if (th < halfpi + cut && th > halfpi - cut) mask[p] = 0;
+    }
+    return CMBL_OK;
+}
+
+int cmbl_mask_regions(int nside, int nreg, const double* theta, const double* phi, const double* angle,
+                      double* mask)
+{
+    if (!mask || nside < 1 || nreg < 0 || (nreg > 0 && (!theta || !phi || !angle)))
+        return fail(CMBL_ERR_ARG, "bad mask arguments");
+    const int nring = 4 * nside - 1;
+    // ring theta table (pixel centres of a ring share theta)
+    std::vector<double> rth(nring);
+    std::vector<long> rofs(nring);
+    std::vector<int> rn(nring);
+    for (int r = 0; r < nring; ++r) {
+        const RingGeom g = ring_geom(nside, r + 1);
+        double th, ph;
+        pix2ang(nside, g.ofs, &th, &ph);
+        rth[r] = th;
+        rofs[r] = g.ofs;
+        rn[r] = g.nph;
+    }
+    for (int j = 0; j < nreg; ++j) {
+        const double x = std::sin(theta[j]) * std::cos(phi[j]), y = std::sin(theta[j]) * std::sin(phi[j]),
+                     z = std::cos(theta[j]);
+        const double ca = std::cos(angle[j]);
+        const double margin = 1e-9;
+        for (int r = 0; r < nring; ++r) {
+            if (std::fabs(rth[r] - theta[j]) > angle[j] + margin) continue;
+            for (int i = 0; i < rn[r]; ++i) {
+                const long p = rofs[r] + i;
+                double th, ph;
+                pix2ang(nside, p, &th, &ph);
+                const double x1 = std::sin(th) * std::cos(ph), y1 = std::sin(th) * std::sin(ph), z1 = std::cos(th);
+                const double dot = x1 * x + y1 * y + z1 * z;
+                if (dot > ca) mask[p] = 0;
+            }
+        }
+    }
+    return CMBL_OK;
+}
+
+int cmbl_weights_from_mask(long npix, const double* mask, double sigma, double* w)
+{
+    if (!mask || !w || npix < 0 || !(sigma > 0)) return fail(CMBL_ERR_ARG, "bad weight arguments");
+    for (long i = 0; i < npix; ++i) w[i] = mask[i] > 0.5 ? 1.0 / (sigma * sigma) : 0.0;
+    return CMBL_OK;
+}
+
+// ---------------------------------------------------------------- device vectors
+int cmbl_vec_copy(double* y, const double* x, double c, long n, void* s)
+{
+    return launch_vec(0, y, x, c, n, (hipStream_t)s);
+}
+int cmbl_vec_set(double* y, double v, long n, void* s) { return launch_vec(1, y, nullptr, v, n, (hipStream_t)s); }
+int cmbl_vec_axpy(double* y, const double* x, double c, long n, void* s)
+{
+    return launch_vec(2, y, x, c, n, (hipStream_t)s);
+}
+int cmbl_vec_mul(double* y, const double* x, long n, void* s) { return launch_vec(3, y, x, 0, n, (hipStream_t)s); }
+int cmbl_vec_div(double* y, const double* x, long n, void* s) { return launch_vec(4, y, x, 0, n, (hipStream_t)s); }
+int cmbl_vec_pow(double* y, double p, long n, void* s) { return launch_vec(5, y, nullptr, p, n, (hipStream_t)s); }
+
+static double* dot_work()
+{
+    static thread_local double* w = nullptr;
+    if (!w) {
+        if (hipMalloc(&w, sizeof(double) * 1025) != hipSuccess) w = nullptr;
+    }
+    return w;
+}
+
+int cmbl_vec_dot_device(const double* x, const double* y, long n, double* d_out, void* s)
+{
+    double* w = dot_work();
+    if (!w) return fail(CMBL_ERR_ALLOC, "dot workspace");
+    return launch_dot(x, y, n, d_out, w, (hipStream_t)s);
+}
+
+int cmbl_vec_dot(const double* x, const double* y, long n, double* out, void* s)
+{
+    double* w = dot_work();
+    if (!w) return fail(CMBL_ERR_ALLOC, "dot workspace");
+    CMBL_TRY(launch_dot(x, y, n, w + 1024, w, (hipStream_t)s));
+    CMBL_HIP(hipMemcpyAsync(out, w + 1024, sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)s));
+    CMBL_HIP(hipStreamSynchronize((hipStream_t)s));
+    return CMBL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+/*
+ * cmbl.h - C ABI of the MI355X-native map-level CMB likelihood
+ * (package cosmopp_amd, library cosmopp_amd/libcosmopp_amd.so).
+ *
+ * The path: -2lnL(a_lm) and its gradient for a HEALPix RING map,
+ *   a_lm -> B_l -> spherical-harmonic synthesis -> r = d - s -> w r
+ *   -> adjoint analysis (exact transpose) -> gradient in the packed layout,
+ * as composed by Cosmo++ around HEALPix alm2map / map2alm in
+ * CmbGibbsCGTreats::multiplyByMatrix (source/cmb_gibbs.cpp:359-405).
+ *
+ * Conventions (all entry points):
+ *   - return CMBL_OK (0) or a CMBL_ERR_* code; cmbl_last_error() gives the
+ *     message of the last failure on the calling thread.  No C++ exception
+ *     crosses this boundary (the reference throws StandardException,
+ *     include/exception_handler.hpp:10-30; the C++ wrapper rethrows).
+ *   - "packed" a_lm vectors use the almToVector layout
+ *     (source/cmb_gibbs.cpp:320-357): for l = 0..lmax: Re a_l0, then for
+ *     m = 1..l: Re a_lm, Im a_lm.  Length (lmax+1)^2.
+ *   - maps are FP64, RING order, 12*nside^2 pixels.
+ *   - buffers are owned by the caller; *_device entry points take device
+ *     pointers and an explicit hipStream_t (passed as void*) and are
+ *     asynchronous on that stream; the others take host pointers and block.
+ *   - a context is not thread-safe; different contexts are independent.
+ */
+#ifndef CMBL_H
+#define CMBL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CMBL_OK 0
+#define CMBL_ERR_ARG 1       /* invalid argument (sizes, null pointers, ranges) */
+#define CMBL_ERR_HIP 2       /* HIP runtime error or no usable GPU */
+#define CMBL_ERR_ALLOC 3     /* device or host allocation failed */
+#define CMBL_ERR_STATE 4     /* call out of order (e.g. no data set yet) */
+#define CMBL_ERR_NONFINITE 5 /* NaN/Inf in the likelihood value */
+
+typedef struct cmbl_map cmbl_map;     /* one map: geometry, tables, data, workspace */
+typedef struct cmbl_batch cmbl_batch; /* many maps evaluated in one launch per stage */
+
+const char* cmbl_last_error(void);
+/* Library version string. */
+const char* cmbl_version(void);
+
+/* ---------------------------------------------------------------- map context */
+/* Replaces the HEALPix map/alm objects and the geometry set-up that
+ * CmbGibbsCGTreats holds (source/cmb_gibbs.cpp:303-318): builds the RING
+ * geometry, the Legendre recurrence tables and the seeds on the current
+ * device.  1 <= nside (power of two), 0 <= lmax <= 4*nside, lmax >= 2. */
+int cmbl_map_create(cmbl_map** out, int nside, int lmax);
+void cmbl_map_destroy(cmbl_map* map);
+int cmbl_map_nside(const cmbl_map* map);
+int cmbl_map_lmax(const cmbl_map* map);
+long cmbl_map_npix(const cmbl_map* map);
+long cmbl_map_nparams(const cmbl_map* map); /* (lmax+1)^2 */
+
+/* Beam B_l (lmax+1 values), the per-l scaling of source/cmb_gibbs.cpp:370-375. */
+int cmbl_map_set_beam(cmbl_map* map, const double* beam);
+/* Prior spectrum C_l (lmax+1 values); l with C_l <= 0 carry no prior term
+ * (the Gibbs sampler sets C_0 = C_1 = 0, source/cmb_gibbs.cpp:180-181). */
+int cmbl_map_set_cl(cmbl_map* map, const double* cl);
+/* Data map d and per-pixel inverse-noise weight w (= mask/sigma^2, the
+ * weighting of source/cmb_gibbs.cpp:381-387), host pointers, npix each. */
+int cmbl_map_set_data(cmbl_map* map, const double* d, const double* w);
+int cmbl_map_set_data_device(cmbl_map* map, const double* d_d, const double* d_w, void* stream);
+
+/* ---------------------------------------------------------------- likelihood */
+/* -2lnL and its gradient at the packed a_lm x.  This is the body of the
+ * function-with-gradient pair Math::RealFunctionMultiDim::evaluate /
+ * Math::RealFunctionMultiToMulti::evaluate (include/function.hpp:138,168)
+ * as adapted by BasicLBFGSFunc::value / derivative (include/lbfgs.hpp:91-93)
+ * and Math::LikelihoodWithDerivs::calculate / calculateDeriv
+ * (include/likelihood_function.hpp:39,46).  g may be NULL (value only). */
+int cmbl_value_and_grad(cmbl_map* map, const double* x, double* f, double* g);
+/* Same, device pointers, asynchronous on stream; *d_f is written on device. */
+int cmbl_value_and_grad_device(cmbl_map* map, const double* d_x, double* d_f, double* d_g,
+                               void* stream);
+
+/* Forward half of the autograd pair: map = Y B a(x)  (HEALPix alm2map with
+ * the beam applied, source/cmb_gibbs.cpp:370-378). */
+int cmbl_forward(cmbl_map* map, const double* x, double* out_map);
+int cmbl_forward_device(cmbl_map* map, const double* d_x, double* d_map, void* stream);
+/* Backward half: g = J^T v with J = d map / d x, i.e. map2alm with ring
+ * weight Npix/4pi (exact transpose, source/cmb_gibbs.cpp:390-399) followed
+ * by the beam and the packing: g[l^2] = B_l Re u_l0,
+ * g[l^2+2m-1] = 2 B_l Re u_lm, g[l^2+2m] = 2 B_l Im u_lm. */
+int cmbl_backward(cmbl_map* map, const double* v, double* g);
+int cmbl_backward_device(cmbl_map* map, const double* d_v, double* d_g, void* stream);
+
+/* ---------------------------------------------------------------- batch */
+/* Many independent maps (uneven nside/lmax) evaluated with one launch per
+ * stage over a cost-ordered work list.  The maps must stay alive. */
+int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps);
+void cmbl_batch_destroy(cmbl_batch* batch);
+/* d_x[i], d_g[i]: device pointers for map i; d_f: device array of nmaps. */
+int cmbl_batch_value_and_grad_device(cmbl_batch* batch, const double* const* d_x, double* d_f,
+                                     double* const* d_g, void* stream);
+
+/* ---------------------------------------------------------------- inputs (host) */
+/* Simulate::simulateAlm (source/simulate.cpp:162-192): mt19937(seed) +
+ * std::normal_distribution, l-outer / m-inner draw order; output packed. */
+int cmbl_simulate_alm(const double* cl, int lmax, long seed, double* x_packed);
+/* Simulate::simulateWhiteNoise (source/simulate.cpp:194-203), RING order. */
+int cmbl_simulate_white_noise(int nside, double sigma, long seed, double* out_map);
+/* Utils::beamFunction (source/utils.cpp:54-64) for l = 0..lmax, fwhm in degrees. */
+int cmbl_beam(int lmax, double fwhm_deg, double* beam);
+/* Latitude cut: mask_p = 0 where |pi/2 - theta_p| < cut (source/test_like_high.cpp:123-130). */
+int cmbl_mask_latitude(int nside, double cut_rad, double* mask);
+/* Utils::maskRegions (source/utils.cpp:235-253): mask_p = 0 where the pixel
+ * centre lies within angle_j of (theta_j, phi_j). */
+int cmbl_mask_regions(int nside, int nreg, const double* theta, const double* phi,
+                      const double* angle, double* mask);
+/* w_p = 1/sigma^2 where mask_p > 0.5, else 0 (source/cmb_gibbs.cpp:383-386). */
+int cmbl_weights_from_mask(long npix, const double* mask, double sigma, double* w);
+/* Pixel centre of a RING-scheme pixel (HEALPix pix2ang_ring). */
+int cmbl_pix2ang_ring(int nside, long pix, double* theta, double* phi);
+
+/* ---------------------------------------------------------------- device vectors */
+/* BLAS-1 on device vectors of n doubles, for a device-resident LargeVector
+ * (the operations of BasicLargeVector, source/lbfgs.cpp:323-399).  All are
+ * asynchronous on stream except the reductions, which block and return the
+ * value through a host pointer. */
+int cmbl_vec_copy(double* d_y, const double* d_x, double c, long n, void* stream);   /* y = c x */
+int cmbl_vec_set(double* d_y, double v, long n, void* stream);                       /* y = v */
+int cmbl_vec_axpy(double* d_y, const double* d_x, double c, long n, void* stream);   /* y += c x */
+int cmbl_vec_mul(double* d_y, const double* d_x, long n, void* stream);              /* y *= x */
+int cmbl_vec_div(double* d_y, const double* d_x, long n, void* stream);              /* y /= x */
+int cmbl_vec_pow(double* d_y, double p, long n, void* stream);                       /* y = y^p */
+int cmbl_vec_dot(const double* d_x, const double* d_y, long n, double* out, void* stream);
+/* Device-side dot product: *d_out = x . y, asynchronous. */
+int cmbl_vec_dot_device(const double* d_x, const double* d_y, long n, double* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,211 @@
+"""ctypes binding of the CPU oracle (oracle/cmbl_oracle.c) and, when built,
+of the reference shim (oracle/_ref/libref.so).
+
+TEST INFRASTRUCTURE ONLY.  Importable from tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg; the product package never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_REF = os.path.join(_HERE, "_ref", "libref.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class _Ring(ctypes.Structure):
+    _fields_ = [("z", ctypes.c_double), ("sint", ctypes.c_double), ("phi0", ctypes.c_double),
+                ("nph", ctypes.c_long), ("ofs", ctypes.c_long)]
+
+
+def build(reference="/root/reference"):
+    """Compile the oracle (and the reference shim when the reference is present)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "liboracle.so"], check=True)
+    if os.path.isdir(os.path.join(reference, "include")):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B", "ref", "REFERENCE=" + reference], check=True)
+
+
+def _load():
+    if not os.path.exists(_LIB):
+        build()
+    lib = ctypes.CDLL(_LIB)
+    lib.orc_nring.argtypes = [ctypes.c_int]
+    lib.orc_nring.restype = ctypes.c_int
+    lib.orc_ring_info.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Ring)]
+    lib.orc_pix2ang_ring.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double)]
+    lib.orc_lambda.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp]
+    lib.orc_fft.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int]
+    lib.orc_dft.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
+    lib.orc_synthesis.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_void_p, _dp]
+    lib.orc_adjoint.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_void_p, _dp]
+    lib.orc_neg2lnL.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_void_p]
+    lib.orc_neg2lnL.restype = ctypes.c_double
+    lib.orc_beam.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
+    lib.orc_mask_latitude.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
+    lib.orc_mask_regions.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
+    lib.orc_num_threads.restype = ctypes.c_int
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def nring(nside):
+    return lib().orc_nring(nside)
+
+
+def ring_info(nside, ring):
+    r = _Ring()
+    lib().orc_ring_info(nside, ring, ctypes.byref(r))
+    return dict(z=r.z, sint=r.sint, phi0=r.phi0, nph=r.nph, ofs=r.ofs)
+
+
+def pix2ang_ring(nside, pix):
+    t, p = ctypes.c_double(), ctypes.c_double()
+    lib().orc_pix2ang_ring(nside, pix, ctypes.byref(t), ctypes.byref(p))
+    return t.value, p.value
+
+
+def lam(L, m, x, sint):
+    out = np.zeros(L - m + 1)
+    lib().orc_lambda(L, m, x, sint, out)
+    return out
+
+
+def fft(z, sign):
+    re = np.ascontiguousarray(np.real(z), dtype=np.float64).copy()
+    im = np.ascontiguousarray(np.imag(z), dtype=np.float64).copy()
+    lib().orc_fft(len(re), re, im, sign)
+    return re + 1j * im
+
+
+def dft(z, sign):
+    re = np.ascontiguousarray(np.real(z), dtype=np.float64)
+    im = np.ascontiguousarray(np.imag(z), dtype=np.float64)
+    ore, oim = np.zeros_like(re), np.zeros_like(im)
+    lib().orc_dft(len(re), re, im, ore, oim, sign)
+    return ore + 1j * oim
+
+
+def synthesis(nside, L, x, beam=None):
+    out = np.zeros(12 * nside * nside)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    b = None if beam is None else np.ascontiguousarray(beam, dtype=np.float64)
+    lib().orc_synthesis(nside, L, x, _ptr(b), out)
+    return out
+
+
+def adjoint(nside, L, m, beam=None):
+    out = np.zeros((L + 1) ** 2)
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    b = None if beam is None else np.ascontiguousarray(beam, dtype=np.float64)
+    lib().orc_adjoint(nside, L, m, _ptr(b), out)
+    return out
+
+
+def neg2lnL(nside, L, x, beam, cl, d, w, grad=True):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (beam, cl, d, w)]
+    g = np.zeros((L + 1) ** 2) if grad else None
+    f = lib().orc_neg2lnL(nside, L, x, *args, _ptr(g))
+    return (f, g) if grad else f
+
+
+def beam(L, fwhm_deg):
+    out = np.zeros(L + 1)
+    lib().orc_beam(L, fwhm_deg, out)
+    return out
+
+
+def mask_latitude(nside, cut_rad, mask=None):
+    mask = np.ones(12 * nside * nside) if mask is None else np.ascontiguousarray(mask, dtype=np.float64)
+    lib().orc_mask_latitude(nside, cut_rad, mask)
+    return mask
+
+
+def mask_regions(nside, theta, phi, angle, mask=None):
+    mask = np.ones(12 * nside * nside) if mask is None else np.ascontiguousarray(mask, dtype=np.float64)
+    th, ph, an = (np.ascontiguousarray(a, dtype=np.float64) for a in (theta, phi, angle))
+    lib().orc_mask_regions(nside, len(th), th, ph, an, mask)
+    return mask
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+# ---------------------------------------------------------------- reference shim
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(_REF)
+
+
+def ref():
+    """The reference's own header-only numerics (only in the build container)."""
+    global _ref
+    if _ref is None:
+        r = ctypes.CDLL(_REF)
+        r.ref_sh.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        r.ref_legendre.argtypes = [ctypes.c_int, ctypes.c_double]
+        r.ref_legendre.restype = ctypes.c_double
+        r.ref_assoc_legendre.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        r.ref_assoc_legendre.restype = ctypes.c_double
+        r.ref_gauss_draws.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_long, _dp]
+        r.ref_uniform_draws.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_long, _dp]
+        r.ref_lambda_table.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
+        r.ref_brute_synthesis.argtypes = [ctypes.c_int, _dp, ctypes.c_void_p, ctypes.c_long, _dp, _dp, _dp]
+        _ref = r
+    return _ref
+
+
+def ref_sh(l, m, theta, phi):
+    re, im = ctypes.c_double(), ctypes.c_double()
+    ref().ref_sh(l, m, theta, phi, ctypes.byref(re), ctypes.byref(im))
+    return complex(re.value, im.value)
+
+
+def ref_lambda_table(L, theta):
+    out = np.zeros((L + 1) * (L + 2) // 2)
+    ref().ref_lambda_table(L, theta, out)
+    return out
+
+
+def ref_brute_synthesis(L, x, beam, theta, phi):
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    out = np.zeros(len(theta))
+    b = None if beam is None else np.ascontiguousarray(beam, dtype=np.float64)
+    ref().ref_brute_synthesis(L, np.ascontiguousarray(x, dtype=np.float64), _ptr(b), len(theta),
+                              theta, phi, out)
+    return out
+
+
+def ref_gauss(seed, mean, sigma, n):
+    out = np.zeros(n)
+    ref().ref_gauss_draws(seed, mean, sigma, n, out)
+    return out
+
+
+def ref_uniform(seed, lo, hi, n):
+    out = np.zeros(n)
+    ref().ref_uniform_draws(seed, lo, hi, n, out)
+    return out

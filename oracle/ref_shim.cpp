@@ -1,0 +1,95 @@
+// Thin C entry points over the reference's own header-only numerics, so the
+// tests can pin the oracle against the reference code itself.
+// TEST INFRASTRUCTURE ONLY: built by oracle/Makefile (target `ref`) from the
+// headers where they lie under /root/reference/include, into oracle/_ref/.
+// Nothing here is part of the product.
+//
+// Reference code exercised:
+//   Math::SphericalHarmonics::calculate   include/spherical_harmonics.hpp:29-37
+//   Math::Legendre / AssociatedLegendre   include/legendre.hpp:26-103
+//   Math::GaussianGenerator               include/random.hpp:32-52
+//   Math::UniformRealGenerator            include/random.hpp:10-30
+#include <cmath>
+#include <vector>
+
+#include <spherical_harmonics.hpp>
+#include <legendre.hpp>
+#include <random.hpp>
+
+extern "C" {
+
+void ref_sh(int l, int m, double theta, double phi, double* re, double* im)
+{
+    Math::SphericalHarmonics sh;
+    const ComplexDouble v = sh.calculate(l, m, theta, phi);
+    *re = v.real();
+    *im = v.imag();
+}
+
+double ref_legendre(int l, double x)
+{
+    Math::Legendre p;
+    return p.calculate(l, x);
+}
+
+double ref_assoc_legendre(int l, int m, double x)
+{
+    Math::AssociatedLegendre p;
+    return p.calculate(l, m, x);
+}
+
+void ref_gauss_draws(int seed, double mean, double sigma, long n, double* out)
+{
+    Math::GaussianGenerator g(seed, mean, sigma);
+    for (long i = 0; i < n; ++i) out[i] = g.generate();
+}
+
+void ref_uniform_draws(int seed, double lo, double hi, long n, double* out)
+{
+    Math::UniformRealGenerator g(seed, lo, hi);
+    for (long i = 0; i < n; ++i) out[i] = g.generate();
+}
+
+// lambda_lm(theta) = Re Y_lm(theta, 0) for 0 <= m <= l <= L, index l(l+1)/2 + m
+void ref_lambda_table(int L, double theta, double* out)
+{
+    Math::SphericalHarmonics sh;
+    for (int l = 0; l <= L; ++l)
+        for (int m = 0; m <= l; ++m)
+            out[(long)l * (l + 1) / 2 + m] = sh.calculate(l, m, theta, 0.0).real();
+}
+
+// Brute-force real-map synthesis at arbitrary points:
+//   s = sum_l [ a_l0 Y_l0 + 2 sum_{m>=1} Re(a_lm Y_lm) ],  a from the packed
+// almToVector layout, each Y_lm from the reference SphericalHarmonics.
+// Points must be grouped by theta (consecutive equal theta reuse the table).
+void ref_brute_synthesis(int L, const double* x_packed, const double* beam, long npts,
+                         const double* theta, const double* phi, double* out)
+{
+    std::vector<double> lam((size_t)(L + 1) * (L + 2) / 2);
+    double last = -1.0;
+    for (long p = 0; p < npts; ++p) {
+        if (theta[p] != last) {
+            ref_lambda_table(L, theta[p], lam.data());
+            last = theta[p];
+        }
+        long double s = 0;
+        for (int m = 0; m <= L; ++m) {
+            const double c = std::cos(m * phi[p]), sn = std::sin(m * phi[p]);
+            for (int l = m; l <= L; ++l) {
+                const double y = lam[(size_t)l * (l + 1) / 2 + m];
+                const double b = beam ? beam[l] : 1.0;
+                if (m == 0) {
+                    s += (long double)b * x_packed[(long)l * l] * y;
+                } else {
+                    const double re = x_packed[(long)l * l + 2 * m - 1];
+                    const double im = x_packed[(long)l * l + 2 * m];
+                    s += 2.0L * b * y * (re * c - im * sn);
+                }
+            }
+        }
+        out[p] = (double)s;
+    }
+}
+
+}  // extern "C"

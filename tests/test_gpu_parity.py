@@ -1,0 +1,149 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bar: -2lnL within 1e-10 relative, maps / gradients within 1e-10 norm-wise
+relative (||d|| / ||ref||), gradient vs central finite differences within
+1e-6 relative.  Sizes are chosen so the oracle finishes in seconds.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cosmopp_amd as cm
+import oracle
+from cases import CONFIG1, CONFIG2, Case
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+# (nside, lmax): regular, lmax = 4 nside (every ring aliased), lmax < nside, tiny maps
+GEOMS = [(1, 2), (1, 4), (2, 8), (4, 5), (8, 32), (16, 32), (32, 20), (64, 128), (128, 300)]
+
+
+@pytest.mark.parametrize("nside,lmax", GEOMS)
+def test_forward_matches_oracle(nside, lmax):
+    print("case forward nside=%d lmax=%d" % (nside, lmax), flush=True)
+    rng = np.random.default_rng(nside + 7 * lmax)
+    x = rng.standard_normal((lmax + 1) ** 2)
+    b = cm.beam(lmax, 3.0)
+    mp = cm.Map(nside, lmax)
+    mp.set_beam(b)
+    s = mp.forward(x)
+    ref = oracle.synthesis(nside, lmax, x, b)
+    assert rel(s, ref) < TOL
+    mp.close()
+
+
+@pytest.mark.parametrize("nside,lmax", GEOMS)
+def test_backward_matches_oracle(nside, lmax):
+    print("case backward nside=%d lmax=%d" % (nside, lmax), flush=True)
+    rng = np.random.default_rng(3 * nside + lmax)
+    v = rng.standard_normal(12 * nside * nside)
+    b = cm.beam(lmax, 3.0)
+    mp = cm.Map(nside, lmax)
+    mp.set_beam(b)
+    g = mp.backward(v)
+    ref = oracle.adjoint(nside, lmax, v, b)
+    assert rel(g, ref) < TOL
+    # adjoint identity on the device pair
+    x = rng.standard_normal((lmax + 1) ** 2)
+    lhs = np.dot(mp.forward(x), v)
+    rhs = np.dot(x, g)
+    assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(x) * np.linalg.norm(v) * math.sqrt(x.size)
+    mp.close()
+
+
+@pytest.mark.parametrize("nside,lmax,cut", [(16, 32, 20.0), (64, 128, 20.0), (32, 128, 10.0), (128, 256, 30.0)])
+def test_value_and_grad_matches_oracle(nside, lmax, cut):
+    print("case like nside=%d lmax=%d" % (nside, lmax), flush=True)
+    c = Case(nside, lmax, 2.0, 10.0, cut, seed=nside + lmax)
+    mp = c.make_map()
+    f, g = mp.value_and_grad(c.x)
+    fo, go = oracle.neg2lnL(nside, lmax, c.x, c.beam, c.cl, c.d, c.w)
+    assert abs(f - fo) / abs(fo) < TOL
+    assert rel(g, go) < TOL
+    # deterministic: bitwise identical on repeat
+    f2, g2 = mp.value_and_grad(c.x)
+    assert f2 == f and np.array_equal(g2, g)
+    mp.close()
+
+
+def test_config1_parity():
+    """Plain method, CPU-runnable configuration: nside 64, lmax 128."""
+    print("case config1", flush=True)
+    c = Case(**CONFIG1, seed=1)
+    mp = c.make_map()
+    f, g = mp.value_and_grad(c.x)
+    fo, go = oracle.neg2lnL(c.nside, c.lmax, c.x, c.beam, c.cl, c.d, c.w)
+    assert abs(f - fo) / abs(fo) < TOL
+    assert rel(g, go) < TOL
+    mp.close()
+
+
+@pytest.fixture(scope="module")
+def config2():
+    print("case config2 setup", flush=True)
+    c = Case(**CONFIG2, seed=2)
+    mp = c.make_map()
+    yield c, mp
+    mp.close()
+
+
+def test_config2_parity(config2):
+    """nside 512, lmax 1024, 200 point-source holes."""
+    c, mp = config2
+    print("case config2 parity", flush=True)
+    f, g = mp.value_and_grad(c.x)
+    fo, go = oracle.neg2lnL(c.nside, c.lmax, c.x, c.beam, c.cl, c.d, c.w)
+    assert abs(f - fo) / abs(fo) < TOL
+    assert rel(g, go) < TOL
+    s = mp.forward(c.x)
+    assert rel(s, oracle.synthesis(c.nside, c.lmax, c.x, c.beam)) < TOL
+
+
+def test_config2_finite_differences(config2):
+    """Gradient vs central differences (step 1e-6 relative) on seeded random directions."""
+    c, mp = config2
+    ndir = 200
+    print("case config2 fd ndir=%d" % ndir, flush=True)
+    f, g = mp.value_and_grad(c.x)
+    rng = np.random.default_rng(77)
+    worst = 0.0
+    for _ in range(ndir):
+        v = rng.standard_normal(c.x.size)
+        h = 1e-6 * np.linalg.norm(c.x) / np.linalg.norm(v)
+        fp = mp.value_and_grad(c.x + h * v, grad=False)
+        fm = mp.value_and_grad(c.x - h * v, grad=False)
+        fd = (fp - fm) / (2 * h)
+        gv = np.dot(g, v)
+        worst = max(worst, abs(fd - gv) / abs(gv))
+    print("config2 fd worst rel %.3e" % worst, flush=True)
+    assert worst < 1e-6
+
+
+def test_nonfinite_input_is_reported():
+    c = Case(8, 16, 2.0, 10.0, 20.0, seed=3)
+    mp = c.make_map()
+    x = c.x.copy()
+    x[5] = np.nan
+    with pytest.raises(cm.CmblError) as e:
+        mp.value_and_grad(x)
+    assert e.value.code == cm.CMBL_ERR_NONFINITE
+    # the context stays usable
+    f, _ = mp.value_and_grad(c.x)
+    assert np.isfinite(f)
+    mp.close()
+
+
+def test_state_errors():
+    mp = cm.Map(4, 8)
+    with pytest.raises(cm.CmblError) as e:
+        mp.value_and_grad(np.zeros(81))
+    assert e.value.code == cm.CMBL_ERR_STATE
+    mp.close()

@@ -1,0 +1,222 @@
+"""Benchmark: map-level -2lnL + gradient evaluations per second, FP64.
+
+Workload (default): nside=1024, lmax=2048 (4,198,401 packed a_lm, 12,582,912
+pixels), Gaussian beam FWHM 10 arcmin, white noise sigma 20 uK, 20 degree
+latitude cut, C_l = 1e3/(l(l+1)).  One step = one likelihood + gradient
+evaluation (cmbl_value_and_grad_device) with x, the data, the weights and all
+tables already resident in HBM.
+
+Prints ONE JSON line.  `roofline` is for the dominant kernel (the slower of the
+two FP64 Legendre kernels), with algorithmic work 8 * R * Nalm FLOP per launch
+(R = 2*nside ring pairs, Nalm = (lmax+1)(lmax+2)/2; 4 FLOP for the recurrence
+step and 4 for the complex accumulation per (l, m, ring pair)), timed with HIP
+events on the evaluation stream during the timed region.  `cpu_baseline` times
+the CPU oracle (C FP64 restatement, OpenMP) on one evaluation of the same
+workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "likelihood+gradient evals/sec at Nside=1024,lmax=2048 FP64; FP64 roofline %"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    1: dict(nside=64, lmax=128, fwhm_deg=2.0, sigma=10.0, cut_deg=20.0),
+    2: dict(nside=512, lmax=1024, fwhm_deg=20.0 / 60.0, sigma=30.0, cut_deg=20.0),
+    4: dict(nside=1024, lmax=2048, fwhm_deg=10.0 / 60.0, sigma=20.0, cut_deg=20.0),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_case(cfg):
+    """Inputs of the workload; the data map is synthesised on the GPU."""
+    import numpy as np
+
+    import cosmopp_amd as cm
+
+    nside, lmax = cfg["nside"], cfg["lmax"]
+    cl = cm.cl_shape(lmax)
+    b = cm.beam(lmax, cfg["fwhm_deg"])
+    a_true = cm.simulate_alm(cl, lmax, 101)
+    mp = cm.Map(nside, lmax)
+    mp.set_beam(b)
+    mp.set_cl(cl)
+    s = mp.forward(a_true)
+    d = s + cm.simulate_white_noise(nside, cfg["sigma"], 102)
+    mask = cm.mask_latitude(nside, math.radians(cfg["cut_deg"]))
+    w = cm.weights_from_mask(mask, cfg["sigma"])
+    mp.set_data(d, w)
+    x = a_true + 0.1 * cm.simulate_alm(cl, lmax, 103)
+    return mp, dict(cl=cl, beam=b, d=d, w=w, x=x)
+
+
+def read_traffic(nside, lmax, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        t = json.load(open(path))
+        key = "%s_n%d_l%d" % (kernel, nside, lmax)
+        return t.get(key)
+    except Exception:
+        return None
+
+
+def cpu_baseline(cfg, inputs, budget_s):
+    """Time the oracle (CPU FP64 restatement) on the same workload."""
+    import oracle
+
+    nside, lmax = cfg["nside"], cfg["lmax"]
+    threads = oracle.num_threads()
+    times = []
+    t_start = time.time()
+    while True:
+        t0 = time.perf_counter()
+        oracle.neg2lnL(nside, lmax, inputs["x"], inputs["beam"], inputs["cl"], inputs["d"], inputs["w"])
+        times.append(time.perf_counter() - t0)
+        if time.time() - t_start + times[-1] > budget_s or len(times) >= 5:
+            break
+    times.sort()
+    med = times[len(times) // 2]
+    return {
+        "value": 1.0 / med,
+        "unit": "evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d full value+gradient evaluation(s) of the same workload (median %.2f s), "
+                  "CPU oracle oracle/cmbl_oracle.c, %d OpenMP threads" % (len(times), med, threads),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=4, choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=40.0, help="seconds for the CPU baseline leg")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box in this setup has a single MI355X, and the "
+              "multi-GPU layout (independent maps per GPU, m-sharded single maps) is described in "
+              "DESIGN.md but not run.  Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import cosmopp_amd as cm
+
+    cfg = CONFIGS[args.config]
+    nside, lmax = cfg["nside"], cfg["lmax"]
+    log("bench: building inputs nside=%d lmax=%d" % (nside, lmax))
+    mp, inputs = build_case(cfg)
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(inputs["x"]).to(dev)
+    g = torch.empty_like(x)
+    f = torch.empty(1, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    log("bench: warmup %d" % args.warmup)
+    for _ in range(args.warmup):
+        mp.value_and_grad_device(x.data_ptr(), f.data_ptr(), g.data_ptr(), sp)
+    torch.cuda.synchronize()
+
+    mp.set_timing(True)
+    log("bench: timing %d steps" % args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mp.value_and_grad_device(x.data_ptr(), f.data_ptr(), g.data_ptr(), sp)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stages, nev = mp.stage_times()
+    mp.set_timing(False)
+    fval = f.item()
+    if not math.isfinite(fval):
+        raise RuntimeError("non-finite -2lnL")
+
+    elapsed = t1 - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    value = args.steps / elapsed
+    stage_avg = {k: v / max(nev, 1) for k, v in stages.items()}
+
+    R = 2 * nside
+    nalm = (lmax + 1) * (lmax + 2) // 2
+    flop_per_launch = 8.0 * R * nalm
+    dom = "adjoint" if stage_avg["adjoint"] >= stage_avg["synthesis"] else "synthesis"
+    t_dom = stage_avg[dom] * 1e-3
+    achieved = flop_per_launch / t_dom / 1e12
+    kname = "k_adj" if dom == "adjoint" else "k_synth"
+    roofline = {
+        "bound": "mfma",
+        "compute": "FP64 VALU (MI355X FP64 vector peak equals its FP64 matrix peak, 78.6 TFLOP/s)",
+        "kernel": "%s (Legendre %s)" % (kname, dom),
+        "achieved": round(achieved, 3),
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+        "traffic": read_traffic(nside, lmax, kname),
+        "flop_per_launch": flop_per_launch,
+        "avg_launch_ms": round(stage_avg[dom], 4),
+        "legendre_pair_frac": round(2 * flop_per_launch / ((stage_avg["adjoint"] + stage_avg["synthesis"]) * 1e-3)
+                                    / 1e12 / FP64_PEAK_TFLOPS, 4),
+    }
+    ring_bytes = 32.0 * (4 * nside - 1) * (lmax + 1) * 2 + 16.0 * 12 * nside * nside
+    roofline["ring_stage_hbm_frac"] = round(ring_bytes / (stage_avg["ring"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline (oracle)")
+        cpu = cpu_baseline(cfg, inputs, args.cpu_budget)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "map-level -2lnL + gradient (synthesis, ring DFTs, weighting, adjoint), "
+                        "nside=%d lmax=%d FWHM=%.1f arcmin sigma=%g uK, %g deg latitude cut"
+                        % (nside, lmax, cfg["fwhm_deg"] * 60, cfg["sigma"], cfg["cut_deg"]),
+            "nside": nside,
+            "lmax": lmax,
+            "nparams": (lmax + 1) ** 2,
+            "npix": 12 * nside * nside,
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stages_ms": {k: round(v, 4) for k, v in stage_avg.items()},
+        "neg2lnL": fval,
+    }
+    mp.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

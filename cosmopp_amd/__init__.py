@@ -61,6 +61,8 @@ _SIGS = {
     "cmbl_forward_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
     "cmbl_backward": ([_vp, _vp, _vp], ctypes.c_int),
     "cmbl_backward_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_map_set_timing": ([_vp, ctypes.c_int], ctypes.c_int),
+    "cmbl_map_stage_times": ([_vp, _dp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
     "cmbl_batch_create": ([ctypes.POINTER(_vp), _vp, ctypes.c_int], ctypes.c_int),
     "cmbl_batch_destroy": ([_vp], None),
     "cmbl_batch_value_and_grad_device": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
@@ -71,6 +73,13 @@ _SIGS = {
     "cmbl_mask_regions": ([ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp], ctypes.c_int),
     "cmbl_weights_from_mask": ([ctypes.c_long, _vp, ctypes.c_double, _vp], ctypes.c_int),
     "cmbl_pix2ang_ring": ([ctypes.c_int, ctypes.c_long, _dp, _dp], ctypes.c_int),
+    "cmbl_whitenoise": ([ctypes.c_int, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
+    "cmbl_device_alloc": ([ctypes.POINTER(_vp), ctypes.c_ulong], ctypes.c_int),
+    "cmbl_device_free": ([_vp], ctypes.c_int),
+    "cmbl_copy_to_device": ([_vp, _vp, ctypes.c_ulong], ctypes.c_int),
+    "cmbl_copy_to_host": ([_vp, _vp, ctypes.c_ulong], ctypes.c_int),
+    "cmbl_copy_device": ([_vp, _vp, ctypes.c_ulong, _vp], ctypes.c_int),
+    "cmbl_stream_synchronize": ([_vp], ctypes.c_int),
     "cmbl_vec_copy": ([_vp, _vp, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
     "cmbl_vec_set": ([_vp, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
     "cmbl_vec_axpy": ([_vp, _vp, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
@@ -167,6 +176,13 @@ def pix2ang_ring(nside, pix):
     return t.value, p.value
 
 
+def whitenoise(seed, amplitude, n):
+    """BasicLBFGSFunc::whitenoise for one process (host array)."""
+    out = np.zeros(n)
+    _check(lib().cmbl_whitenoise(seed, amplitude, n, _p(out)))
+    return out
+
+
 def cl_shape(lmax, amp=1e3):
     """C_l = amp / (l (l+1)) for l >= 2, C_0 = C_1 = 0."""
     cl = np.zeros(lmax + 1)
@@ -246,6 +262,19 @@ class Map:
 
     def backward_device(self, v_ptr, g_ptr, stream=0):
         _check(lib().cmbl_backward_device(self._h, v_ptr, g_ptr, stream))
+
+    STAGES = ("pre", "synthesis", "ring", "adjoint", "post", "final")
+
+    def set_timing(self, on=True):
+        """Enable (and reset) or disable HIP-event timing of the evaluation stages."""
+        _check(lib().cmbl_map_set_timing(self._h, 1 if on else 0))
+
+    def stage_times(self):
+        """(dict stage -> accumulated ms, number of timed evaluations)."""
+        ms = (ctypes.c_double * 6)()
+        n = ctypes.c_long()
+        _check(lib().cmbl_map_stage_times(self._h, ms, ctypes.byref(n)))
+        return dict(zip(self.STAGES, list(ms))), n.value
 
 
 class Batch:

@@ -39,6 +39,7 @@ constexpr int kRingThreads = 256;
 constexpr int kMaxFFT = 8192;          // largest LDS FFT (complex doubles)
 constexpr int kRedChunk = 8;           // l-steps per cross-lane reduction in the adjoint
 constexpr int kRedStride = 17;         // doubles per lane row in the reduction buffer
+constexpr int kChunk = 64;             // l-steps per LDS-staged coefficient chunk
 
 struct MapDesc {
     int nside, L, npair, npair_pad, nring;
@@ -56,6 +57,14 @@ struct MapDesc {
     // Legendre tables
     const double* seed_v;     // [(L+1) * npair_pad] scaled lambda_mm
     const int* seed_k;        // [(L+1) * npair_pad] scale count
+    // recurrence state at each work item's first live l, per stage layout:
+    // (mu_{l-1}, mu_l) scaled, and the scale count; [(L+1) * npair_pad]
+    const double* s1_prev;
+    const double* s1_cur;
+    const int* s1_k;
+    const double* s5_prev;
+    const double* s5_cur;
+    const int* s5_k;
     const double* A;          // [nalm] recurrence coefficient of mu_l = A x mu_{l-1} - mu_{l-2}
     const double* Nrm;        // [nalm] lambda_lm = Nrm * mu_lm
     const double* beam;       // [L+1]
@@ -73,11 +82,20 @@ struct MapDesc {
     int nprior;
 };
 
+// One Legendre work item: degree m of one ring block of one map, starting at
+// the first l at which any of its ring pairs is live.  mcnt = 0 marks padding.
 struct WorkItem {
     int map;
     int m0;
     int mcnt;
     int rb;
+    int lfirst;
+    int pad_;
+};
+
+struct ItemCost {
+    WorkItem it;
+    double cost;
 };
 
 struct RingItem {
@@ -112,6 +130,8 @@ int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_b
                            const double2* d_tw, hipStream_t s);
 int launch_scan(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, int* d_lstart,
                 hipStream_t s);
+int launch_capture(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, double* d_prev, double* d_cur,
+                   int* d_k, hipStream_t s);
 
 // BLAS-1
 int launch_vec(int op, double* y, const double* x, double c, long n, hipStream_t s);
@@ -134,7 +154,8 @@ struct cmbl_map {
     cmbl::WorkItem* d_items5 = nullptr;
     cmbl::RingItem* d_ring_items = nullptr;
     int nitems1 = 0, nitems5 = 0, nring_items = 0;
-    std::vector<cmbl::WorkItem> items1, items5;
+    std::vector<cmbl::WorkItem> items1, items5;   // launch order (XCD-interleaved)
+    std::vector<cmbl::ItemCost> cost1, cost5;      // live items with cost estimates
     std::vector<cmbl::RingItem> ring_items;
     double2* d_tw = nullptr;
     double* d_f = nullptr;
@@ -146,6 +167,13 @@ struct cmbl_map {
     double* d_invcl = nullptr;
     double* d_data = nullptr;
     double* d_w = nullptr;
+    // stage timing: a pool of event sets recorded without host synchronisation
+    bool timing = false;
+    std::vector<hipEvent_t> ev;  // kEvSets * (CMBL_NSTAGES + 1)
+    int ev_next = 0, ev_pending = 0;
+    double stage_ms[CMBL_NSTAGES] = {0, 0, 0, 0, 0, 0};
+    long timed_evals = 0;
+    static constexpr int kEvSets = 64;
 };
 
 struct cmbl_batch {

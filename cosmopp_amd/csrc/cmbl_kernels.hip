@@ -26,16 +26,6 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b)  // a * conj(b)
 {
     return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
-// Wave-uniform read-only tables are read through the constant address space
-// so that the compiler emits scalar loads (s_load) and feeds SGPR operands.
-template <typename T>
-using cptr = const __attribute__((address_space(4))) T*;
-template <typename T>
-__device__ __forceinline__ cptr<T> as_const(const T* p)
-{
-    return (cptr<T>)p;
-}
-
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
@@ -98,13 +88,17 @@ __global__ __launch_bounds__(256) void k_pre(const MapDesc* __restrict__ md_, co
 
 // ================================================================== Legendre stages
 //
-// One wavefront per work item (map, m0..m0+mcnt-1, ring block rb).  Lane
-// `lane` handles ring pairs p_k = rb*64*K + k*64 + lane, k < K.  The
-// recurrence mu_l = A_lm x mu_{l-1} - mu_{l-2} runs in registers; A_lm and
-// the a_lm coefficients are wave-uniform and come through scalar loads.
-// Lanes whose scale count is > 0 are negligible (|lambda| < 2^-100): the
-// wave runs a "scaled" loop with rescale checks until every lane is live,
-// then a check-free loop.
+// One wavefront (= one workgroup) per work item: degree m of one ring block
+// of one map.  Lane `lane` handles ring pairs p_k = rb*64*K + k*64 + lane,
+// k < K.  The recurrence mu_l = A_lm x mu_{l-1} - mu_{l-2} runs in registers,
+// starting from the state precomputed at set-up for the item's first live l.
+// The wave-uniform coefficients (A_lm and, for the synthesis, B_l N_lm a_lm)
+// are staged through a per-wave LDS double buffer, kChunk l-steps at a time:
+// every lane issues one vector load of the next chunk before the current
+// chunk is processed, so the loads overlap kChunk steps of FP64 work.
+// Lanes whose scale count is > 0 are negligible (|lambda| < 2^-100): the wave
+// runs a "scaled" path with rescale checks until every lane is live, then a
+// check-free path of 4 FP64 VALU operations per (l, ring pair).
 
 template <int K>
 __device__ __forceinline__ bool all_live(const int (&sc)[K])
@@ -141,171 +135,190 @@ __device__ __forceinline__ bool rescale(double (&mu1)[K], double (&mu2)[K], int 
     return true;
 }
 
-// -------------------------------------------------------------- synthesis
-template <int K, bool ODD>
-__device__ __forceinline__ void syn_step(double Al, double ar, double ai, const double (&x)[K], double (&mu1)[K],
-                                         double (&mu2)[K], double (&er)[K], double (&ei)[K],
-                                         double (&orr)[K], double (&oi)[K])
-{
+// Phase state shared by both Legendre kernels.
+template <int K>
+struct LegState {
+    double x[K];
+    double mu1[K], mu2[K];  // mu_l, mu_{l-1}
+    double gate[K];         // 1 for live lanes, 0 otherwise (scaled path only)
+    int sc[K];
+    bool live, all;
+    __device__ __forceinline__ void refresh()
+    {
+        live = any_live<K>(sc);
+        all = all_live<K>(sc);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const double nu = fma(Al * x[k], mu1[k], -mu2[k]);
-        mu2[k] = mu1[k];
-        mu1[k] = nu;
-        if (ODD) {
-            orr[k] = fma(nu, ar, orr[k]);
-            oi[k] = fma(nu, ai, oi[k]);
-        } else {
-            er[k] = fma(nu, ar, er[k]);
-            ei[k] = fma(nu, ai, ei[k]);
-        }
+        for (int k = 0; k < K; ++k) gate[k] = sc[k] == 0 ? 1.0 : 0.0;
     }
-}
-
-// One step of the scaled region: recurrence for every lane, accumulation of
-// the live lanes only (gate = 1 for live lanes, 0 otherwise).
-template <int K, bool ODD>
-__device__ __forceinline__ void syn_scaled_step(double Al, double ar, double ai, bool live, const double (&gate)[K],
-                                                const double (&x)[K], double (&mu1)[K], double (&mu2)[K],
-                                                double (&er)[K], double (&ei)[K], double (&orr)[K], double (&oi)[K])
-{
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const double nu = fma(Al * x[k], mu1[k], -mu2[k]);
-        mu2[k] = mu1[k];
-        mu1[k] = nu;
-    }
-    if (live) {
+    __device__ __forceinline__ void advance(double Al)
+    {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const double f = gate[k] * mu1[k];
-            if (ODD) {
-                orr[k] = fma(f, ar, orr[k]);
-                oi[k] = fma(f, ai, oi[k]);
-            } else {
-                er[k] = fma(f, ar, er[k]);
-                ei[k] = fma(f, ai, ei[k]);
-            }
+            const double nu = fma(Al * x[k], mu1[k], -mu2[k]);
+            mu2[k] = mu1[k];
+            mu1[k] = nu;
+        }
+    }
+};
+
+// -------------------------------------------------------------- synthesis
+template <int K>
+struct SynAcc {
+    double er[K], ei[K], orr[K], oi[K];
+};
+
+template <int K, bool ODD>
+__device__ __forceinline__ void syn_live(LegState<K>& st, SynAcc<K>& ac, double Al, double ar, double ai)
+{
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double nu = fma(Al * st.x[k], st.mu1[k], -st.mu2[k]);
+        st.mu2[k] = st.mu1[k];
+        st.mu1[k] = nu;
+        if (ODD) {
+            ac.orr[k] = fma(nu, ar, ac.orr[k]);
+            ac.oi[k] = fma(nu, ai, ac.oi[k]);
+        } else {
+            ac.er[k] = fma(nu, ar, ac.er[k]);
+            ac.ei[k] = fma(nu, ai, ac.ei[k]);
         }
     }
 }
 
-// Four live steps l+1 .. l+4 with parity pattern starting at ODD0, with the
-// coefficients of the next four steps prefetched into SGPRs.
-template <int K, bool ODD0>
-__device__ __forceinline__ void syn_run(int& l, int L, cptr<double> A, cptr<double> a, const double (&x)[K],
-                                        double (&mu1)[K], double (&mu2)[K], double (&er)[K], double (&ei)[K],
-                                        double (&orr)[K], double (&oi)[K])
+template <int K, bool ODD>
+__device__ __forceinline__ void syn_gated(LegState<K>& st, SynAcc<K>& ac, double ar, double ai)
 {
-    if (l + 4 <= L) {
-        double c0 = A[l + 1], c1 = A[l + 2], c2 = A[l + 3], c3 = A[l + 4];
-        double r0 = a[2 * (l + 1)], i0 = a[2 * (l + 1) + 1], r1 = a[2 * (l + 2)], i1 = a[2 * (l + 2) + 1];
-        double r2 = a[2 * (l + 3)], i2 = a[2 * (l + 3) + 1], r3 = a[2 * (l + 4)], i3 = a[2 * (l + 4) + 1];
-        for (; l + 8 <= L; l += 4) {
-            const double n0 = A[l + 5], n1 = A[l + 6], n2 = A[l + 7], n3 = A[l + 8];
-            const double s0 = a[2 * (l + 5)], t0 = a[2 * (l + 5) + 1], s1 = a[2 * (l + 6)], t1 = a[2 * (l + 6) + 1];
-            const double s2 = a[2 * (l + 7)], t2 = a[2 * (l + 7) + 1], s3 = a[2 * (l + 8)], t3 = a[2 * (l + 8) + 1];
-            syn_step<K, ODD0>(c0, r0, i0, x, mu1, mu2, er, ei, orr, oi);
-            syn_step<K, !ODD0>(c1, r1, i1, x, mu1, mu2, er, ei, orr, oi);
-            syn_step<K, ODD0>(c2, r2, i2, x, mu1, mu2, er, ei, orr, oi);
-            syn_step<K, !ODD0>(c3, r3, i3, x, mu1, mu2, er, ei, orr, oi);
-            c0 = n0; c1 = n1; c2 = n2; c3 = n3;
-            r0 = s0; i0 = t0; r1 = s1; i1 = t1; r2 = s2; i2 = t2; r3 = s3; i3 = t3;
+    if (!st.live) return;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double f = st.gate[k] * st.mu1[k];
+        if (ODD) {
+            ac.orr[k] = fma(f, ar, ac.orr[k]);
+            ac.oi[k] = fma(f, ai, ac.oi[k]);
+        } else {
+            ac.er[k] = fma(f, ar, ac.er[k]);
+            ac.ei[k] = fma(f, ai, ac.ei[k]);
         }
-        syn_step<K, ODD0>(c0, r0, i0, x, mu1, mu2, er, ei, orr, oi);
-        syn_step<K, !ODD0>(c1, r1, i1, x, mu1, mu2, er, ei, orr, oi);
-        syn_step<K, ODD0>(c2, r2, i2, x, mu1, mu2, er, ei, orr, oi);
-        syn_step<K, !ODD0>(c3, r3, i3, x, mu1, mu2, er, ei, orr, oi);
-        l += 4;
     }
-    // tail (< 4 steps); the parity of step l+1 is ODD0 again since 4 is even
-    bool odd = ODD0;
-    while (l < L) {
-        ++l;
-        if (odd)
-            syn_step<K, true>(A[l], a[2 * l], a[2 * l + 1], x, mu1, mu2, er, ei, orr, oi);
+}
+
+// Steps l = l .. cend of one staged chunk (coefficients cA[l - cb], ca[l - cb]).
+template <int K>
+__device__ __forceinline__ void syn_chunk(LegState<K>& st, SynAcc<K>& ac, int& l, int cend, int cb, int m,
+                                          const double* cA, const double2* ca)
+{
+    while (!st.all && l <= cend) {
+        const double Al = cA[l - cb];
+        const double2 al = ca[l - cb];
+        st.advance(Al);
+        if ((l - m) & 1)
+            syn_gated<K, true>(st, ac, al.x, al.y);
         else
-            syn_step<K, false>(A[l], a[2 * l], a[2 * l + 1], x, mu1, mu2, er, ei, orr, oi);
-        odd = !odd;
+            syn_gated<K, false>(st, ac, al.x, al.y);
+        if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
+        ++l;
+    }
+    if (l > cend) return;
+    // all lanes live: four steps per iteration, parity pattern fixed by (l - m)
+    if ((l - m) & 1) {
+        for (; l + 3 <= cend; l += 4) {
+            const int j = l - cb;
+            syn_live<K, true>(st, ac, cA[j], ca[j].x, ca[j].y);
+            syn_live<K, false>(st, ac, cA[j + 1], ca[j + 1].x, ca[j + 1].y);
+            syn_live<K, true>(st, ac, cA[j + 2], ca[j + 2].x, ca[j + 2].y);
+            syn_live<K, false>(st, ac, cA[j + 3], ca[j + 3].x, ca[j + 3].y);
+        }
+    } else {
+        for (; l + 3 <= cend; l += 4) {
+            const int j = l - cb;
+            syn_live<K, false>(st, ac, cA[j], ca[j].x, ca[j].y);
+            syn_live<K, true>(st, ac, cA[j + 1], ca[j + 1].x, ca[j + 1].y);
+            syn_live<K, false>(st, ac, cA[j + 2], ca[j + 2].x, ca[j + 2].y);
+            syn_live<K, true>(st, ac, cA[j + 3], ca[j + 3].x, ca[j + 3].y);
+        }
+    }
+    for (; l <= cend; ++l) {
+        const int j = l - cb;
+        if ((l - m) & 1)
+            syn_live<K, true>(st, ac, cA[j], ca[j].x, ca[j].y);
+        else
+            syn_live<K, false>(st, ac, cA[j], ca[j].x, ca[j].y);
     }
 }
 
 template <int K>
 __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
 {
+    __shared__ double sA[2][kChunk];
+    __shared__ double2 sa[2][kChunk];
     const WorkItem it = items[blockIdx.x];
+    if (it.mcnt == 0) return;
     const MapDesc& md = maps[it.map];
     const int L = md.L;
     const int npad = md.npair_pad;
     const int lane = threadIdx.x;
-    double x[K];
+    const int m = it.m0;
+    const int l0 = it.lfirst;
+    const long base = moff(L, m) - m;
+    const double* __restrict__ A = md.A + base;
+    const double2* __restrict__ a = md.alm + base;
+
+    LegState<K> st;
+    SynAcc<K> ac;
     int p[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         p[k] = it.rb * 64 * K + k * 64 + lane;
-        x[k] = md.ring_x[p[k]];
+        const long q = (long)m * npad + p[k];
+        st.x[k] = md.ring_x[p[k]];
+        st.mu1[k] = md.s1_cur[q];
+        st.mu2[k] = md.s1_prev[q];
+        st.sc[k] = md.s1_k[q];
+        ac.er[k] = ac.ei[k] = ac.orr[k] = ac.oi[k] = 0.0;
     }
-    for (int mi = 0; mi < it.mcnt; ++mi) {
-        const int m = it.m0 + mi;
-        const long base = moff(L, m) - m;
-        cptr<double> A = as_const(md.A + base);
-        cptr<double> a = as_const(reinterpret_cast<const double*>(md.alm + base));
-        double mu1[K], mu2[K], er[K], ei[K], orr[K], oi[K];
-        int sc[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            mu1[k] = md.seed_v[(long)m * npad + p[k]];
-            sc[k] = md.seed_k[(long)m * npad + p[k]];
-            mu2[k] = 0.0;
-            er[k] = ei[k] = orr[k] = oi[k] = 0.0;
-        }
-        bool live = any_live<K>(sc);
-        bool all = all_live<K>(sc);
-        double gate[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) gate[k] = sc[k] == 0 ? 1.0 : 0.0;
-        if (live) {
-            const double ar = a[2 * m], ai = a[2 * m + 1];
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const double f = gate[k] * mu1[k];
-                er[k] = fma(f, ar, er[k]);
-                ei[k] = fma(f, ai, ei[k]);
-            }
-        }
-        int l = m;
-        // scaled region: some lanes still below 2^-100; steps alternate odd, even
-        while (!all && l < L) {
-            ++l;
-            syn_scaled_step<K, true>(A[l], a[2 * l], a[2 * l + 1], live, gate, x, mu1, mu2, er, ei, orr, oi);
-            if (rescale<K>(mu1, mu2, sc)) {
-                live = any_live<K>(sc);
-                all = all_live<K>(sc);
-#pragma unroll
-                for (int k = 0; k < K; ++k) gate[k] = sc[k] == 0 ? 1.0 : 0.0;
-            }
-            if (all || l >= L) break;
-            ++l;
-            syn_scaled_step<K, false>(A[l], a[2 * l], a[2 * l + 1], live, gate, x, mu1, mu2, er, ei, orr, oi);
-            if (rescale<K>(mu1, mu2, sc)) {
-                live = any_live<K>(sc);
-                all = all_live<K>(sc);
-#pragma unroll
-                for (int k = 0; k < K; ++k) gate[k] = sc[k] == 0 ? 1.0 : 0.0;
-            }
-        }
-        // live region: no checks
-        if (((l - m) & 1) == 0)
-            syn_run<K, true>(l, L, A, a, x, mu1, mu2, er, ei, orr, oi);
+    st.refresh();
+    {
+        const double2 al = a[l0];
+        if ((l0 - m) & 1)
+            syn_gated<K, true>(st, ac, al.x, al.y);
         else
-            syn_run<K, false>(l, L, A, a, x, mu1, mu2, er, ei, orr, oi);
+            syn_gated<K, false>(st, ac, al.x, al.y);
+    }
+    // staged chunks of steps l0+1 .. L
+    int cb = l0 + 1;
+    double tA = 0.0;
+    double2 ta = make_double2(0.0, 0.0);
+    if (cb + lane <= L) {
+        tA = A[cb + lane];
+        ta = a[cb + lane];
+    }
+    sA[0][lane] = tA;
+    sa[0][lane] = ta;
+    __syncthreads();
+    int buf = 0;
+    int l = cb;
+    while (cb <= L) {
+        const int nb = cb + kChunk;
+        if (nb + lane <= L) {
+            tA = A[nb + lane];
+            ta = a[nb + lane];
+        }
+        const int cend = min(L, nb - 1);
+        syn_chunk<K>(st, ac, l, cend, cb, m, sA[buf], sa[buf]);
+        if (nb <= L) {
+            sA[buf ^ 1][lane] = tA;
+            sa[buf ^ 1][lane] = ta;
+        }
+        __syncthreads();
+        buf ^= 1;
+        cb = nb;
+    }
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (p[k] < md.npair) {
-                double2* dst = reinterpret_cast<double2*>(md.P + ((long)p[k] * (L + 1) + m) * 4);
-                dst[0] = make_double2(er[k], ei[k]);
-                dst[1] = make_double2(orr[k], oi[k]);
-            }
+    for (int k = 0; k < K; ++k) {
+        if (p[k] < md.npair) {
+            double2* dst = reinterpret_cast<double2*>(md.P + ((long)p[k] * (L + 1) + m) * 4);
+            dst[0] = make_double2(ac.er[k], ac.ei[k]);
+            dst[1] = make_double2(ac.orr[k], ac.oi[k]);
         }
     }
 }
@@ -315,231 +328,175 @@ __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items
 // in LDS; every kRedChunk steps the wave reduces the 64 lanes and writes the
 // ring-block partial u_lm (re, im) with one coalesced store.
 
-__device__ __forceinline__ void adj_flush(double* red, int lane, int cnt, double* dst)
-{
-    __syncthreads();
-    const int c = lane & 15;
-    const int q = lane >> 4;
-    double s = 0.0;
+template <int K>
+struct AdjIn {
+    double per[K], pei[K], por[K], poi[K];
+};
+
+struct AdjRed {
+    double* red;   // [64 * kRedStride]
+    double* mine;  // this lane's row
+    double* dst;   // output row, dst[2l], dst[2l+1]
+    int cnt, lbase, lane;
+    __device__ __forceinline__ void flush()
+    {
+        __syncthreads();
+        const int c = lane & 15;
+        const int q = lane >> 4;
+        double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) s += red[(16 * q + i) * kRedStride + c];
-    s += __shfl_xor(s, 16);
-    s += __shfl_xor(s, 32);
-    if (lane < 2 * cnt) dst[lane] = s;
-    __syncthreads();
-}
+        for (int i = 0; i < 16; ++i) s += red[(16 * q + i) * kRedStride + c];
+        s += __shfl_xor(s, 16);
+        s += __shfl_xor(s, 32);
+        if (lane < 2 * cnt) dst[2 * lbase + lane] = s;
+        __syncthreads();
+        lbase += cnt;
+        cnt = 0;
+    }
+    __device__ __forceinline__ void push(double sr, double si)
+    {
+        mine[2 * cnt] = sr;
+        mine[2 * cnt + 1] = si;
+        if (++cnt == kRedChunk) flush();
+    }
+};
 
 template <int K, bool ODD>
-__device__ __forceinline__ void adj_step(double Al, const double (&x)[K], double (&mu1)[K], double (&mu2)[K],
-                                         const double (&per)[K], const double (&pei)[K],
-                                         const double (&por)[K], const double (&poi)[K], double& sr,
-                                         double& si)
+__device__ __forceinline__ void adj_live(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, double Al)
 {
-    sr = 0.0;
-    si = 0.0;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const double nu = fma(Al * x[k], mu1[k], -mu2[k]);
-        mu2[k] = mu1[k];
-        mu1[k] = nu;
-        if (ODD) {
-            sr = fma(nu, por[k], sr);
-            si = fma(nu, poi[k], si);
-        } else {
-            sr = fma(nu, per[k], sr);
-            si = fma(nu, pei[k], si);
-        }
-    }
-}
-
-template <int K, bool ODD>
-__device__ __forceinline__ void adj_scaled_step(double Al, bool live, const double (&gate)[K], const double (&x)[K],
-                                                double (&mu1)[K], double (&mu2)[K], const double (&per)[K],
-                                                const double (&pei)[K], const double (&por)[K],
-                                                const double (&poi)[K], double* out)
-{
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const double nu = fma(Al * x[k], mu1[k], -mu2[k]);
-        mu2[k] = mu1[k];
-        mu1[k] = nu;
-    }
     double sr = 0.0, si = 0.0;
-    if (live) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double nu = fma(Al * st.x[k], st.mu1[k], -st.mu2[k]);
+        st.mu2[k] = st.mu1[k];
+        st.mu1[k] = nu;
+        sr = fma(nu, ODD ? in.por[k] : in.per[k], sr);
+        si = fma(nu, ODD ? in.poi[k] : in.pei[k], si);
+    }
+    rd.push(sr, si);
+}
+
+template <int K, bool ODD>
+__device__ __forceinline__ void adj_gated(const LegState<K>& st, const AdjIn<K>& in, AdjRed& rd)
+{
+    double sr = 0.0, si = 0.0;
+    if (st.live) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const double f = gate[k] * mu1[k];
-            sr = fma(f, ODD ? por[k] : per[k], sr);
-            si = fma(f, ODD ? poi[k] : pei[k], si);
+            const double f = st.gate[k] * st.mu1[k];
+            sr = fma(f, ODD ? in.por[k] : in.per[k], sr);
+            si = fma(f, ODD ? in.poi[k] : in.pei[k], si);
         }
     }
-    out[0] = sr;
-    out[1] = si;
+    rd.push(sr, si);
+}
+
+template <int K>
+__device__ __forceinline__ void adj_chunk(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, int& l, int cend, int cb,
+                                          int m, const double* cA)
+{
+    while (!st.all && l <= cend) {
+        st.advance(cA[l - cb]);
+        if ((l - m) & 1)
+            adj_gated<K, true>(st, in, rd);
+        else
+            adj_gated<K, false>(st, in, rd);
+        if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
+        ++l;
+    }
+    if ((l - m) & 1) {
+        for (; l + 1 <= cend; l += 2) {
+            adj_live<K, true>(st, in, rd, cA[l - cb]);
+            adj_live<K, false>(st, in, rd, cA[l + 1 - cb]);
+        }
+    } else {
+        for (; l + 1 <= cend; l += 2) {
+            adj_live<K, false>(st, in, rd, cA[l - cb]);
+            adj_live<K, true>(st, in, rd, cA[l + 1 - cb]);
+        }
+    }
+    if (l <= cend) {
+        if ((l - m) & 1)
+            adj_live<K, true>(st, in, rd, cA[l - cb]);
+        else
+            adj_live<K, false>(st, in, rd, cA[l - cb]);
+        ++l;
+    }
 }
 
 template <int K>
 __global__ __launch_bounds__(64) void k_adj(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
 {
     __shared__ double red[64 * kRedStride];
+    __shared__ double sA[2][kChunk];
     const WorkItem it = items[blockIdx.x];
+    if (it.mcnt == 0) return;
     const MapDesc& md = maps[it.map];
     const int L = md.L;
     const int npad = md.npair_pad;
     const int lane = threadIdx.x;
-    double x[K];
+    const int m = it.m0;
+    const int l0 = it.lfirst;
+    const long base = moff(L, m) - m;
+    const double* __restrict__ A = md.A + base;
+
+    LegState<K> st;
+    AdjIn<K> in;
     int p[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         p[k] = it.rb * 64 * K + k * 64 + lane;
-        x[k] = md.ring_x[p[k]];
-    }
-    double* urow = md.upart + 2 * (long)it.rb * md.nalm;
-    double* myred = red + lane * kRedStride;
-    for (int mi = 0; mi < it.mcnt; ++mi) {
-        const int m = it.m0 + mi;
-        const long base = moff(L, m) - m;
-        cptr<double> A = as_const(md.A + base);
-        double* dst = urow + 2 * base;  // dst[2l], dst[2l+1]
-        double mu1[K], mu2[K], per[K], pei[K], por[K], poi[K];
-        int sc[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            mu1[k] = md.seed_v[(long)m * npad + p[k]];
-            sc[k] = md.seed_k[(long)m * npad + p[k]];
-            mu2[k] = 0.0;
-            if (p[k] < md.npair) {
-                const double2* src = reinterpret_cast<const double2*>(md.Q + ((long)p[k] * (L + 1) + m) * 4);
-                const double2 e = src[0], o = src[1];
-                per[k] = e.x;
-                pei[k] = e.y;
-                por[k] = o.x;
-                poi[k] = o.y;
-            } else {
-                per[k] = pei[k] = por[k] = poi[k] = 0.0;
-            }
-        }
-        bool live = any_live<K>(sc);
-        bool all = all_live<K>(sc);
-        double gate[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) gate[k] = sc[k] == 0 ? 1.0 : 0.0;
-        int cnt = 0;
-        int lbase = m;
-        {
-            double sr = 0.0, si = 0.0;
-            if (live) {
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const double f = gate[k] * mu1[k];
-                    sr = fma(f, per[k], sr);
-                    si = fma(f, pei[k], si);
-                }
-            }
-            myred[0] = sr;
-            myred[1] = si;
-            cnt = 1;
-            if (m == L) {
-                adj_flush(red, lane, cnt, dst + 2 * lbase);
-                lbase += cnt;
-                cnt = 0;
-            }
-        }
-        int l = m;
-        // scaled region: steps alternate odd, even
-        while (!all && l < L) {
-            ++l;
-            adj_scaled_step<K, true>(A[l], live, gate, x, mu1, mu2, per, pei, por, poi, myred + 2 * cnt);
-            ++cnt;
-            if (cnt == kRedChunk || l == L) {
-                adj_flush(red, lane, cnt, dst + 2 * lbase);
-                lbase += cnt;
-                cnt = 0;
-            }
-            if (rescale<K>(mu1, mu2, sc)) {
-                live = any_live<K>(sc);
-                all = all_live<K>(sc);
-#pragma unroll
-                for (int k = 0; k < K; ++k) gate[k] = sc[k] == 0 ? 1.0 : 0.0;
-            }
-            if (all || l >= L) break;
-            ++l;
-            adj_scaled_step<K, false>(A[l], live, gate, x, mu1, mu2, per, pei, por, poi, myred + 2 * cnt);
-            ++cnt;
-            if (cnt == kRedChunk || l == L) {
-                adj_flush(red, lane, cnt, dst + 2 * lbase);
-                lbase += cnt;
-                cnt = 0;
-            }
-            if (rescale<K>(mu1, mu2, sc)) {
-                live = any_live<K>(sc);
-                all = all_live<K>(sc);
-#pragma unroll
-                for (int k = 0; k < K; ++k) gate[k] = sc[k] == 0 ? 1.0 : 0.0;
-            }
-        }
-        // live region: two steps (one of each parity) per iteration
-        bool odd_next = ((l - m) & 1) == 0;
-        if (l < L && !odd_next) {
-            // align so that each iteration starts with an odd step
-            ++l;
-            double sr, si;
-            adj_step<K, false>(A[l], x, mu1, mu2, per, pei, por, poi, sr, si);
-            myred[2 * cnt] = sr;
-            myred[2 * cnt + 1] = si;
-            ++cnt;
-            if (cnt == kRedChunk || l == L) {
-                adj_flush(red, lane, cnt, dst + 2 * lbase);
-                lbase += cnt;
-                cnt = 0;
-            }
-        }
-        if (l + 2 <= L) {
-            double c0 = A[l + 1], c1 = A[l + 2];
-            for (; l + 2 <= L; l += 2) {
-                double n0 = 0.0, n1 = 0.0;
-                if (l + 4 <= L) {
-                    n0 = A[l + 3];
-                    n1 = A[l + 4];
-                }
-                if (cnt > kRedChunk - 2) {  // keep room for two entries in the LDS row
-                    adj_flush(red, lane, cnt, dst + 2 * lbase);
-                    lbase += cnt;
-                    cnt = 0;
-                }
-                double sr, si;
-                adj_step<K, true>(c0, x, mu1, mu2, per, pei, por, poi, sr, si);
-                myred[2 * cnt] = sr;
-                myred[2 * cnt + 1] = si;
-                adj_step<K, false>(c1, x, mu1, mu2, per, pei, por, poi, sr, si);
-                myred[2 * cnt + 2] = sr;
-                myred[2 * cnt + 3] = si;
-                cnt += 2;
-                if (cnt == kRedChunk || l + 2 == L) {
-                    adj_flush(red, lane, cnt, dst + 2 * lbase);
-                    lbase += cnt;
-                    cnt = 0;
-                }
-                c0 = n0;
-                c1 = n1;
-            }
-        }
-        if (l < L) {
-            ++l;
-            double sr, si;
-            adj_step<K, true>(A[l], x, mu1, mu2, per, pei, por, poi, sr, si);
-            myred[2 * cnt] = sr;
-            myred[2 * cnt + 1] = si;
-            ++cnt;
-            adj_flush(red, lane, cnt, dst + 2 * lbase);
-            lbase += cnt;
-            cnt = 0;
+        const long q = (long)m * npad + p[k];
+        st.x[k] = md.ring_x[p[k]];
+        st.mu1[k] = md.s5_cur[q];
+        st.mu2[k] = md.s5_prev[q];
+        st.sc[k] = md.s5_k[q];
+        if (p[k] < md.npair) {
+            const double2* src = reinterpret_cast<const double2*>(md.Q + ((long)p[k] * (L + 1) + m) * 4);
+            const double2 e = src[0], o = src[1];
+            in.per[k] = e.x;
+            in.pei[k] = e.y;
+            in.por[k] = o.x;
+            in.poi[k] = o.y;
+        } else {
+            in.per[k] = in.pei[k] = in.por[k] = in.poi[k] = 0.0;
         }
     }
+    st.refresh();
+    AdjRed rd;
+    rd.red = red;
+    rd.mine = red + lane * kRedStride;
+    rd.dst = md.upart + 2 * (long)it.rb * md.nalm + 2 * base;
+    rd.cnt = 0;
+    rd.lbase = l0;
+    rd.lane = lane;
+    if ((l0 - m) & 1)
+        adj_gated<K, true>(st, in, rd);
+    else
+        adj_gated<K, false>(st, in, rd);
+
+    int cb = l0 + 1;
+    double tA = (cb + lane <= L) ? A[cb + lane] : 0.0;
+    sA[0][lane] = tA;
+    __syncthreads();
+    int buf = 0;
+    int l = cb;
+    while (cb <= L) {
+        const int nb = cb + kChunk;
+        if (nb + lane <= L) tA = A[nb + lane];
+        const int cend = min(L, nb - 1);
+        adj_chunk<K>(st, in, rd, l, cend, cb, m, sA[buf]);
+        if (nb <= L) sA[buf ^ 1][lane] = tA;
+        __syncthreads();
+        buf ^= 1;
+        cb = nb;
+    }
+    if (rd.cnt > 0) rd.flush();
 }
 
-// -------------------------------------------------------------- scan (set-up)
-// Runs the scaled recurrence once per (m, pair) and records the first l at
-// which the pair becomes live (L+1 if never).  Used to drop dead work items.
+// -------------------------------------------------------------- set-up passes
+// Both run the scaled recurrence from the seed lambda_mm.
+// k_scan: first l at which each ring pair becomes live (L+1 if never).
 template <int K>
 __global__ __launch_bounds__(64) void k_scan(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps,
                                              int* __restrict__ lstart)
@@ -549,45 +506,71 @@ __global__ __launch_bounds__(64) void k_scan(const WorkItem* __restrict__ items,
     const int L = md.L;
     const int npad = md.npair_pad;
     const int lane = threadIdx.x;
-    double x[K];
-    int p[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        p[k] = it.rb * 64 * K + k * 64 + lane;
-        x[k] = md.ring_x[p[k]];
-    }
     for (int mi = 0; mi < it.mcnt; ++mi) {
         const int m = it.m0 + mi;
         const double* __restrict__ A = md.A + moff(L, m) - m;
-        double mu1[K], mu2[K];
-        int sc[K], ls[K];
+        LegState<K> st;
+        int p[K], ls[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            mu1[k] = md.seed_v[(long)m * npad + p[k]];
-            sc[k] = md.seed_k[(long)m * npad + p[k]];
-            mu2[k] = 0.0;
-            ls[k] = sc[k] == 0 ? m : L + 1;
+            p[k] = it.rb * 64 * K + k * 64 + lane;
+            st.x[k] = md.ring_x[p[k]];
+            st.mu1[k] = md.seed_v[(long)m * npad + p[k]];
+            st.sc[k] = md.seed_k[(long)m * npad + p[k]];
+            st.mu2[k] = 0.0;
+            ls[k] = st.sc[k] == 0 ? m : L + 1;
         }
-        bool all = all_live<K>(sc);
+        bool all = all_live<K>(st.sc);
         int l = m;
         while (!all && l < L) {
             ++l;
-            const double Al = A[l];
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const double nu = fma(Al * x[k], mu1[k], -mu2[k]);
-                mu2[k] = mu1[k];
-                mu1[k] = nu;
-            }
-            if (rescale<K>(mu1, mu2, sc)) {
+            st.advance(A[l]);
+            if (rescale<K>(st.mu1, st.mu2, st.sc)) {
 #pragma unroll
                 for (int k = 0; k < K; ++k)
-                    if (sc[k] == 0 && ls[k] > L) ls[k] = l + 1;
-                all = all_live<K>(sc);
+                    if (st.sc[k] == 0 && ls[k] > L) ls[k] = l + 1;
+                all = all_live<K>(st.sc);
             }
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) lstart[(long)m * npad + p[k]] = ls[k];
+    }
+}
+
+// k_capture: the state (mu_{l-1}, mu_l, scale count) at l = item.lfirst.
+template <int K>
+__global__ __launch_bounds__(64) void k_capture(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps,
+                                                double* __restrict__ prev, double* __restrict__ cur,
+                                                int* __restrict__ kout)
+{
+    const WorkItem it = items[blockIdx.x];
+    if (it.mcnt == 0) return;
+    const MapDesc& md = maps[it.map];
+    const int L = md.L;
+    const int npad = md.npair_pad;
+    const int lane = threadIdx.x;
+    const int m = it.m0;
+    const double* __restrict__ A = md.A + moff(L, m) - m;
+    LegState<K> st;
+    int p[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        p[k] = it.rb * 64 * K + k * 64 + lane;
+        st.x[k] = md.ring_x[p[k]];
+        st.mu1[k] = md.seed_v[(long)m * npad + p[k]];
+        st.sc[k] = md.seed_k[(long)m * npad + p[k]];
+        st.mu2[k] = 0.0;
+    }
+    for (int l = m + 1; l <= it.lfirst; ++l) {
+        st.advance(A[l]);
+        rescale<K>(st.mu1, st.mu2, st.sc);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const long q = (long)m * npad + p[k];
+        prev[q] = st.mu2[k];
+        cur[q] = st.mu1[k];
+        kout[q] = st.sc[k];
     }
 }
 
@@ -951,6 +934,27 @@ int launch_adj(int K, const WorkItem* items, int n, const MapDesc* d_maps, hipSt
     }
     return fail(CMBL_ERR_ARG, "unsupported rings-per-lane K");
 }
+template <int K>
+static int capture_k(const WorkItem* items, int n, const MapDesc* d_maps, double* prev, double* cur, int* kk,
+                     hipStream_t s)
+{
+    if (n <= 0) return CMBL_OK;
+    hipLaunchKernelGGL(k_capture<K>, dim3(n), dim3(64), 0, s, items, d_maps, prev, cur, kk);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+int launch_capture(int K, const WorkItem* items, int n, const MapDesc* d_maps, double* prev, double* cur, int* kk,
+                   hipStream_t s)
+{
+    switch (K) {
+    case 1: return capture_k<1>(items, n, d_maps, prev, cur, kk, s);
+    case 2: return capture_k<2>(items, n, d_maps, prev, cur, kk, s);
+    case 4: return capture_k<4>(items, n, d_maps, prev, cur, kk, s);
+    case 8: return capture_k<8>(items, n, d_maps, prev, cur, kk, s);
+    }
+    return fail(CMBL_ERR_ARG, "unsupported rings-per-lane K");
+}
+
 int launch_scan(int K, const WorkItem* items, int n, const MapDesc* d_maps, int* d_lstart, hipStream_t s)
 {
     switch (K) {

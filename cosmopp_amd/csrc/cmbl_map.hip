@@ -130,56 +130,82 @@ static int dupload(cmbl_map* mp, T** p, const std::vector<T>& h)
         if (rc_ != CMBL_OK) return rc_; \
     } while (0)
 
-// Work lists for a Legendre stage with K rings per lane.  Items are chunks
-// of up to kMChunk consecutive live m for one ring block, ordered by
-// estimated cost, largest first.
-static const int kMChunk = 4;
+// Work lists for a Legendre stage with K rings per lane.  One item per live
+// (m, ring block); lfirst is the first l at which any of its pairs is live.
+// The cost estimate counts VALU work per lane: 3 per scaled step, 4 per live
+// step, K rings.
+static const int kXcds = 8;
 
 static void build_items(const cmbl_map* mp, int K, const std::vector<int>& lstart, int map_index,
-                        std::vector<WorkItem>* out, std::vector<double>* cost)
+                        std::vector<ItemCost>* out)
 {
     const int L = mp->L;
     const int npad = mp->npair_pad;
     const int nrb = npad / (64 * K);
     for (int rb = 0; rb < nrb; ++rb) {
-        int m = 0;
-        while (m <= L) {
-            WorkItem it{map_index, m, 0, rb};
-            double c = 0.0;
-            while (m <= L && it.mcnt < kMChunk) {
-                int lmin = L + 1;
-                for (int j = 0; j < 64 * K; ++j) lmin = std::min(lmin, lstart[(long)m * npad + rb * 64 * K + j]);
-                if (lmin > L) {
-                    if (it.mcnt > 0) break;  // close the chunk at a dead m
-                    ++m;
-                    it.m0 = m;
-                    continue;
+        for (int m = 0; m <= L; ++m) {
+            int lmin = L + 1, lmax = -1;
+            for (int j = 0; j < 64 * K; ++j) {
+                const int v = lstart[(size_t)m * npad + rb * 64 * K + j];
+                if (v <= L) {
+                    lmin = std::min(lmin, v);
+                    lmax = std::max(lmax, v);
                 }
-                c += K * (2.0 * (lmin - m) + 4.0 * (L - lmin + 1));
-                ++it.mcnt;
-                ++m;
             }
-            if (it.mcnt > 0) {
-                out->push_back(it);
-                cost->push_back(c);
-            }
+            if (lmin > L) continue;
+            const double cost = K * (3.0 * (lmax - lmin) + 4.0 * (L - lmax + 1)) + 64.0 * K;
+            out->push_back(ItemCost{WorkItem{map_index, m, 1, rb, lmin, 0}, cost});
         }
     }
 }
 
-static void sort_by_cost(std::vector<WorkItem>* items, std::vector<double>* cost)
+// XCD-aware order.  Workgroup i runs on XCD i % 8, so the list is built as
+// eight interleaved per-XCD lists.  Items of four consecutive m of one map
+// (which share the 128-byte lines of the pair-major stage buffers) go to the
+// same XCD; groups are spread over the XCDs by total cost; each XCD list is
+// ordered by cost, largest first; short lists are padded with empty items.
+static std::vector<WorkItem> order_items(std::vector<ItemCost>& v)
 {
-    std::vector<size_t> idx(items->size());
-    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
-    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return (*cost)[a] > (*cost)[b]; });
-    std::vector<WorkItem> s(items->size());
-    std::vector<double> c(items->size());
-    for (size_t i = 0; i < idx.size(); ++i) {
-        s[i] = (*items)[idx[i]];
-        c[i] = (*cost)[idx[i]];
+    std::vector<WorkItem> out;
+    if (v.empty()) return out;
+    std::vector<std::pair<long, double>> groups;  // (key, cost)
+    std::vector<long> keys(v.size());
+    {
+        std::vector<std::pair<long, size_t>> kv;
+        for (size_t i = 0; i < v.size(); ++i) {
+            keys[i] = ((long)v[i].it.map << 32) | (long)(v[i].it.m0 >> 2);
+            kv.push_back({keys[i], i});
+        }
+        std::sort(kv.begin(), kv.end());
+        for (size_t i = 0; i < kv.size(); ++i) {
+            if (groups.empty() || groups.back().first != kv[i].first) groups.push_back({kv[i].first, 0.0});
+            groups.back().second += v[kv[i].second].cost;
+        }
     }
-    items->swap(s);
-    cost->swap(c);
+    std::sort(groups.begin(), groups.end(),
+              [](const std::pair<long, double>& a, const std::pair<long, double>& b) { return a.second > b.second; });
+    std::vector<double> load(kXcds, 0.0);
+    std::vector<std::pair<long, int>> assign;
+    for (const auto& g : groups) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += g.second;
+        assign.push_back({g.first, x});
+    }
+    std::sort(assign.begin(), assign.end());
+    std::vector<std::vector<ItemCost>> lists(kXcds);
+    for (size_t i = 0; i < v.size(); ++i) {
+        const auto itx = std::lower_bound(assign.begin(), assign.end(), std::make_pair(keys[i], -1));
+        lists[itx->second].push_back(v[i]);
+    }
+    size_t len = 0;
+    for (auto& l : lists) {
+        std::stable_sort(l.begin(), l.end(), [](const ItemCost& a, const ItemCost& b) { return a.cost > b.cost; });
+        len = std::max(len, l.size());
+    }
+    out.resize(len * kXcds, WorkItem{0, 0, 0, 0, 0, 0});
+    for (int x = 0; x < kXcds; ++x)
+        for (size_t j = 0; j < lists[x].size(); ++j) out[j * kXcds + x] = lists[x][j].it;
+    return out;
 }
 
 static int choose_k(int npair, int kmax)
@@ -375,10 +401,10 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
         CMBL_HIP(hipDeviceSynchronize());
     }
 
-    // ---- scan: first live l per (m, pair), then the work lists
+    // ---- scan: first live l per (m, pair); work lists; start states
     std::vector<WorkItem> all;
     for (int rb = 0; rb < mp->nrb1; ++rb)
-        for (int m = 0; m <= L; m += kMChunk) all.push_back(WorkItem{0, m, std::min(kMChunk, L + 1 - m), rb});
+        for (int m = 0; m <= L; m += 4) all.push_back(WorkItem{0, m, std::min(4, L + 1 - m), rb, m, 0});
     WorkItem* d_all;
     int* d_ls;
     CMBL_TRY(dupload(mp, &d_all, all));
@@ -390,15 +416,37 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     for (int m = 0; m <= L; ++m)
         for (int p = mp->npair; p < npad; ++p) lstart[(size_t)m * npad + p] = L + 1;
 
-    std::vector<double> c1, c5;
-    build_items(mp, mp->K1, lstart, 0, &mp->items1, &c1);
-    build_items(mp, mp->K5, lstart, 0, &mp->items5, &c5);
-    sort_by_cost(&mp->items1, &c1);
-    sort_by_cost(&mp->items5, &c5);
+    std::vector<ItemCost> c1, c5;
+    build_items(mp, mp->K1, lstart, 0, &c1);
+    build_items(mp, mp->K5, lstart, 0, &c5);
+    mp->cost1 = c1;
+    mp->cost5 = c5;
+    mp->items1 = order_items(c1);
+    mp->items5 = order_items(c5);
     mp->nitems1 = (int)mp->items1.size();
     mp->nitems5 = (int)mp->items5.size();
     CMBL_TRY(dupload(mp, &mp->d_items1, mp->items1));
     CMBL_TRY(dupload(mp, &mp->d_items5, mp->items5));
+    {
+        const size_t nst = (size_t)(L + 1) * npad;
+        double *p1, *c1p, *p5, *c5p;
+        int *k1, *k5;
+        CMBL_TRY(dalloc(mp, &p1, nst));
+        CMBL_TRY(dalloc(mp, &c1p, nst));
+        CMBL_TRY(dalloc(mp, &k1, nst));
+        CMBL_TRY(dalloc(mp, &p5, nst));
+        CMBL_TRY(dalloc(mp, &c5p, nst));
+        CMBL_TRY(dalloc(mp, &k5, nst));
+        CMBL_TRY(launch_capture(mp->K1, mp->d_items1, mp->nitems1, mp->d_desc, p1, c1p, k1, nullptr));
+        CMBL_TRY(launch_capture(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, p5, c5p, k5, nullptr));
+        d.s1_prev = p1;
+        d.s1_cur = c1p;
+        d.s1_k = k1;
+        d.s5_prev = p5;
+        d.s5_cur = c5p;
+        d.s5_k = k5;
+        CMBL_HIP(hipMemcpy(mp->d_desc, &d, sizeof(d), hipMemcpyHostToDevice));
+    }
     // ring items: largest transform first
     for (int p = 0; p < mp->npair; ++p) mp->ring_items.push_back(RingItem{0, p});
     std::stable_sort(mp->ring_items.begin(), mp->ring_items.end(),
@@ -411,6 +459,8 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
 
 static void map_free(cmbl_map* mp)
 {
+    for (hipEvent_t e : mp->ev) (void)hipEventDestroy(e);
+    mp->ev.clear();
     for (void* p : mp->allocs) (void)hipFree(p);
     mp->allocs.clear();
 }
@@ -421,17 +471,64 @@ static int check_map(const cmbl_map* mp)
     return CMBL_OK;
 }
 
+// Stage timing: accumulate the oldest recorded event sets (blocks on them).
+static int drain_timing(cmbl_map* mp, int keep)
+{
+    const int S = CMBL_NSTAGES + 1;
+    while (mp->ev_pending > keep) {
+        const int set = (mp->ev_next - mp->ev_pending + cmbl_map::kEvSets) % cmbl_map::kEvSets;
+        hipEvent_t* e = &mp->ev[(size_t)set * S];
+        CMBL_HIP(hipEventSynchronize(e[CMBL_NSTAGES]));
+        for (int i = 0; i < CMBL_NSTAGES; ++i) {
+            float ms = 0.0f;
+            CMBL_HIP(hipEventElapsedTime(&ms, e[i], e[i + 1]));
+            mp->stage_ms[i] += ms;
+        }
+        ++mp->timed_evals;
+        --mp->ev_pending;
+    }
+    return CMBL_OK;
+}
+
+struct StageMarks {
+    cmbl_map* mp;
+    hipStream_t s;
+    hipEvent_t* e = nullptr;
+    int begin()
+    {
+        if (!mp->timing) return CMBL_OK;
+        CMBL_TRY(drain_timing(mp, cmbl_map::kEvSets - 1));
+        e = &mp->ev[(size_t)mp->ev_next * (CMBL_NSTAGES + 1)];
+        mp->ev_next = (mp->ev_next + 1) % cmbl_map::kEvSets;
+        ++mp->ev_pending;
+        return mark(0);
+    }
+    int mark(int i)
+    {
+        if (e) CMBL_HIP(hipEventRecord(e[i], s));
+        return CMBL_OK;
+    }
+};
+
 // One evaluation on stream s with device x; f and g are device pointers.
 static int eval_device(cmbl_map* mp, const double* x, double* f, double* g, hipStream_t s)
 {
     const MapDesc& d = mp->desc;
+    StageMarks t{mp, s};
+    CMBL_TRY(t.begin());
     CMBL_TRY(launch_pre(d, mp->d_desc, x, s));
+    CMBL_TRY(t.mark(1));
     CMBL_TRY(launch_synth(mp->K1, mp->d_items1, mp->nitems1, mp->d_desc, s));
+    CMBL_TRY(t.mark(2));
     CMBL_TRY(launch_ring(kRingLike, mp->d_ring_items, mp->nring_items, mp->d_desc, mp->lds_ring, nullptr, nullptr,
                          mp->d_tw, s));
+    CMBL_TRY(t.mark(3));
     CMBL_TRY(launch_adj(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, s));
+    CMBL_TRY(t.mark(4));
     CMBL_TRY(launch_post(d, mp->d_desc, x, g, 1, s));
+    CMBL_TRY(t.mark(5));
     CMBL_TRY(launch_final(mp->d_desc, f, s));
+    CMBL_TRY(t.mark(6));
     return CMBL_OK;
 }
 
@@ -478,6 +575,30 @@ int cmbl_map_nside(const cmbl_map* mp) { return mp ? mp->nside : -1; }
 int cmbl_map_lmax(const cmbl_map* mp) { return mp ? mp->L : -1; }
 long cmbl_map_npix(const cmbl_map* mp) { return mp ? mp->npix : -1; }
 long cmbl_map_nparams(const cmbl_map* mp) { return mp ? mp->nparam : -1; }
+
+int cmbl_map_set_timing(cmbl_map* mp, int on)
+{
+    CMBL_TRY(check_map(mp));
+    CMBL_TRY(drain_timing(mp, 0));
+    if (on && mp->ev.empty()) {
+        mp->ev.resize((size_t)cmbl_map::kEvSets * (CMBL_NSTAGES + 1));
+        for (hipEvent_t& e : mp->ev) CMBL_HIP(hipEventCreate(&e));
+    }
+    mp->timing = on != 0;
+    for (double& v : mp->stage_ms) v = 0.0;
+    mp->timed_evals = 0;
+    return CMBL_OK;
+}
+
+int cmbl_map_stage_times(cmbl_map* mp, double* ms, long* nevals)
+{
+    CMBL_TRY(check_map(mp));
+    if (!ms) return fail(CMBL_ERR_ARG, "ms is null");
+    CMBL_TRY(drain_timing(mp, 0));
+    for (int i = 0; i < CMBL_NSTAGES; ++i) ms[i] = mp->stage_ms[i];
+    if (nevals) *nevals = mp->timed_evals;
+    return CMBL_OK;
+}
 
 int cmbl_map_set_beam(cmbl_map* mp, const double* beam)
 {
@@ -598,10 +719,8 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
         return rc;
     };
     std::vector<MapDesc> descs;
-    std::vector<WorkItem> w1[4], w5[4];
-    std::vector<double> c1[4], c5[4];
+    std::vector<ItemCost> w1[4], w5[4];
     std::vector<RingItem> rings;
-    std::vector<int> ringM;
     for (int i = 0; i < nmaps; ++i) {
         cmbl_map* mp = maps[i];
         if (!mp) return cleanup(fail(CMBL_ERR_ARG, "null map in batch"));
@@ -610,21 +729,17 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
         int k1 = 0, k5 = 0;
         while ((1 << k1) < mp->K1) ++k1;
         while ((1 << k5) < mp->K5) ++k5;
-        for (WorkItem it : mp->items1) {
-            it.map = i;
-            w1[k1].push_back(it);
+        for (ItemCost ic : mp->cost1) {
+            ic.it.map = i;
+            w1[k1].push_back(ic);
         }
-        for (WorkItem it : mp->items5) {
-            it.map = i;
-            w5[k5].push_back(it);
+        for (ItemCost ic : mp->cost5) {
+            ic.it.map = i;
+            w5[k5].push_back(ic);
         }
-        // costs are recomputed approximately from the m range
-        for (const WorkItem& it : mp->items1) c1[k1].push_back(mp->K1 * (double)it.mcnt * (mp->L + 1 - it.m0));
-        for (const WorkItem& it : mp->items5) c5[k5].push_back(mp->K5 * (double)it.mcnt * (mp->L + 1 - it.m0));
         for (RingItem r : mp->ring_items) {
             r.map = i;
             rings.push_back(r);
-            ringM.push_back(mp->lds_ring);
         }
         b->lds_ring = std::max(b->lds_ring, mp->lds_ring);
     }
@@ -637,10 +752,9 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
         return cleanup(fail(CMBL_ERR_HIP, "batch upload"));
     for (int k = 0; k < 4; ++k) {
         for (int stage = 0; stage < 2; ++stage) {
-            std::vector<WorkItem>& w = stage ? w5[k] : w1[k];
-            std::vector<double>& c = stage ? c5[k] : c1[k];
-            if (w.empty()) continue;
-            sort_by_cost(&w, &c);
+            std::vector<ItemCost>& wc = stage ? w5[k] : w1[k];
+            if (wc.empty()) continue;
+            const std::vector<WorkItem> w = order_items(wc);
             if (hipMalloc(&q, sizeof(WorkItem) * w.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
             b->allocs.push_back(q);
             if (hipMemcpy(q, w.data(), sizeof(WorkItem) * w.size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -813,6 +927,56 @@ int cmbl_weights_from_mask(long npix, const double* mask, double sigma, double* 
 {
     if (!mask || !w || npix < 0 || !(sigma > 0)) return fail(CMBL_ERR_ARG, "bad weight arguments");
     for (long i = 0; i < npix; ++i) w[i] = mask[i] > 0.5 ? 1.0 / (sigma * sigma) : 0.0;
+    return CMBL_OK;
+}
+
+int cmbl_whitenoise(int seed, double amplitude, long n, double* out)
+{
+    if (!out || n < 0) return fail(CMBL_ERR_ARG, "bad whitenoise arguments");
+    // one process: the master seed list has a single entry
+    std::mt19937 g1(seed);
+    std::uniform_real_distribution<double> u(0, 1);
+    const int s1 = static_cast<int>(std::ceil(1000000 * u(g1)));
+    std::mt19937 g2(s1);
+    std::normal_distribution<> gauss(0, 1);
+    for (long i = 0; i < n; ++i) out[i] = amplitude * gauss(g2);
+    return CMBL_OK;
+}
+
+// ---------------------------------------------------------------- device memory
+int cmbl_device_alloc(void** ptr, unsigned long bytes)
+{
+    if (!ptr) return fail(CMBL_ERR_ARG, "ptr is null");
+    *ptr = nullptr;
+    CMBL_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+    return CMBL_OK;
+}
+int cmbl_device_free(void* ptr)
+{
+    if (ptr) CMBL_HIP(hipFree(ptr));
+    return CMBL_OK;
+}
+int cmbl_copy_to_device(void* dst, const void* src, unsigned long bytes)
+{
+    if (bytes && (!dst || !src)) return fail(CMBL_ERR_ARG, "null pointer");
+    if (bytes) CMBL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return CMBL_OK;
+}
+int cmbl_copy_to_host(void* dst, const void* src, unsigned long bytes)
+{
+    if (bytes && (!dst || !src)) return fail(CMBL_ERR_ARG, "null pointer");
+    if (bytes) CMBL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return CMBL_OK;
+}
+int cmbl_copy_device(void* dst, const void* src, unsigned long bytes, void* stream)
+{
+    if (bytes && (!dst || !src)) return fail(CMBL_ERR_ARG, "null pointer");
+    if (bytes) CMBL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return CMBL_OK;
+}
+int cmbl_stream_synchronize(void* stream)
+{
+    CMBL_HIP(hipStreamSynchronize((hipStream_t)stream));
     return CMBL_OK;
 }
 

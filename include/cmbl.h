@@ -88,6 +88,18 @@ int cmbl_forward_device(cmbl_map* map, const double* d_x, double* d_map, void* s
 int cmbl_backward(cmbl_map* map, const double* v, double* g);
 int cmbl_backward_device(cmbl_map* map, const double* d_v, double* d_g, void* stream);
 
+/* ---------------------------------------------------------------- stage timing */
+/* Diagnostics with no counterpart in the reference (it times whole calls
+ * with Timer, include/timer.hpp:9-49).  When enabled, every evaluation of the
+ * map records HIP events between its stages on the evaluation stream (no
+ * host synchronisation); cmbl_map_stage_times synchronises and returns the
+ * accumulated milliseconds of each stage since the last reset:
+ *   [0] a_lm preparation  [1] Legendre synthesis  [2] ring transforms
+ *   [3] adjoint Legendre  [4] gradient assembly   [5] -2lnL reduction */
+#define CMBL_NSTAGES 6
+int cmbl_map_set_timing(cmbl_map* map, int on); /* on != 0 enables and resets */
+int cmbl_map_stage_times(cmbl_map* map, double* ms, long* nevals);
+
 /* ---------------------------------------------------------------- batch */
 /* Many independent maps (uneven nside/lmax) evaluated with one launch per
  * stage over a cost-ordered work list.  The maps must stay alive. */
@@ -115,6 +127,21 @@ int cmbl_mask_regions(int nside, int nreg, const double* theta, const double* ph
 int cmbl_weights_from_mask(long npix, const double* mask, double sigma, double* w);
 /* Pixel centre of a RING-scheme pixel (HEALPix pix2ang_ring). */
 int cmbl_pix2ang_ring(int nside, long pix, double* theta, double* phi);
+
+/* BasicLBFGSFunc::whitenoise for one process (source/lbfgs.cpp:401-416):
+ * seed' = ceil(1e6 * UniformRealGenerator(seed).generate()), then n draws of
+ * amplitude * GaussianGenerator(seed', 0, 1), into a host array. */
+int cmbl_whitenoise(int seed, double amplitude, long n, double* out);
+
+/* ---------------------------------------------------------------- device memory */
+/* Plain device buffers for callers that hold no HIP headers (the C++ wrapper
+ * include/cmbl_likelihood.hpp).  Copies block until complete. */
+int cmbl_device_alloc(void** ptr, unsigned long bytes);
+int cmbl_device_free(void* ptr);
+int cmbl_copy_to_device(void* d_dst, const void* src, unsigned long bytes);
+int cmbl_copy_to_host(void* dst, const void* d_src, unsigned long bytes);
+int cmbl_copy_device(void* d_dst, const void* d_src, unsigned long bytes, void* stream);
+int cmbl_stream_synchronize(void* stream);
 
 /* ---------------------------------------------------------------- device vectors */
 /* BLAS-1 on device vectors of n doubles, for a device-resident LargeVector

@@ -27,6 +27,9 @@ def build(reference="/root/reference"):
     subprocess.run(["make", "-s", "-C", _HERE, "-B", "liboracle.so"], check=True)
     if os.path.isdir(os.path.join(reference, "include")):
         subprocess.run(["make", "-s", "-C", _HERE, "-B", "ref", "REFERENCE=" + reference], check=True)
+        lib_path = os.path.join(os.path.dirname(_HERE), "cosmopp_amd", "libcosmopp_amd.so")
+        if os.path.exists(lib_path):
+            subprocess.run(["make", "-s", "-C", _HERE, "-B", "consumers", "REFERENCE=" + reference], check=True)
 
 
 def _load():

@@ -1,0 +1,169 @@
+// Drop-in check: the reference's own optimisers and samplers, compiled from
+// /root/reference/source (lbfgs.cpp, hmc.cpp, cosmo_mpi.cpp, macros.cpp) and
+// the header-only LBFGS_General / HMCGeneral (include/lbfgs_general.hpp,
+// include/hmc_general.hpp), driving the MI355X likelihood through
+// include/cmbl_likelihood.hpp.  TEST INFRASTRUCTURE ONLY: built by
+// oracle/Makefile (target `consumers`) into oracle/_ref/.
+//
+// Usage: ref_consumers <mode> <nside> <lmax> [iters]
+//   lbfgs          Math::LBFGS on cmbl::MapLikelihood (host vectors)
+//   lbfgs_device   Math::LBFGS_General on cmbl::DeviceLargeVector / DeviceMapFunction
+//   hmc            Math::HMC on cmbl::MapLikelihood (LikelihoodWithDerivs)
+//   hmc_device     Math::HMCGeneral on device vectors
+// Prints one JSON object on stdout.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include <function.hpp>
+#include <likelihood_function.hpp>
+#include <lbfgs.hpp>
+#include <lbfgs_general.hpp>
+#include <hmc.hpp>
+#include <hmc_general.hpp>
+
+#include <cmbl_likelihood.hpp>
+
+namespace {
+
+struct Inputs {
+    std::vector<double> cl, beam, d, w, x0;
+};
+
+Inputs make_inputs(cmbl::MapContext& ctx, int nside, int lmax)
+{
+    Inputs in;
+    const long npix = 12L * nside * nside;
+    in.cl.assign(lmax + 1, 0.0);
+    for (int l = 2; l <= lmax; ++l) in.cl[l] = 1e3 / (l * (l + 1.0));
+    in.beam.resize(lmax + 1);
+    cmbl::raise_on_error(cmbl_beam(lmax, 5.0, in.beam.data()));
+    std::vector<double> a((size_t)(lmax + 1) * (lmax + 1));
+    cmbl::raise_on_error(cmbl_simulate_alm(in.cl.data(), lmax, 501, a.data()));
+    ctx.setBeam(in.beam);
+    ctx.setCl(in.cl);
+    std::vector<double> s(npix), noise(npix), mask(npix, 1.0);
+    cmbl::raise_on_error(cmbl_forward(ctx.get(), a.data(), s.data()));
+    cmbl::raise_on_error(cmbl_simulate_white_noise(nside, 10.0, 502, noise.data()));
+    in.d.resize(npix);
+    for (long i = 0; i < npix; ++i) in.d[i] = s[i] + noise[i];
+    cmbl::raise_on_error(cmbl_mask_latitude(nside, 20.0 * M_PI / 180.0, mask.data()));
+    in.w.resize(npix);
+    cmbl::raise_on_error(cmbl_weights_from_mask(npix, mask.data(), 10.0, in.w.data()));
+    ctx.setData(in.d, in.w);
+    in.x0.assign(a.size(), 0.0);
+    return in;
+}
+
+void print_vec(const char* name, const std::vector<double>& v)
+{
+    std::printf("\"%s\": [", name);
+    for (size_t i = 0; i < v.size(); ++i) std::printf("%s%.17g", i ? "," : "", v[i]);
+    std::printf("]");
+}
+
+struct ChainStats {
+    long n = 0;
+    double like_sum = 0;
+    std::vector<double> mean;
+    void operator()(const cmbl::DeviceLargeVector& x, double like)
+    {
+        std::vector<double> h;
+        x.download(&h);
+        if (mean.empty()) mean.assign(h.size(), 0.0);
+        for (size_t i = 0; i < h.size(); ++i) mean[i] += h[i];
+        like_sum += like;
+        ++n;
+    }
+};
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc < 4) {
+        std::fprintf(stderr, "usage: %s <lbfgs|lbfgs_device|hmc|hmc_device> <nside> <lmax> [iters]\n", argv[0]);
+        return 2;
+    }
+    const std::string mode = argv[1];
+    const int nside = std::atoi(argv[2]);
+    const int lmax = std::atoi(argv[3]);
+    const int iters = argc > 4 ? std::atoi(argv[4]) : 200;
+    try {
+        cmbl::MapContext ctx(nside, lmax);
+        Inputs in = make_inputs(ctx, nside, lmax);
+        const long n = ctx.nParams();
+        if (mode == "lbfgs") {
+            cmbl::MapLikelihood like(ctx);
+            Math::LBFGS lbfgs((int)n, like, like, in.x0, 10);
+            std::vector<double> res;
+            const double f = lbfgs.minimize(&res, 1e-14, 1e-8, iters);
+            std::vector<double> g;
+            like.evaluate(res, &g);
+            double gn = 0;
+            for (double v : g) gn += v * v;
+            std::printf("{\"mode\": \"lbfgs\", \"f\": %.17g, \"gnorm\": %.17g, \"evals\": %ld, ", f, std::sqrt(gn),
+                        like.evaluations());
+            print_vec("x", res);
+            std::printf("}\n");
+        } else if (mode == "lbfgs_device") {
+            cmbl::DeviceLargeVectorFactory factory(n);
+            cmbl::DeviceMapFunction func(ctx);
+            cmbl::DeviceLargeVector start(n);
+            start.upload(in.x0);
+            Math::LBFGS_General<cmbl::DeviceLargeVector, cmbl::DeviceLargeVectorFactory, cmbl::DeviceMapFunction> lbfgs(
+                &factory, &func, start, 10);
+            cmbl::DeviceLargeVector res(n);
+            const double f = lbfgs.minimize(&res, 1e-14, 1e-8, iters);
+            cmbl::DeviceLargeVector g(n);
+            lbfgs.getGradient(&g);
+            std::vector<double> h;
+            res.download(&h);
+            std::printf("{\"mode\": \"lbfgs_device\", \"f\": %.17g, \"gnorm\": %.17g, \"evals\": %ld, ", f, g.norm(),
+                        func.evaluations());
+            print_vec("x", h);
+            std::printf("}\n");
+        } else if (mode == "hmc") {
+            cmbl::MapLikelihood like(ctx);
+            const std::string root = "hmc_chain_tmp";
+            Math::HMC hmc((int)n, like, root, 0.05, 10, 777);
+            for (long i = 0; i < n; ++i) hmc.setParam((int)i, "a" + std::to_string(i), 1.0, in.x0[i]);
+            hmc.run(iters);
+            std::ifstream chain(root + ".txt");
+            long lines = 0;
+            std::string line;
+            while (std::getline(chain, line)) ++lines;
+            std::remove((root + ".txt").c_str());
+            std::printf("{\"mode\": \"hmc\", \"chain_lines\": %ld, \"evals\": %ld}\n", lines, like.evaluations());
+        } else if (mode == "hmc_device") {
+            cmbl::DeviceLargeVectorFactory factory(n);
+            cmbl::DeviceMapFunction func(ctx);
+            cmbl::DeviceLargeVector start(n), masses(n);
+            start.upload(in.x0);
+            std::vector<double> m(n);
+            // mass ~ inverse prior variance keeps the leapfrog stable across l
+            for (int l = 0; l <= lmax; ++l)
+                for (int k = 0; k < 2 * l + 1; ++k) m[(size_t)l * l + k] = l >= 2 ? 2.0 / in.cl[l] : 1.0;
+            masses.upload(m);
+            Math::HMCGeneral<cmbl::DeviceLargeVector, cmbl::DeviceLargeVectorFactory, cmbl::DeviceMapFunction> hmc(
+                &factory, &func, start, masses, 0.5, 10, 778);
+            ChainStats cb;
+            hmc.run(iters, &cb);
+            for (double& v : cb.mean) v /= (double)cb.n;
+            std::printf("{\"mode\": \"hmc_device\", \"samples\": %ld, \"evals\": %ld, \"mean_like\": %.17g, ", cb.n,
+                        func.evaluations(), cb.like_sum / cb.n);
+            print_vec("mean", cb.mean);
+            std::printf("}\n");
+        } else {
+            std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
+            return 2;
+        }
+    } catch (std::exception& e) {
+        std::fprintf(stderr, "EXCEPTION: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -108,13 +108,20 @@ def test_config2_parity(config2):
 
 
 def test_config2_finite_differences(config2):
-    """Gradient vs central differences (step 1e-6 relative) on seeded random directions."""
+    """Gradient vs central differences (step 1e-6 relative) on seeded random directions.
+
+    -2lnL is exactly quadratic, so the difference quotient equals <g, v> up to
+    rounding of f (~1e-16 * |f|, |f| ~ 3e6).  The error of each direction is
+    measured relative to max(|<g,v>|, ||g|| ||v|| / sqrt(n)): the second term is
+    the typical size of a random projection and keeps directions that happen to
+    be nearly orthogonal to g from turning rounding into a large ratio.
+    """
     c, mp = config2
     ndir = 200
     print("case config2 fd ndir=%d" % ndir, flush=True)
     f, g = mp.value_and_grad(c.x)
     rng = np.random.default_rng(77)
-    worst = 0.0
+    errs = []
     for _ in range(ndir):
         v = rng.standard_normal(c.x.size)
         h = 1e-6 * np.linalg.norm(c.x) / np.linalg.norm(v)
@@ -122,9 +129,11 @@ def test_config2_finite_differences(config2):
         fm = mp.value_and_grad(c.x - h * v, grad=False)
         fd = (fp - fm) / (2 * h)
         gv = np.dot(g, v)
-        worst = max(worst, abs(fd - gv) / abs(gv))
-    print("config2 fd worst rel %.3e" % worst, flush=True)
-    assert worst < 1e-6
+        scale = max(abs(gv), np.linalg.norm(g) * np.linalg.norm(v) / math.sqrt(v.size))
+        errs.append(abs(fd - gv) / scale)
+    errs = np.array(errs)
+    print("config2 fd rel err max %.3e median %.3e" % (errs.max(), np.median(errs)), flush=True)
+    assert errs.max() < 1e-6
 
 
 def test_nonfinite_input_is_reported():

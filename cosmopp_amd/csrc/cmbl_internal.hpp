@@ -35,7 +35,7 @@ constexpr double kScaleUp = 0x1p+1000;
 constexpr double kScaleLo = 0x1p-100;
 
 constexpr int kWave = 64;
-constexpr int kRingThreads = 256;
+constexpr int kRingThreads = 512;
 constexpr int kMaxFFT = 8192;          // largest LDS FFT (complex doubles)
 constexpr int kRedChunk = 8;           // l-steps per cross-lane reduction in the adjoint
 constexpr int kRedStride = 17;         // doubles per lane row in the reduction buffer
@@ -53,7 +53,9 @@ struct MapDesc {
     const long* ring_ofsS;    // first pixel of the south ring, -1 for the equator
     const int* ring_M;        // FFT length used (nph if a power of two, else Bluestein length)
     const long* ring_bsofs;   // offset of the Bluestein filter in bs_filt (-1 if none)
-    const double2* bs_filt;   // FFT_M of the conjugate chirp, per non-power-of-two ring
+    const long* ring_chofs;   // offset of the Bluestein chirp in bs_chirp (-1 if none)
+    const double2* bs_filt;   // FFT_M of e^{+i pi j^2/n}, bit-reversed order, per non-power-of-two ring
+    const double2* bs_chirp;  // e^{-i pi j^2/n}, j < n, per non-power-of-two ring
     // Legendre tables
     const double* seed_v;     // [(L+1) * npair_pad] scaled lambda_mm
     const int* seed_k;        // [(L+1) * npair_pad] scale count
@@ -93,6 +95,12 @@ struct WorkItem {
     int pad_;
 };
 
+// Ring items of one launch: consecutive entries of the ring list that share
+// an LDS size (the largest transform length of the class).
+struct RingClass {
+    int first, count, lds;
+};
+
 struct ItemCost {
     WorkItem it;
     double cost;
@@ -126,8 +134,8 @@ int launch_ring(int mode, const RingItem* items, int nitems, const MapDesc* d_ma
 int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, double* d_g, int like,
                 hipStream_t s);
 int launch_final(const MapDesc* d_md, double* d_f, hipStream_t s);
-int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_bs, double2* d_out, int lds_bytes,
-                           const double2* d_tw, hipStream_t s);
+int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_bs, double2* d_out,
+                           double2* d_chirp, int lds_bytes, const double2* d_tw, hipStream_t s);
 int launch_scan(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, int* d_lstart,
                 hipStream_t s);
 int launch_capture(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, double* d_prev, double* d_cur,
@@ -154,6 +162,8 @@ struct cmbl_map {
     cmbl::WorkItem* d_items5 = nullptr;
     cmbl::RingItem* d_ring_items = nullptr;
     int nitems1 = 0, nitems5 = 0, nring_items = 0;
+    std::vector<cmbl::RingClass> ring_classes;
+    std::vector<int> ring_M;  // transform length per pair
     std::vector<cmbl::WorkItem> items1, items5;   // launch order (XCD-interleaved)
     std::vector<cmbl::ItemCost> cost1, cost5;      // live items with cost estimates
     std::vector<cmbl::RingItem> ring_items;
@@ -186,6 +196,7 @@ struct cmbl_batch {
     cmbl::RingItem* d_ring_items = nullptr;
     int nring_items = 0;
     int lds_ring = 0;
+    std::vector<cmbl::RingClass> ring_classes;
     double2* d_tw = nullptr;
     std::vector<void*> allocs;
 };

@@ -575,83 +575,137 @@ __global__ __launch_bounds__(64) void k_capture(const WorkItem* __restrict__ ite
 }
 
 // ================================================================== ring stage
+//
+// One workgroup per ring pair.  The north and south rings of a pair share
+// nph = n and phi0, so they travel as the real and imaginary parts of one
+// complex sequence of length n held in LDS:
+//   synthesis: Z_k = G_N,k + i G_S,k  ->  inverse DFT  ->  s_N + i s_S
+//   adjoint:   f_N + i f_S  ->  forward DFT  ->  Y, split into H_N, H_S.
+// Power-of-two n uses a radix-2^2 FFT directly; other n (the polar rings,
+// n = 4i) use Bluestein's chirp-z with a power-of-two length M >= 2n-1.
+// The phase of the first pixel, phi0 = pi/n, enters per Fourier bin:
+// e^{i pi m/n} = e^{i pi k/n} (-1)^j for m = k + j n.
 
-// In-place power-of-two FFT in LDS, radix-2 DIT with bit reversal.
-// tw[k] = e^{-2 pi i k / kMaxFFT}, k < kMaxFFT/2.  sign = -1 forward, +1 inverse.
-__device__ void lds_fft(double2* z, int M, int sign, const double2* __restrict__ tw)
+__device__ __forceinline__ double2 tw_get(const double2* __restrict__ tw, int t, int sign)
 {
-    const int tid = threadIdx.x;
-    const int nt = blockDim.x;
-    int logM = 0;
-    while ((1 << logM) < M) ++logM;
-    if (logM == 0) return;
-    for (int i = tid; i < M; i += nt) {
-        const int j = (int)(__brev((unsigned)i) >> (32 - logM));
-        if (i < j) {
-            const double2 t = z[i];
-            z[i] = z[j];
-            z[j] = t;
+    double2 w = tw[t];
+    if (sign > 0) w.y = -w.y;
+    return w;
+}
+// w * (sign i)
+__device__ __forceinline__ double2 mul_si(double2 w, int sign)
+{
+    return sign < 0 ? make_double2(w.y, -w.x) : make_double2(-w.y, w.x);
+}
+__device__ __forceinline__ int brev_m(int k, int logM) { return (int)(__brev((unsigned)k) >> (32 - logM)); }
+
+// Decimation in frequency, natural order in, bit-reversed order out.
+// A pass with inner span q applies the stages of span 2q and q to groups of
+// four points (one twiddle lookup per group); an odd log2(M) ends with one
+// span-1 stage.
+__device__ void fft_dif(double2* z, int M, int logM, int sign, const double2* __restrict__ tw)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int q = M >> 2; q >= 1; q >>= 2) {
+        const int ts = kMaxFFT / (4 * q);
+        for (int g = tid; g < (M >> 2); g += nt) {
+            const int k = g & (q - 1);
+            const int p0 = ((g - k) << 2) + k;
+            const double2 w4 = tw_get(tw, k * ts, sign);
+            const double2 w2 = cmul(w4, w4);
+            const double2 w4q = mul_si(w4, sign);
+            const double2 a = z[p0], b = z[p0 + q], c = z[p0 + 2 * q], d = z[p0 + 3 * q];
+            const double2 a1 = cadd(a, c), c1 = cmul(csub(a, c), w4);
+            const double2 b1 = cadd(b, d), d1 = cmul(csub(b, d), w4q);
+            z[p0] = cadd(a1, b1);
+            z[p0 + q] = cmul(csub(a1, b1), w2);
+            z[p0 + 2 * q] = cadd(c1, d1);
+            z[p0 + 3 * q] = cmul(csub(c1, d1), w2);
         }
+        __syncthreads();
     }
-    __syncthreads();
-    for (int s = 1; s <= logM; ++s) {
-        const int half = 1 << (s - 1);
-        const int tstride = kMaxFFT >> s;
+    if (logM & 1) {
         for (int b = tid; b < (M >> 1); b += nt) {
-            const int k = b & (half - 1);
-            const int i0 = ((b >> (s - 1)) << s) + k;
-            const int i1 = i0 + half;
-            double2 w = tw[k * tstride];
-            if (sign > 0) w.y = -w.y;
-            const double2 u = z[i0];
-            const double2 v = cmul(z[i1], w);
-            z[i0] = cadd(u, v);
-            z[i1] = csub(u, v);
+            const double2 u = z[2 * b], v = z[2 * b + 1];
+            z[2 * b] = cadd(u, v);
+            z[2 * b + 1] = csub(u, v);
         }
         __syncthreads();
     }
 }
 
-// DFT of length n held in z[0..n-1] (capacity M), result in z[0..n-1].
-// Power-of-two n: direct.  Otherwise Bluestein with the pair's filter
-// bf = FFT_M(e^{+i pi j^2/n}) (wrapped), which serves both signs.
-__device__ void ring_dft(double2* z, int n, int M, int sign, const double2* __restrict__ bf,
-                         const double2* __restrict__ tw)
+// Decimation in time, bit-reversed order in, natural order out.
+__device__ void fft_dit(double2* z, int M, int logM, int sign, const double2* __restrict__ tw)
 {
-    if (M == n) {
-        lds_fft(z, n, sign, tw);
-        return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    int q = 1;
+    if (logM & 1) {
+        for (int b = tid; b < (M >> 1); b += nt) {
+            const double2 u = z[2 * b], v = z[2 * b + 1];
+            z[2 * b] = cadd(u, v);
+            z[2 * b + 1] = csub(u, v);
+        }
+        __syncthreads();
+        q = 2;
     }
-    const int tid = threadIdx.x;
-    const int nt = blockDim.x;
-    // a_j = z_j c_j, c_j = e^{sign i pi j^2 / n}
+    for (; 4 * q <= M; q <<= 2) {
+        const int ts = kMaxFFT / (4 * q);
+        for (int g = tid; g < (M >> 2); g += nt) {
+            const int k = g & (q - 1);
+            const int p0 = ((g - k) << 2) + k;
+            const double2 w4 = tw_get(tw, k * ts, sign);
+            const double2 w2 = cmul(w4, w4);
+            const double2 w4q = mul_si(w4, sign);
+            const double2 a = z[p0];
+            const double2 b = cmul(z[p0 + q], w2);
+            const double2 c = z[p0 + 2 * q];
+            const double2 d = cmul(z[p0 + 3 * q], w2);
+            const double2 a1 = cadd(a, b), b1 = csub(a, b);
+            const double2 c1 = cmul(cadd(c, d), w4), d1 = cmul(csub(c, d), w4q);
+            z[p0] = cadd(a1, c1);
+            z[p0 + 2 * q] = csub(a1, c1);
+            z[p0 + q] = cadd(b1, d1);
+            z[p0 + 3 * q] = csub(b1, d1);
+        }
+        __syncthreads();
+    }
+}
+
+// Bluestein DFT of z[0..n) (natural order, z zero-padded by this call to M):
+// result in natural order in z[0..n).  chirp[j] = e^{-i pi j^2/n};
+// filt = FFT_M of e^{+i pi j^2/n} (wrapped) stored in bit-reversed order.
+__device__ void bluestein(double2* z, int n, int M, int logM, int sign, const double2* __restrict__ chirp,
+                          const double2* __restrict__ filt, const double2* __restrict__ tw)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
     for (int j = tid; j < M; j += nt) {
         if (j < n) {
-            double2 c = cis_pi_frac(((long)j * j) % (2L * n), n);
-            if (sign < 0) c.y = -c.y;
+            double2 c = chirp[j];
+            if (sign > 0) c.y = -c.y;
             z[j] = cmul(z[j], c);
         } else {
             z[j] = make_double2(0.0, 0.0);
         }
     }
     __syncthreads();
-    lds_fft(z, M, -1, tw);
-    for (int k = tid; k < M; k += nt) {
+    fft_dif(z, M, logM, -1, tw);
+    for (int r = tid; r < M; r += nt) {
         double2 f;
         if (sign < 0) {
-            f = bf[k];
+            f = filt[r];
         } else {
-            const double2 g = bf[(M - k) & (M - 1)];
+            const int k = brev_m(r, logM);
+            const double2 g = filt[brev_m((M - k) & (M - 1), logM)];
             f = make_double2(g.x, -g.y);
         }
-        z[k] = cmul(z[k], f);
+        z[r] = cmul(z[r], f);
     }
     __syncthreads();
-    lds_fft(z, M, +1, tw);
+    fft_dit(z, M, logM, +1, tw);
     const double inv = 1.0 / (double)M;
     for (int k = tid; k < n; k += nt) {
-        double2 c = cis_pi_frac(((long)k * k) % (2L * n), n);
-        if (sign < 0) c.y = -c.y;
+        double2 c = chirp[k];
+        if (sign > 0) c.y = -c.y;
         const double2 v = z[k];
         z[k] = cmul(make_double2(v.x * inv, v.y * inv), c);
     }
@@ -673,47 +727,52 @@ __global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restric
     const int p = it.pair;
     const int n = md.ring_nph[p];
     const int M = md.ring_M[p];
+    int logM = 0;
+    while ((1 << logM) < M) ++logM;
+    const bool pow2 = (M == n);
     const bool shifted = md.ring_shift[p] != 0;
     const long oN = md.ring_ofsN[p];
     const long oS = md.ring_ofsS[p];
-    const double2* bf = md.ring_bsofs[p] >= 0 ? md.bs_filt + md.ring_bsofs[p] : nullptr;
-    const double* row = md.P + (long)p * (L + 1) * 4;
-    double* qrow = md.Q + (long)p * (L + 1) * 4;
+    const double2* filt = pow2 ? nullptr : md.bs_filt + md.ring_bsofs[p];
+    const double2* chirp = pow2 ? nullptr : md.bs_chirp + md.ring_chofs[p];
     const int tid = threadIdx.x;
 
     if (MODE != kRingAna) {
-        // fold: Z_k = G_N,k + i G_S,k
+        const double2* row = reinterpret_cast<const double2*>(md.P + (long)p * (L + 1) * 4);
+        // fold: Z_k = G_N,k + i G_S,k (natural order for Bluestein, bit-reversed for the power-of-two FFT)
         for (int k = tid; k < n; k += kRingThreads) {
             double2 gN = make_double2(0.0, 0.0), gS = make_double2(0.0, 0.0);
-            for (int mm = k; mm <= L; mm += n) {
-                const double2* src = reinterpret_cast<const double2*>(row + 4L * mm);
-                const double2 e = src[0], o = src[1];
-                double2 fN = cadd(e, o), fS = csub(e, o);
-                if (shifted) {
-                    const double2 ph = cis_pi_frac(mm, n);
-                    fN = cmul(fN, ph);
-                    fS = cmul(fS, ph);
-                }
-                gN = cadd(gN, fN);
-                gS = cadd(gS, fS);
+            int j = 0;
+            for (int mm = k; mm <= L; mm += n, ++j) {
+                const double2 e = row[2 * mm], o = row[2 * mm + 1];
+                const double sg = (shifted && (j & 1)) ? -1.0 : 1.0;
+                gN.x += sg * (e.x + o.x);
+                gN.y += sg * (e.y + o.y);
+                gS.x += sg * (e.x - o.x);
+                gS.y += sg * (e.y - o.y);
             }
-            for (int mm = (k == 0 ? n : n - k); mm <= L; mm += n) {
-                const double2* src = reinterpret_cast<const double2*>(row + 4L * mm);
-                const double2 e = src[0], o = src[1];
-                double2 fN = cadd(e, o), fS = csub(e, o);
-                if (shifted) {
-                    const double2 ph = cis_pi_frac(mm, n);
-                    fN = cmul(fN, ph);
-                    fS = cmul(fS, ph);
-                }
-                gN = make_double2(gN.x + fN.x, gN.y - fN.y);
-                gS = make_double2(gS.x + fS.x, gS.y - fS.y);
+            j = 1;  // m = j n - k, j >= 1
+            for (int mm = (k == 0 ? n : n - k); mm <= L; mm += n, ++j) {
+                const double2 e = row[2 * mm], o = row[2 * mm + 1];
+                const double sg = (shifted && (j & 1)) ? -1.0 : 1.0;
+                gN.x += sg * (e.x + o.x);
+                gN.y -= sg * (e.y + o.y);
+                gS.x += sg * (e.x - o.x);
+                gS.y -= sg * (e.y - o.y);
             }
             if (oS < 0) gS = make_double2(0.0, 0.0);
-            Z[k] = make_double2(gN.x - gS.y, gN.y + gS.x);
+            if (shifted) {
+                const double2 ph = cis_pi_frac(k, n);
+                gN = cmul(gN, ph);
+                gS = cmul(gS, ph);
+            }
+            Z[pow2 ? brev_m(k, logM) : k] = make_double2(gN.x - gS.y, gN.y + gS.x);
         }
         __syncthreads();
-        ring_dft(Z, n, M, +1, bf, tw);
+        if (pow2)
+            fft_dit(Z, n, logM, +1, tw);
+        else
+            bluestein(Z, n, M, logM, +1, chirp, filt, tw);
     }
 
     if (MODE == kRingSyn) {
@@ -745,34 +804,45 @@ __global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restric
     } else {  // kRingAna: load the map to analyse
         for (int j = tid; j < n; j += kRingThreads)
             Z[j] = make_double2(map_in[oN + j], oS >= 0 ? map_in[oS + j] : 0.0);
+        __syncthreads();
     }
-    __syncthreads();
-    ring_dft(Z, n, M, -1, bf, tw);
+    if (pow2)
+        fft_dif(Z, n, logM, -1, tw);
+    else
+        bluestein(Z, n, M, logM, -1, chirp, filt, tw);
 
-    // unfold: Psi_X,m = e^{-i m phi0} H_X,(m mod n)
-    for (int mm = tid; mm <= L; mm += kRingThreads) {
-        const int b = mm % n;
-        const int nb = (b == 0) ? 0 : n - b;
-        const double2 y = Z[b];
-        const double2 yc = make_double2(Z[nb].x, -Z[nb].y);
+    // unfold: Psi_X,m = e^{-i pi k/n} (-1)^j H_X,k for m = k + j n
+    double2* qrow = reinterpret_cast<double2*>(md.Q + (long)p * (L + 1) * 4);
+    for (int k = tid; k < n && k <= L; k += kRingThreads) {
+        const int nk = (k == 0) ? 0 : n - k;
+        const double2 y = Z[pow2 ? brev_m(k, logM) : k];
+        const double2 z2 = Z[pow2 ? brev_m(nk, logM) : nk];
+        const double2 yc = make_double2(z2.x, -z2.y);
         double2 hN = make_double2(0.5 * (y.x + yc.x), 0.5 * (y.y + yc.y));
         double2 hS = make_double2(0.5 * (y.y - yc.y), -0.5 * (y.x - yc.x));
         if (shifted) {
-            double2 ph = cis_pi_frac(mm, n);
+            double2 ph = cis_pi_frac(k, n);
             ph.y = -ph.y;
             hN = cmul(hN, ph);
             hS = cmul(hS, ph);
         }
-        double2* dst = reinterpret_cast<double2*>(qrow + 4L * mm);
-        dst[0] = cadd(hN, hS);
-        dst[1] = csub(hN, hS);
+        const double2 pe = cadd(hN, hS), po = csub(hN, hS);
+        int j = 0;
+        for (int mm = k; mm <= L; mm += n, ++j) {
+            const double sg = (shifted && (j & 1)) ? -1.0 : 1.0;
+            qrow[2 * mm] = make_double2(sg * pe.x, sg * pe.y);
+            qrow[2 * mm + 1] = make_double2(sg * po.x, sg * po.y);
+        }
     }
 }
 
-// Bluestein filter: bf = FFT_M(b), b_j = e^{+i pi j^2/n} for |j| < n, wrapped mod M.
+// Bluestein set-up for one ring pair: chirp[j] = e^{-i pi j^2/n} (j < n) and
+// filt = FFT_M of b_j = e^{+i pi j^2/n} for |j| < n wrapped mod M, stored in
+// the bit-reversed order of the DIF transform.
 __global__ __launch_bounds__(kRingThreads) void k_bluestein_setup(const int* __restrict__ pairs,
                                                                   const MapDesc* __restrict__ md_,
-                                                                  double2* __restrict__ out,
+                                                                  double2* __restrict__ filt_out,
+                                                                  double2* __restrict__ chirp_out,
                                                                   const double2* __restrict__ tw)
 {
     extern __shared__ double2 Z[];
@@ -780,15 +850,25 @@ __global__ __launch_bounds__(kRingThreads) void k_bluestein_setup(const int* __r
     const int p = pairs[blockIdx.x];
     const int n = md.ring_nph[p];
     const int M = md.ring_M[p];
+    int logM = 0;
+    while ((1 << logM) < M) ++logM;
     for (int j = threadIdx.x; j < M; j += blockDim.x) {
         int jj = -1;
-        if (j < n) jj = j;
-        else if (j > M - n) jj = M - j;
+        if (j < n)
+            jj = j;
+        else if (j > M - n)
+            jj = M - j;
         Z[j] = jj >= 0 ? cis_pi_frac(((long)jj * jj) % (2L * n), n) : make_double2(0.0, 0.0);
     }
+    double2* ch = chirp_out + md.ring_chofs[p];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        double2 c = cis_pi_frac(((long)j * j) % (2L * n), n);
+        c.y = -c.y;
+        ch[j] = c;
+    }
     __syncthreads();
-    lds_fft(Z, M, -1, tw);
-    double2* dst = out + md.ring_bsofs[p];
+    fft_dif(Z, M, logM, -1, tw);
+    double2* dst = filt_out + md.ring_bsofs[p];
     for (int j = threadIdx.x; j < M; j += blockDim.x) dst[j] = Z[j];
 }
 
@@ -993,8 +1073,8 @@ int launch_ring(int mode, const RingItem* items, int n, const MapDesc* d_maps, i
     return fail(CMBL_ERR_ARG, "bad ring mode");
 }
 
-int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_bs, double2* d_out, int lds,
-                           const double2* tw, hipStream_t s)
+int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_bs, double2* d_out,
+                           double2* d_chirp, int lds, const double2* tw, hipStream_t s)
 {
     if (npairs_bs <= 0) return CMBL_OK;
     static bool attr_set = false;
@@ -1003,7 +1083,8 @@ int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_b
                                      kMaxFFT * (int)sizeof(double2)));
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_bluestein_setup, dim3(npairs_bs), dim3(kRingThreads), lds, s, d_pairs, d_md, d_out, tw);
+    hipLaunchKernelGGL(k_bluestein_setup, dim3(npairs_bs), dim3(kRingThreads), lds, s, d_pairs, d_md, d_out, d_chirp,
+                       tw);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }

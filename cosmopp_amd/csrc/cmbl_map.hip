@@ -208,6 +208,33 @@ static std::vector<WorkItem> order_items(std::vector<ItemCost>& v)
     return out;
 }
 
+// Ring items ordered by transform length, largest first, and cut into at
+// most four launches by LDS size (M > 4096, > 2048, > 512, the rest), so that
+// small transforms are not limited to one workgroup per CU by the largest.
+template <typename MOf>
+static std::vector<RingClass> ring_classes(std::vector<RingItem>* items, MOf Mof)
+{
+    std::stable_sort(items->begin(), items->end(),
+                     [&](const RingItem& a, const RingItem& b) { return Mof(a) > Mof(b); });
+    auto cls = [](int M) { return M > 4096 ? 0 : M > 2048 ? 1 : M > 512 ? 2 : 3; };
+    std::vector<RingClass> out;
+    for (int i = 0; i < (int)items->size(); ++i) {
+        const int M = Mof((*items)[i]);
+        if (out.empty() || cls(Mof((*items)[out.back().first])) != cls(M))
+            out.push_back(RingClass{i, 0, M * (int)sizeof(double2)});
+        ++out.back().count;
+    }
+    return out;
+}
+
+static int run_ring(int mode, const RingItem* d_items, const std::vector<RingClass>& classes, const MapDesc* d_maps,
+                    double* out, const double* in, const double2* tw, hipStream_t s)
+{
+    for (const RingClass& c : classes)
+        CMBL_TRY(launch_ring(mode, d_items + c.first, c.count, d_maps, c.lds, out, in, tw, s));
+    return CMBL_OK;
+}
+
 static int choose_k(int npair, int kmax)
 {
     int k = 1;
@@ -238,9 +265,9 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     // ---- ring-pair tables
     std::vector<double> rx(npad, 0.0), rs(npad, 0.0);
     std::vector<int> nph(npad, 0), shift(npad, 0), Mfft(npad, 0);
-    std::vector<long> ofsN(npad, 0), ofsS(npad, -1), bsofs(npad, -1);
+    std::vector<long> ofsN(npad, 0), ofsS(npad, -1), bsofs(npad, -1), chofs(npad, -1);
     std::vector<int> bs_pairs;
-    long bs_total = 0;
+    long bs_total = 0, ch_total = 0;
     int maxM = 1;
     for (int p = 0; p < mp->npair; ++p) {
         const RingGeom g = ring_geom(N, p + 1);
@@ -258,6 +285,8 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
             Mfft[p] = M;
             bsofs[p] = bs_total;
             bs_total += M;
+            chofs[p] = ch_total;
+            ch_total += g.nph;
             bs_pairs.push_back(p);
         }
         maxM = std::max(maxM, Mfft[p]);
@@ -345,6 +374,9 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     CMBL_TRY(dupload(mp, &p_oS, ofsS));
     CMBL_TRY(dupload(mp, &p_M, Mfft));
     CMBL_TRY(dupload(mp, &p_bo, bsofs));
+    long* p_co;
+    CMBL_TRY(dupload(mp, &p_co, chofs));
+    d.ring_chofs = p_co;
     CMBL_TRY(dupload(mp, &p_A, A));
     CMBL_TRY(dupload(mp, &p_N, Nrm));
     CMBL_TRY(dupload(mp, &p_sv, seed_v));
@@ -363,6 +395,9 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     d.seed_k = p_sk;
     double2* p_bs;
     CMBL_TRY(dalloc(mp, &p_bs, (size_t)std::max<long>(bs_total, 1)));
+    double2* p_ch;
+    CMBL_TRY(dalloc(mp, &p_ch, (size_t)std::max<long>(ch_total, 1)));
+    d.bs_chirp = p_ch;
     d.bs_filt = p_bs;
     // beam = 1, no prior until set
     CMBL_TRY(dupload(mp, &mp->d_beam, std::vector<double>(L + 1, 1.0)));
@@ -396,7 +431,7 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     if (!bs_pairs.empty()) {
         int* d_pairs;
         CMBL_TRY(dupload(mp, &d_pairs, bs_pairs));
-        CMBL_TRY(launch_bluestein_setup(mp->d_desc, d_pairs, (int)bs_pairs.size(), p_bs, mp->lds_ring, mp->d_tw,
+        CMBL_TRY(launch_bluestein_setup(mp->d_desc, d_pairs, (int)bs_pairs.size(), p_bs, p_ch, mp->lds_ring, mp->d_tw,
                                         nullptr));
         CMBL_HIP(hipDeviceSynchronize());
     }
@@ -449,8 +484,8 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     }
     // ring items: largest transform first
     for (int p = 0; p < mp->npair; ++p) mp->ring_items.push_back(RingItem{0, p});
-    std::stable_sort(mp->ring_items.begin(), mp->ring_items.end(),
-                     [&](const RingItem& a, const RingItem& b) { return Mfft[a.pair] > Mfft[b.pair]; });
+    mp->ring_M.assign(Mfft.begin(), Mfft.begin() + mp->npair);
+    mp->ring_classes = ring_classes(&mp->ring_items, [&](const RingItem& r) { return Mfft[r.pair]; });
     mp->nring_items = (int)mp->ring_items.size();
     CMBL_TRY(dupload(mp, &mp->d_ring_items, mp->ring_items));
     CMBL_HIP(hipDeviceSynchronize());
@@ -520,8 +555,7 @@ static int eval_device(cmbl_map* mp, const double* x, double* f, double* g, hipS
     CMBL_TRY(t.mark(1));
     CMBL_TRY(launch_synth(mp->K1, mp->d_items1, mp->nitems1, mp->d_desc, s));
     CMBL_TRY(t.mark(2));
-    CMBL_TRY(launch_ring(kRingLike, mp->d_ring_items, mp->nring_items, mp->d_desc, mp->lds_ring, nullptr, nullptr,
-                         mp->d_tw, s));
+    CMBL_TRY(run_ring(kRingLike, mp->d_ring_items, mp->ring_classes, mp->d_desc, nullptr, nullptr, mp->d_tw, s));
     CMBL_TRY(t.mark(3));
     CMBL_TRY(launch_adj(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, s));
     CMBL_TRY(t.mark(4));
@@ -670,8 +704,7 @@ int cmbl_forward_device(cmbl_map* mp, const double* x, double* map, void* stream
     hipStream_t s = (hipStream_t)stream;
     CMBL_TRY(launch_pre(mp->desc, mp->d_desc, x, s));
     CMBL_TRY(launch_synth(mp->K1, mp->d_items1, mp->nitems1, mp->d_desc, s));
-    CMBL_TRY(launch_ring(kRingSyn, mp->d_ring_items, mp->nring_items, mp->d_desc, mp->lds_ring, map, nullptr,
-                         mp->d_tw, s));
+    CMBL_TRY(run_ring(kRingSyn, mp->d_ring_items, mp->ring_classes, mp->d_desc, map, nullptr, mp->d_tw, s));
     return CMBL_OK;
 }
 
@@ -690,8 +723,7 @@ int cmbl_backward_device(cmbl_map* mp, const double* v, double* g, void* stream)
     CMBL_TRY(check_map(mp));
     if (!v || !g) return fail(CMBL_ERR_ARG, "null pointer");
     hipStream_t s = (hipStream_t)stream;
-    CMBL_TRY(launch_ring(kRingAna, mp->d_ring_items, mp->nring_items, mp->d_desc, mp->lds_ring, nullptr, v,
-                         mp->d_tw, s));
+    CMBL_TRY(run_ring(kRingAna, mp->d_ring_items, mp->ring_classes, mp->d_desc, nullptr, v, mp->d_tw, s));
     CMBL_TRY(launch_adj(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, s));
     CMBL_TRY(launch_post(mp->desc, mp->d_desc, nullptr, g, 0, s));
     return CMBL_OK;
@@ -768,18 +800,7 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
             }
         }
     }
-    {
-        std::vector<size_t> idx(rings.size());
-        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
-        std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t c) {
-            const cmbl_map* ma = maps[rings[a].map];
-            const cmbl_map* mc = maps[rings[c].map];
-            return ma->nside > mc->nside;
-        });
-        std::vector<RingItem> sorted;
-        for (size_t i : idx) sorted.push_back(rings[i]);
-        rings.swap(sorted);
-    }
+    b->ring_classes = ring_classes(&rings, [&](const RingItem& r) { return maps[r.map]->ring_M[r.pair]; });
     if (hipMalloc(&q, sizeof(RingItem) * rings.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
     b->allocs.push_back(q);
     b->d_ring_items = (RingItem*)q;
@@ -809,8 +830,7 @@ int cmbl_batch_value_and_grad_device(cmbl_batch* b, const double* const* x, doub
         CMBL_TRY(launch_pre(b->maps[i]->desc, b->d_descs + i, x[i], s));
     }
     for (int k = 0; k < 4; ++k) CMBL_TRY(launch_synth(1 << k, b->d_items1[k], b->n1[k], b->d_descs, s));
-    CMBL_TRY(launch_ring(kRingLike, b->d_ring_items, b->nring_items, b->d_descs, b->lds_ring, nullptr, nullptr,
-                         b->d_tw, s));
+    CMBL_TRY(run_ring(kRingLike, b->d_ring_items, b->ring_classes, b->d_descs, nullptr, nullptr, b->d_tw, s));
     for (int k = 0; k < 4; ++k) CMBL_TRY(launch_adj(1 << k, b->d_items5[k], b->n5[k], b->d_descs, s));
     for (int i = 0; i < n; ++i) {
         CMBL_TRY(launch_post(b->maps[i]->desc, b->d_descs + i, x[i], g[i], 1, s));

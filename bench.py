@@ -109,7 +109,22 @@ def main():
     ap.add_argument("--config", type=int, default=4, choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=40.0, help="seconds for the CPU baseline leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import torch
+
+        n = args.copy_probe * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(max(args.steps, 1)):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
+                          "copies": max(args.steps, 1)}))
+        return
 
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has a single MI355X, and the "

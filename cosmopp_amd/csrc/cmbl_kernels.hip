@@ -712,6 +712,48 @@ __device__ void bluestein(double2* z, int n, int M, int logM, int sign, const do
     __syncthreads();
 }
 
+// Alias sums of Fourier bin k over the parity rows (E, O) of a ring pair:
+// sum_j (-1)^{j shift} F_{k+jn} + sum_{j>=1} (-1)^{j shift} conj(F_{jn-k}),
+// for j = r, r+R, r+2R, ... (one thread group of R).
+__device__ __forceinline__ void fold_aliases(const double2* __restrict__ row, int L, int n, int k, int r, int R,
+                                             bool shifted, double2& gN, double2& gS)
+{
+    gN = make_double2(0.0, 0.0);
+    gS = make_double2(0.0, 0.0);
+    for (int j = r; k + (long)j * n <= L; j += R) {
+        const int mm = k + j * n;
+        const double2 e = row[2 * mm], o = row[2 * mm + 1];
+        const double sg = (shifted && (j & 1)) ? -1.0 : 1.0;
+        gN.x += sg * (e.x + o.x);
+        gN.y += sg * (e.y + o.y);
+        gS.x += sg * (e.x - o.x);
+        gS.y += sg * (e.y - o.y);
+    }
+    for (int j = 1 + r; (long)j * n - k <= L; j += R) {
+        const int mm = j * n - k;
+        const double2 e = row[2 * mm], o = row[2 * mm + 1];
+        const double sg = (shifted && (j & 1)) ? -1.0 : 1.0;
+        gN.x += sg * (e.x + o.x);
+        gN.y -= sg * (e.y + o.y);
+        gS.x += sg * (e.x - o.x);
+        gS.y -= sg * (e.y - o.y);
+    }
+}
+
+// Bin k of the packed pair spectrum: phase e^{i pi k/n} for shifted rings,
+// no south ring at the equator, bit-reversed position for the power-of-two FFT.
+__device__ __forceinline__ void fold_store(double2* Z, int k, int n, int logM, bool pow2, bool shifted, bool no_south,
+                                           double2 gN, double2 gS)
+{
+    if (no_south) gS = make_double2(0.0, 0.0);
+    if (shifted) {
+        const double2 ph = cis_pi_frac(k, n);
+        gN = cmul(gN, ph);
+        gS = cmul(gS, ph);
+    }
+    Z[pow2 ? brev_m(k, logM) : k] = make_double2(gN.x - gS.y, gN.y + gS.x);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restrict__ items,
                                                        const MapDesc* __restrict__ maps,
@@ -739,34 +781,35 @@ __global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restric
 
     if (MODE != kRingAna) {
         const double2* row = reinterpret_cast<const double2*>(md.P + (long)p * (L + 1) * 4);
-        // fold: Z_k = G_N,k + i G_S,k (natural order for Bluestein, bit-reversed for the power-of-two FFT)
-        for (int k = tid; k < n; k += kRingThreads) {
+        // fold: Z_k = G_N,k + i G_S,k (natural order for Bluestein, bit-reversed
+        // for the power-of-two FFT).  For n < kRingThreads the aliases of bin k
+        // are split over R = kRingThreads / n thread groups (reads stay
+        // coalesced over k) and the R partial sums are added in a fixed order.
+        const int R = n < kRingThreads ? kRingThreads / n : 1;
+        if (R == 1) {
+            for (int k = tid; k < n; k += kRingThreads) {
+                double2 gN, gS;
+                fold_aliases(row, L, n, k, 0, 1, shifted, gN, gS);
+                fold_store(Z, k, n, logM, pow2, shifted, oS < 0, gN, gS);
+            }
+        } else {
             double2 gN = make_double2(0.0, 0.0), gS = make_double2(0.0, 0.0);
-            int j = 0;
-            for (int mm = k; mm <= L; mm += n, ++j) {
-                const double2 e = row[2 * mm], o = row[2 * mm + 1];
-                const double sg = (shifted && (j & 1)) ? -1.0 : 1.0;
-                gN.x += sg * (e.x + o.x);
-                gN.y += sg * (e.y + o.y);
-                gS.x += sg * (e.x - o.x);
-                gS.y += sg * (e.y - o.y);
+            if (tid < R * n) {
+                fold_aliases(row, L, n, tid % n, tid / n, R, shifted, gN, gS);
+                Z[2 * tid] = gN;
+                Z[2 * tid + 1] = gS;
             }
-            j = 1;  // m = j n - k, j >= 1
-            for (int mm = (k == 0 ? n : n - k); mm <= L; mm += n, ++j) {
-                const double2 e = row[2 * mm], o = row[2 * mm + 1];
-                const double sg = (shifted && (j & 1)) ? -1.0 : 1.0;
-                gN.x += sg * (e.x + o.x);
-                gN.y -= sg * (e.y + o.y);
-                gS.x += sg * (e.x - o.x);
-                gS.y -= sg * (e.y - o.y);
+            __syncthreads();
+            if (tid < n) {
+                gN = make_double2(0.0, 0.0);
+                gS = make_double2(0.0, 0.0);
+                for (int r = 0; r < R; ++r) {
+                    gN = cadd(gN, Z[2 * (r * n + tid)]);
+                    gS = cadd(gS, Z[2 * (r * n + tid) + 1]);
+                }
             }
-            if (oS < 0) gS = make_double2(0.0, 0.0);
-            if (shifted) {
-                const double2 ph = cis_pi_frac(k, n);
-                gN = cmul(gN, ph);
-                gS = cmul(gS, ph);
-            }
-            Z[pow2 ? brev_m(k, logM) : k] = make_double2(gN.x - gS.y, gN.y + gS.x);
+            __syncthreads();
+            if (tid < n) fold_store(Z, tid, n, logM, pow2, shifted, oS < 0, gN, gS);
         }
         __syncthreads();
         if (pow2)
@@ -811,9 +854,13 @@ __global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restric
     else
         bluestein(Z, n, M, logM, -1, chirp, filt, tw);
 
-    // unfold: Psi_X,m = e^{-i pi k/n} (-1)^j H_X,k for m = k + j n
+    // unfold: Psi_X,m = e^{-i pi k/n} (-1)^j H_X,k for m = k + j n, with the
+    // aliases of bin k split over R thread groups as in the fold
     double2* qrow = reinterpret_cast<double2*>(md.Q + (long)p * (L + 1) * 4);
-    for (int k = tid; k < n && k <= L; k += kRingThreads) {
+    const int R = n < kRingThreads ? kRingThreads / n : 1;
+    for (int t = tid; t < R * n; t += kRingThreads) {
+        const int k = t % n, r = t / n;
+        if (k > L) continue;
         const int nk = (k == 0) ? 0 : n - k;
         const double2 y = Z[pow2 ? brev_m(k, logM) : k];
         const double2 z2 = Z[pow2 ? brev_m(nk, logM) : nk];
@@ -827,8 +874,8 @@ __global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restric
             hS = cmul(hS, ph);
         }
         const double2 pe = cadd(hN, hS), po = csub(hN, hS);
-        int j = 0;
-        for (int mm = k; mm <= L; mm += n, ++j) {
+        for (int j = r; k + (long)j * n <= L; j += R) {
+            const int mm = k + j * n;
             const double sg = (shifted && (j & 1)) ? -1.0 : 1.0;
             qrow[2 * mm] = make_double2(sg * pe.x, sg * pe.y);
             qrow[2 * mm + 1] = make_double2(sg * po.x, sg * po.y);

@@ -221,7 +221,8 @@ static std::vector<RingClass> ring_classes(std::vector<RingItem>* items, MOf Mof
     for (int i = 0; i < (int)items->size(); ++i) {
         const int M = Mof((*items)[i]);
         if (out.empty() || cls(Mof((*items)[out.back().first])) != cls(M))
-            out.push_back(RingClass{i, 0, M * (int)sizeof(double2)});
+            // at least 2 * kRingThreads entries: the fold's partial sums live in LDS
+            out.push_back(RingClass{i, 0, std::max(M, 2 * kRingThreads) * (int)sizeof(double2)});
         ++out.back().count;
     }
     return out;

@@ -40,6 +40,7 @@ constexpr int kMaxFFT = 8192;          // largest LDS FFT (complex doubles)
 constexpr int kRedChunk = 8;           // l-steps per cross-lane reduction in the adjoint
 constexpr int kRedStride = 17;         // doubles per lane row in the reduction buffer
 constexpr int kChunk = 64;             // l-steps per LDS-staged coefficient chunk
+constexpr int kChunkPad = 8;           // padding for speculative reads past a chunk
 
 struct MapDesc {
     int nside, L, npair, npair_pad, nring;

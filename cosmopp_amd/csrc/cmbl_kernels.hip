@@ -12,6 +12,8 @@
 //   k_adj     adjoint Legendre: u_lm = sum_rings lambda_lm Psi_m
 //   k_post    gradient assembly into the packed layout
 //   k_final   deterministic -2lnL reduction
+#include <cstdlib>
+
 #include "cmbl_internal.hpp"
 
 namespace cmbl {
@@ -202,41 +204,78 @@ __device__ __forceinline__ void syn_gated(LegState<K>& st, SynAcc<K>& ac, double
     }
 }
 
+// Four live steps with coefficients c (A), r/i (B_l N_lm a_lm), parity
+// pattern ODD0, !ODD0, ODD0, !ODD0.
+template <int K, bool ODD0>
+__device__ __forceinline__ void syn_live4(LegState<K>& st, SynAcc<K>& ac, const double (&c)[4], const double (&r)[4],
+                                          const double (&i)[4])
+{
+    syn_live<K, ODD0>(st, ac, c[0], r[0], i[0]);
+    syn_live<K, !ODD0>(st, ac, c[1], r[1], i[1]);
+    syn_live<K, ODD0>(st, ac, c[2], r[2], i[2]);
+    syn_live<K, !ODD0>(st, ac, c[3], r[3], i[3]);
+}
+
+template <int K, bool ODD0>
+__device__ __forceinline__ void syn_load4(int j, const double* cA, const double2* ca, double (&c)[4], double (&r)[4],
+                                          double (&i)[4])
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        c[q] = cA[j + q];
+        r[q] = ca[j + q].x;
+        i[q] = ca[j + q].y;
+    }
+}
+
+// Groups of four live steps.  Two register sets alternate: each iteration
+// does eight steps (load Y, compute X, load next X, compute Y), so loads
+// overlap arithmetic without register copies.  The speculative load of the
+// next X may read up to kChunkPad entries past the chunk; the LDS rows are
+// padded for it and the values are not used.
+template <int K, bool ODD0>
+__device__ __forceinline__ void syn_live_run(LegState<K>& st, SynAcc<K>& ac, int& l, int cend, int cb,
+                                             const double* cA, const double2* ca)
+{
+    if (l + 3 > cend) return;
+    double xc[4], xr[4], xi[4], yc[4], yr[4], yi[4];
+    syn_load4<K, ODD0>(l - cb, cA, ca, xc, xr, xi);
+    for (; l + 7 <= cend; l += 8) {
+        syn_load4<K, ODD0>(l + 4 - cb, cA, ca, yc, yr, yi);
+        syn_live4<K, ODD0>(st, ac, xc, xr, xi);
+        syn_load4<K, ODD0>(l + 8 - cb, cA, ca, xc, xr, xi);
+        syn_live4<K, ODD0>(st, ac, yc, yr, yi);
+    }
+    if (l + 3 <= cend) {
+        syn_live4<K, ODD0>(st, ac, xc, xr, xi);
+        l += 4;
+    }
+}
+
 // Steps l = l .. cend of one staged chunk (coefficients cA[l - cb], ca[l - cb]).
+// Scaled path: rescale checks after every second step (|mu| grows by less
+// than 2^7 per step, so 2^900 * 2^14 stays finite).
 template <int K>
 __device__ __forceinline__ void syn_chunk(LegState<K>& st, SynAcc<K>& ac, int& l, int cend, int cb, int m,
                                           const double* cA, const double2* ca)
 {
     while (!st.all && l <= cend) {
-        const double Al = cA[l - cb];
-        const double2 al = ca[l - cb];
-        st.advance(Al);
-        if ((l - m) & 1)
-            syn_gated<K, true>(st, ac, al.x, al.y);
-        else
-            syn_gated<K, false>(st, ac, al.x, al.y);
+        for (int rep = 0; rep < 2 && l <= cend; ++rep, ++l) {
+            const double2 al = ca[l - cb];
+            st.advance(cA[l - cb]);
+            if ((l - m) & 1)
+                syn_gated<K, true>(st, ac, al.x, al.y);
+            else
+                syn_gated<K, false>(st, ac, al.x, al.y);
+        }
         if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
-        ++l;
     }
     if (l > cend) return;
-    // all lanes live: four steps per iteration, parity pattern fixed by (l - m)
-    if ((l - m) & 1) {
-        for (; l + 3 <= cend; l += 4) {
-            const int j = l - cb;
-            syn_live<K, true>(st, ac, cA[j], ca[j].x, ca[j].y);
-            syn_live<K, false>(st, ac, cA[j + 1], ca[j + 1].x, ca[j + 1].y);
-            syn_live<K, true>(st, ac, cA[j + 2], ca[j + 2].x, ca[j + 2].y);
-            syn_live<K, false>(st, ac, cA[j + 3], ca[j + 3].x, ca[j + 3].y);
-        }
-    } else {
-        for (; l + 3 <= cend; l += 4) {
-            const int j = l - cb;
-            syn_live<K, false>(st, ac, cA[j], ca[j].x, ca[j].y);
-            syn_live<K, true>(st, ac, cA[j + 1], ca[j + 1].x, ca[j + 1].y);
-            syn_live<K, false>(st, ac, cA[j + 2], ca[j + 2].x, ca[j + 2].y);
-            syn_live<K, true>(st, ac, cA[j + 3], ca[j + 3].x, ca[j + 3].y);
-        }
-    }
+    // all lanes live: parity pattern fixed by (l - m)
+    if ((l - m) & 1)
+        syn_live_run<K, true>(st, ac, l, cend, cb, cA, ca);
+    else
+        syn_live_run<K, false>(st, ac, l, cend, cb, cA, ca);
     for (; l <= cend; ++l) {
         const int j = l - cb;
         if ((l - m) & 1)
@@ -249,8 +288,8 @@ __device__ __forceinline__ void syn_chunk(LegState<K>& st, SynAcc<K>& ac, int& l
 template <int K>
 __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
 {
-    __shared__ double sA[2][kChunk];
-    __shared__ double2 sa[2][kChunk];
+    __shared__ double sA[2][kChunk + kChunkPad];
+    __shared__ double2 sa[2][kChunk + kChunkPad];
     const WorkItem it = items[blockIdx.x];
     if (it.mcnt == 0) return;
     const MapDesc& md = maps[it.map];
@@ -312,6 +351,108 @@ __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items
         __syncthreads();
         buf ^= 1;
         cb = nb;
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (p[k] < md.npair) {
+            double2* dst = reinterpret_cast<double2*>(md.P + ((long)p[k] * (L + 1) + m) * 4);
+            dst[0] = make_double2(ac.er[k], ac.ei[k]);
+            dst[1] = make_double2(ac.orr[k], ac.oi[k]);
+        }
+    }
+}
+
+// Variant with SGPR operands: the coefficients stream through scalar loads
+// (constant address space) with an 8-step software prefetch, so no LDS ->
+// VGPR broadcast traffic is spent on wave-uniform values.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+
+template <int K>
+__global__ __launch_bounds__(64) void k_synth_s(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
+{
+    const WorkItem it = items[blockIdx.x];
+    if (it.mcnt == 0) return;
+    const MapDesc& md = maps[it.map];
+    const int L = md.L;
+    const int npad = md.npair_pad;
+    const int lane = threadIdx.x;
+    const int m = it.m0;
+    const int l0 = it.lfirst;
+    const long base = moff(L, m) - m;
+    cptr<double> A = (cptr<double>)(md.A + base);
+    cptr<double> a = (cptr<double>)(reinterpret_cast<const double*>(md.alm + base));
+
+    LegState<K> st;
+    SynAcc<K> ac;
+    int p[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        p[k] = it.rb * 64 * K + k * 64 + lane;
+        const long q = (long)m * npad + p[k];
+        st.x[k] = md.ring_x[p[k]];
+        st.mu1[k] = md.s1_cur[q];
+        st.mu2[k] = md.s1_prev[q];
+        st.sc[k] = md.s1_k[q];
+        ac.er[k] = ac.ei[k] = ac.orr[k] = ac.oi[k] = 0.0;
+    }
+    st.refresh();
+    if ((l0 - m) & 1)
+        syn_gated<K, true>(st, ac, a[2 * l0], a[2 * l0 + 1]);
+    else
+        syn_gated<K, false>(st, ac, a[2 * l0], a[2 * l0 + 1]);
+    int l = l0 + 1;
+    while (!st.all && l <= L) {
+        for (int rep = 0; rep < 2 && l <= L; ++rep, ++l) {
+            st.advance(A[l]);
+            if ((l - m) & 1)
+                syn_gated<K, true>(st, ac, a[2 * l], a[2 * l + 1]);
+            else
+                syn_gated<K, false>(st, ac, a[2 * l], a[2 * l + 1]);
+        }
+        if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
+    }
+    // live: groups of four steps, the next two groups' coefficients in flight
+    if (l + 11 <= L) {
+        double c0[4], r0[4], i0[4], c1[4], r1[4], i1[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            c0[q] = A[l + q];
+            r0[q] = a[2 * (l + q)];
+            i0[q] = a[2 * (l + q) + 1];
+            c1[q] = A[l + 4 + q];
+            r1[q] = a[2 * (l + 4 + q)];
+            i1[q] = a[2 * (l + 4 + q) + 1];
+        }
+        const bool odd = ((l - m) & 1) != 0;
+        for (; l + 11 <= L; l += 4) {
+            double c2[4], r2[4], i2[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                c2[q] = A[l + 8 + q];
+                r2[q] = a[2 * (l + 8 + q)];
+                i2[q] = a[2 * (l + 8 + q) + 1];
+            }
+            if (odd)
+                syn_live4<K, true>(st, ac, c0, r0, i0);
+            else
+                syn_live4<K, false>(st, ac, c0, r0, i0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                c0[q] = c1[q];
+                r0[q] = r1[q];
+                i0[q] = i1[q];
+                c1[q] = c2[q];
+                r1[q] = r2[q];
+                i1[q] = i2[q];
+            }
+        }
+    }
+    for (; l <= L; ++l) {
+        if ((l - m) & 1)
+            syn_live<K, true>(st, ac, A[l], a[2 * l], a[2 * l + 1]);
+        else
+            syn_live<K, false>(st, ac, A[l], a[2 * l], a[2 * l + 1]);
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -391,36 +532,93 @@ __device__ __forceinline__ void adj_gated(const LegState<K>& st, const AdjIn<K>&
     rd.push(sr, si);
 }
 
+// Eight live steps starting at an empty reduction row, parities alternating
+// from `odd`; the row slots are static and the row is flushed at the end.
+template <int K>
+__device__ __forceinline__ void adj_live8(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, bool odd,
+                                          const double (&c)[8])
+{
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const bool o = odd ^ (q & 1);
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const double nu = fma(c[q] * st.x[k], st.mu1[k], -st.mu2[k]);
+            st.mu2[k] = st.mu1[k];
+            st.mu1[k] = nu;
+            sr = fma(nu, o ? in.por[k] : in.per[k], sr);
+            si = fma(nu, o ? in.poi[k] : in.pei[k], si);
+        }
+        rd.mine[2 * q] = sr;
+        rd.mine[2 * q + 1] = si;
+    }
+    rd.cnt = kRedChunk;
+    rd.flush();
+}
+
+// Two live steps; the first has parity `odd`.
+template <int K>
+__device__ __forceinline__ void adj_live2(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, bool odd, double c0,
+                                          double c1)
+{
+    if (odd) {
+        adj_live<K, true>(st, in, rd, c0);
+        adj_live<K, false>(st, in, rd, c1);
+    } else {
+        adj_live<K, false>(st, in, rd, c0);
+        adj_live<K, true>(st, in, rd, c1);
+    }
+}
+
 template <int K>
 __device__ __forceinline__ void adj_chunk(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, int& l, int cend, int cb,
                                           int m, const double* cA)
 {
     while (!st.all && l <= cend) {
-        st.advance(cA[l - cb]);
-        if ((l - m) & 1)
-            adj_gated<K, true>(st, in, rd);
-        else
-            adj_gated<K, false>(st, in, rd);
+        for (int rep = 0; rep < 2 && l <= cend; ++rep, ++l) {
+            st.advance(cA[l - cb]);
+            if ((l - m) & 1)
+                adj_gated<K, true>(st, in, rd);
+            else
+                adj_gated<K, false>(st, in, rd);
+        }
         if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
-        ++l;
     }
-    if ((l - m) & 1) {
-        for (; l + 1 <= cend; l += 2) {
-            adj_live<K, true>(st, in, rd, cA[l - cb]);
-            adj_live<K, false>(st, in, rd, cA[l + 1 - cb]);
-        }
-    } else {
-        for (; l + 1 <= cend; l += 2) {
-            adj_live<K, false>(st, in, rd, cA[l - cb]);
-            adj_live<K, true>(st, in, rd, cA[l + 1 - cb]);
-        }
-    }
-    if (l <= cend) {
+    // live: align to the reduction chunk with single steps, then 16 steps per
+    // iteration as two halves of kRedChunk = 8 (static LDS row slots, one flush
+    // per half) with two coefficient register sets alternating; the
+    // speculative load may read past the chunk into the padding
+    while (rd.cnt != 0 && l <= cend) {
         if ((l - m) & 1)
             adj_live<K, true>(st, in, rd, cA[l - cb]);
         else
             adj_live<K, false>(st, in, rd, cA[l - cb]);
         ++l;
+    }
+    if (l + 7 <= cend) {
+        const bool odd = ((l - m) & 1) != 0;
+        double x[8], y[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = cA[l - cb + q];
+        for (; l + 15 <= cend; l += 16) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) y[q] = cA[l - cb + 8 + q];
+            adj_live8<K>(st, in, rd, odd, x);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) x[q] = cA[l - cb + 16 + q];
+            adj_live8<K>(st, in, rd, odd, y);
+        }
+        if (l + 7 <= cend) {
+            adj_live8<K>(st, in, rd, odd, x);
+            l += 8;
+        }
+    }
+    for (; l <= cend; ++l) {
+        if ((l - m) & 1)
+            adj_live<K, true>(st, in, rd, cA[l - cb]);
+        else
+            adj_live<K, false>(st, in, rd, cA[l - cb]);
     }
 }
 
@@ -428,7 +626,7 @@ template <int K>
 __global__ __launch_bounds__(64) void k_adj(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
 {
     __shared__ double red[64 * kRedStride];
-    __shared__ double sA[2][kChunk];
+    __shared__ double sA[2][kChunk + kChunkPad];
     const WorkItem it = items[blockIdx.x];
     if (it.mcnt == 0) return;
     const MapDesc& md = maps[it.map];
@@ -1016,11 +1214,24 @@ int launch_pre(const MapDesc& md, const MapDesc* d_md, const double* d_x, hipStr
     return CMBL_OK;
 }
 
+static int synth_variant()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char* e = std::getenv("CMBL_SYNTH_SMEM");
+        v = (e && e[0] == '1') ? 1 : 0;
+    }
+    return v;
+}
+
 template <int K>
 static int synth_k(const WorkItem* items, int n, const MapDesc* d_maps, hipStream_t s)
 {
     if (n <= 0) return CMBL_OK;
-    hipLaunchKernelGGL(k_synth<K>, dim3(n), dim3(64), 0, s, items, d_maps);
+    if (synth_variant())
+        hipLaunchKernelGGL(k_synth_s<K>, dim3(n), dim3(64), 0, s, items, d_maps);
+    else
+        hipLaunchKernelGGL(k_synth<K>, dim3(n), dim3(64), 0, s, items, d_maps);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }

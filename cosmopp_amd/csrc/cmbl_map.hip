@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <string>
@@ -255,8 +256,19 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     mp->nalm = (long)(L + 1) * (L + 2) / 2;
     mp->nparam = (long)(L + 1) * (L + 1);
     CMBL_HIP(hipGetDevice(&mp->device));
-    mp->K1 = choose_k(mp->npair, 4);
-    mp->K5 = choose_k(mp->npair, 8);
+    // synthesis: no cross-lane reduction, so small ring blocks (less mixed
+    // live/negligible work per wave); adjoint: four rings per lane amortise
+    // the per-l lane reduction
+    mp->K1 = choose_k(mp->npair, 2);
+    mp->K5 = choose_k(mp->npair, 4);
+    // tuning override: CMBL_K1 / CMBL_K5 in {1, 2, 4, 8}
+    for (int which = 0; which < 2; ++which) {
+        const char* e = std::getenv(which ? "CMBL_K5" : "CMBL_K1");
+        if (e) {
+            const int v = std::atoi(e);
+            if (v == 1 || v == 2 || v == 4 || v == 8) (which ? mp->K5 : mp->K1) = v;
+        }
+    }
     const int kmax = std::max(mp->K1, mp->K5);
     mp->npair_pad = ((mp->npair + 64 * kmax - 1) / (64 * kmax)) * (64 * kmax);
     mp->nrb1 = mp->npair_pad / (64 * mp->K1);

@@ -103,6 +103,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("cosmopp_amd: %s is missing; run `python -m cosmopp_amd.build` "
                                "(there is no CPU fallback)" % LIB_PATH)
+        # A torch wheel for ROCm bundles its own HIP and HSA runtimes.  Its HSA
+        # runtime has the soname this library needs (libhsa-runtime64.so.1), so
+        # loading torch first makes both share one HSA runtime; loaded the other
+        # way round, torch's HIP opens a second HSA runtime that finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         l = ctypes.CDLL(LIB_PATH)
         for name, (args, res) in _SIGS.items():
             f = getattr(l, name)

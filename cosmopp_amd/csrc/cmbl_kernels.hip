@@ -49,6 +49,18 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Ordering point for LDS data exchanged inside one wavefront.  The Legendre
+// kernels run one wavefront per workgroup and the LDS executes the operations
+// of a wavefront in issue order, so a store is visible to every later load of
+// the same wavefront: the compiler only must not move memory operations across
+// this point.  Unlike __syncthreads() it costs no lgkmcnt(0) wait.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Block-wide sum for blockDim.x == nthreads (multiple of 64); result valid in thread 0.
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* sh)
@@ -229,8 +241,8 @@ __device__ __forceinline__ void syn_load4(int j, const double* cA, const double2
 }
 
 // Groups of four live steps.  Two register sets alternate: each iteration
-// does eight steps (load Y, compute X, load next X, compute Y), so loads
-// overlap arithmetic without register copies.  The speculative load of the
+// does eight steps (compute X while Y loads, compute Y while the next X
+// loads), so loads overlap arithmetic without register copies.  The speculative load of the
 // next X may read up to kChunkPad entries past the chunk; the LDS rows are
 // padded for it and the values are not used.
 template <int K, bool ODD0>
@@ -241,10 +253,21 @@ __device__ __forceinline__ void syn_live_run(LegState<K>& st, SynAcc<K>& ac, int
     double xc[4], xr[4], xi[4], yc[4], yr[4], yi[4];
     syn_load4<K, ODD0>(l - cb, cA, ca, xc, xr, xi);
     for (; l + 7 <= cend; l += 8) {
+        // each set's loads are issued after the first step that uses the
+        // other set, so the wait before that step covers only loads issued
+        // half an iteration earlier
+        syn_live<K, ODD0>(st, ac, xc[0], xr[0], xi[0]);
+        __builtin_amdgcn_sched_barrier(0);
         syn_load4<K, ODD0>(l + 4 - cb, cA, ca, yc, yr, yi);
-        syn_live4<K, ODD0>(st, ac, xc, xr, xi);
+        syn_live<K, !ODD0>(st, ac, xc[1], xr[1], xi[1]);
+        syn_live<K, ODD0>(st, ac, xc[2], xr[2], xi[2]);
+        syn_live<K, !ODD0>(st, ac, xc[3], xr[3], xi[3]);
+        syn_live<K, ODD0>(st, ac, yc[0], yr[0], yi[0]);
+        __builtin_amdgcn_sched_barrier(0);
         syn_load4<K, ODD0>(l + 8 - cb, cA, ca, xc, xr, xi);
-        syn_live4<K, ODD0>(st, ac, yc, yr, yi);
+        syn_live<K, !ODD0>(st, ac, yc[1], yr[1], yi[1]);
+        syn_live<K, ODD0>(st, ac, yc[2], yr[2], yi[2]);
+        syn_live<K, !ODD0>(st, ac, yc[3], yr[3], yi[3]);
     }
     if (l + 3 <= cend) {
         syn_live4<K, ODD0>(st, ac, xc, xr, xi);
@@ -333,7 +356,7 @@ __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items
     }
     sA[0][lane] = tA;
     sa[0][lane] = ta;
-    __syncthreads();
+    wave_sync();
     int buf = 0;
     int l = cb;
     while (cb <= L) {
@@ -348,7 +371,7 @@ __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items
             sA[buf ^ 1][lane] = tA;
             sa[buf ^ 1][lane] = ta;
         }
-        __syncthreads();
+        wave_sync();
         buf ^= 1;
         cb = nb;
     }
@@ -474,14 +497,43 @@ struct AdjIn {
     double per[K], pei[K], por[K], poi[K];
 };
 
+// A double in the global address space: stores through it compile to
+// global_store (counted by vmcnt only) rather than flat_store, whose lgkmcnt
+// share would make every LDS wait a full drain.
+typedef __attribute__((address_space(1))) double gdouble;
+
+// Lanes 0..31 get a summed over lanes (j, j + 32), lanes 32..63 get b summed
+// likewise (v_permlane32_swap exchanges lanes 32..63 of its first operand
+// with lanes 0..31 of its second).
+__device__ __forceinline__ double swap32_add(double a, double b)
+{
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b),
+                                                     false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b),
+                                                     false, false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+// Rows of 16 lanes: rows 0 and 2 get a summed over row pairs (0,1) and (2,3),
+// rows 1 and 3 get b summed likewise (v_permlane16_swap exchanges the odd rows
+// of its first operand with the even rows of its second).
+__device__ __forceinline__ double swap16_add(double a, double b)
+{
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b),
+                                                     false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b),
+                                                     false, false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
 struct AdjRed {
     double* red;   // [64 * kRedStride]
     double* mine;  // this lane's row
-    double* dst;   // output row, dst[2l], dst[2l+1]
+    double* dst;   // output row, dst[2l], dst[2l+1] (global memory)
     int cnt, lbase, lane;
     __device__ __forceinline__ void flush()
     {
-        __syncthreads();
+        wave_sync();
         const int c = lane & 15;
         const int q = lane >> 4;
         double s = 0.0;
@@ -489,8 +541,8 @@ struct AdjRed {
         for (int i = 0; i < 16; ++i) s += red[(16 * q + i) * kRedStride + c];
         s += __shfl_xor(s, 16);
         s += __shfl_xor(s, 32);
-        if (lane < 2 * cnt) dst[2 * lbase + lane] = s;
-        __syncthreads();
+        if (lane < 2 * cnt) ((gdouble*)dst)[2 * lbase + lane] = s;
+        wave_sync();
         lbase += cnt;
         cnt = 0;
     }
@@ -499,6 +551,49 @@ struct AdjRed {
         mine[2 * cnt] = sr;
         mine[2 * cnt + 1] = si;
         if (++cnt == kRedChunk) flush();
+    }
+    // Rows of kRedChunk = 8 steps held in registers, software-pipelined over
+    // the next row's steps: fold_row(v0, v1, z) folds a row to 16 columns with
+    // two swap-and-add stages (registers only); tile_store puts the column
+    // partials in the LDS tile, tile_load fetches a lane's four partials back,
+    // finish_row sums and stores the row.  v0 = values 0..7, v1 = values 8..15
+    // (value 2q + c is step q, c = 0 real / 1 imaginary).  The tile holds lane
+    // L's 4 values at 4 L + 4 (L >> 4) (the skew spreads a 16-lane read over
+    // distinct banks).
+    __device__ __forceinline__ void fold_row(const double (&v0)[8], const double (&v1)[8], double (&z)[4]) const
+    {
+        double w[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) w[i] = swap32_add(v0[i], v1[i]);   // value i + 8 (lane >> 5)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) z[i] = swap16_add(w[i], w[i + 4]);  // value i + 4 (lane >> 4)
+    }
+    __device__ __forceinline__ void tile_store(const double (&z)[4])
+    {
+        double* t = red + 4 * lane + 4 * (lane >> 4);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) t[i] = z[i];
+        wave_sync();
+    }
+    __device__ __forceinline__ void tile_load(double (&t4)[4]) const
+    {
+        const int o = lane & 15, part = lane >> 4, r = o >> 2, i = o & 3;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int src = 16 * r + 4 * part + c;
+            t4[c] = red[4 * src + 4 * (src >> 4) + i];
+        }
+    }
+    __device__ __forceinline__ void finish_row(const double (&t4)[4])
+    {
+        double s = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+        s = swap32_add(s, s);
+        s = swap16_add(s, s);
+        // every lane holds the sum of column lane & 15: the four copies store
+        // the same value (no branch, so the store stays in the loop's block)
+        gdouble* g = (gdouble*)dst;
+        g[2 * lbase + (lane & 15)] = s;
+        lbase += kRedChunk;
     }
 };
 
@@ -532,48 +627,38 @@ __device__ __forceinline__ void adj_gated(const LegState<K>& st, const AdjIn<K>&
     rd.push(sr, si);
 }
 
-// Eight live steps starting at an empty reduction row, parities alternating
-// from `odd`; the row slots are static and the row is flushed at the end.
+// One live step returning its (real, imaginary) lane sums; o = parity of l - m.
 template <int K>
-__device__ __forceinline__ void adj_live8(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, bool odd,
-                                          const double (&c)[8])
+__device__ __forceinline__ void adj_step_v(LegState<K>& st, const AdjIn<K>& in, bool o, double Al, double& sr,
+                                           double& si)
 {
+    sr = 0.0;
+    si = 0.0;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const bool o = odd ^ (q & 1);
-        double sr = 0.0, si = 0.0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const double nu = fma(c[q] * st.x[k], st.mu1[k], -st.mu2[k]);
-            st.mu2[k] = st.mu1[k];
-            st.mu1[k] = nu;
-            sr = fma(nu, o ? in.por[k] : in.per[k], sr);
-            si = fma(nu, o ? in.poi[k] : in.pei[k], si);
-        }
-        rd.mine[2 * q] = sr;
-        rd.mine[2 * q + 1] = si;
-    }
-    rd.cnt = kRedChunk;
-    rd.flush();
-}
-
-// Two live steps; the first has parity `odd`.
-template <int K>
-__device__ __forceinline__ void adj_live2(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, bool odd, double c0,
-                                          double c1)
-{
-    if (odd) {
-        adj_live<K, true>(st, in, rd, c0);
-        adj_live<K, false>(st, in, rd, c1);
-    } else {
-        adj_live<K, false>(st, in, rd, c0);
-        adj_live<K, true>(st, in, rd, c1);
+    for (int k = 0; k < K; ++k) {
+        const double nu = fma(Al * st.x[k], st.mu1[k], -st.mu2[k]);
+        st.mu2[k] = st.mu1[k];
+        st.mu1[k] = nu;
+        sr = fma(nu, o ? in.por[k] : in.per[k], sr);
+        si = fma(nu, o ? in.poi[k] : in.pei[k], si);
     }
 }
 
 template <int K>
-__device__ __forceinline__ void adj_chunk(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, int& l, int cend, int cb,
-                                          int m, const double* cA)
+__device__ __forceinline__ void adj_step(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, int l, int m, double Al)
+{
+    if ((l - m) & 1)
+        adj_live<K, true>(st, in, rd, Al);
+    else
+        adj_live<K, false>(st, in, rd, Al);
+}
+
+// Steps l..cend of one staged chunk while some lane is still scaled: gated
+// steps (rescale check every two) until every lane is live, then the rest of
+// the chunk one live step at a time.
+template <int K>
+__device__ __forceinline__ void adj_chunk_scaled(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, int& l, int cend,
+                                                 int cb, int m, const double* cA)
 {
     while (!st.all && l <= cend) {
         for (int rep = 0; rep < 2 && l <= cend; ++rep, ++l) {
@@ -585,41 +670,57 @@ __device__ __forceinline__ void adj_chunk(LegState<K>& st, const AdjIn<K>& in, A
         }
         if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
     }
-    // live: align to the reduction chunk with single steps, then 16 steps per
-    // iteration as two halves of kRedChunk = 8 (static LDS row slots, one flush
-    // per half) with two coefficient register sets alternating; the
-    // speculative load may read past the chunk into the padding
-    while (rd.cnt != 0 && l <= cend) {
-        if ((l - m) & 1)
-            adj_live<K, true>(st, in, rd, cA[l - cb]);
-        else
-            adj_live<K, false>(st, in, rd, cA[l - cb]);
-        ++l;
-    }
+    for (; l <= cend; ++l) adj_step<K>(st, in, rd, l, m, cA[l - cb]);
+}
+
+// Steps l..cend of one staged chunk with every lane live: single steps up to
+// the next reduction row boundary, then rows of kRedChunk = 8 steps held in
+// registers and reduced by reduce_row, with two 4-coefficient register sets
+// alternating (the speculative load of the next set may read up to 4 entries
+// past the chunk, into the row padding).
+template <int K>
+__device__ __forceinline__ void adj_chunk_live(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, int& l, int cend,
+                                               int cb, int m, const double* cA)
+{
+    for (; rd.cnt != 0 && l <= cend; ++l) adj_step<K>(st, in, rd, l, m, cA[l - cb]);
     if (l + 7 <= cend) {
         const bool odd = ((l - m) & 1) != 0;
-        double x[8], y[8];
+        double x[4], y[4], z[4], t4[4];
+        bool pend = false;  // z holds a folded row not yet stored
 #pragma unroll
-        for (int q = 0; q < 8; ++q) x[q] = cA[l - cb + q];
-        for (; l + 15 <= cend; l += 16) {
+        for (int q = 0; q < 4; ++q) x[q] = cA[l - cb + q];
+        for (; l + 7 <= cend; l += 8) {
+            double v0[8], v1[8];
+            // each set's loads are issued after the first step that uses the
+            // other set, so the wait before that step covers only LDS traffic
+            // issued half an iteration earlier; the previous row's tile round
+            // trip rides along with the Y loads
+            adj_step_v<K>(st, in, odd, x[0], v0[0], v0[1]);
+            __builtin_amdgcn_sched_barrier(0);
+            if (pend) {
+                rd.tile_store(z);
+                rd.tile_load(t4);
+            }
 #pragma unroll
-            for (int q = 0; q < 8; ++q) y[q] = cA[l - cb + 8 + q];
-            adj_live8<K>(st, in, rd, odd, x);
+            for (int q = 0; q < 4; ++q) y[q] = cA[l - cb + 4 + q];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) x[q] = cA[l - cb + 16 + q];
-            adj_live8<K>(st, in, rd, odd, y);
+            for (int q = 1; q < 4; ++q) adj_step_v<K>(st, in, odd ^ (q & 1), x[q], v0[2 * q], v0[2 * q + 1]);
+            adj_step_v<K>(st, in, odd, y[0], v1[0], v1[1]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x[q] = cA[l - cb + 8 + q];
+#pragma unroll
+            for (int q = 1; q < 4; ++q) adj_step_v<K>(st, in, odd ^ (q & 1), y[q], v1[2 * q], v1[2 * q + 1]);
+            if (pend) rd.finish_row(t4);
+            rd.fold_row(v0, v1, z);
+            pend = true;
         }
-        if (l + 7 <= cend) {
-            adj_live8<K>(st, in, rd, odd, x);
-            l += 8;
-        }
+        rd.tile_store(z);
+        rd.tile_load(t4);
+        rd.finish_row(t4);
+        wave_sync();
     }
-    for (; l <= cend; ++l) {
-        if ((l - m) & 1)
-            adj_live<K, true>(st, in, rd, cA[l - cb]);
-        else
-            adj_live<K, false>(st, in, rd, cA[l - cb]);
-    }
+    for (; l <= cend; ++l) adj_step<K>(st, in, rd, l, m, cA[l - cb]);
 }
 
 template <int K>
@@ -676,16 +777,25 @@ __global__ __launch_bounds__(64) void k_adj(const WorkItem* __restrict__ items, 
     int cb = l0 + 1;
     double tA = (cb + lane <= L) ? A[cb + lane] : 0.0;
     sA[0][lane] = tA;
-    __syncthreads();
+    wave_sync();
     int buf = 0;
     int l = cb;
+    // two loops so that the scaled-path state is dead in the second
+    while (cb <= L && !st.all) {
+        const int nb = cb + kChunk;
+        if (nb + lane <= L) tA = A[nb + lane];
+        adj_chunk_scaled<K>(st, in, rd, l, min(L, nb - 1), cb, m, sA[buf]);
+        if (nb <= L) sA[buf ^ 1][lane] = tA;
+        wave_sync();
+        buf ^= 1;
+        cb = nb;
+    }
     while (cb <= L) {
         const int nb = cb + kChunk;
         if (nb + lane <= L) tA = A[nb + lane];
-        const int cend = min(L, nb - 1);
-        adj_chunk<K>(st, in, rd, l, cend, cb, m, sA[buf]);
+        adj_chunk_live<K>(st, in, rd, l, min(L, nb - 1), cb, m, sA[buf]);
         if (nb <= L) sA[buf ^ 1][lane] = tA;
-        __syncthreads();
+        wave_sync();
         buf ^= 1;
         cb = nb;
     }

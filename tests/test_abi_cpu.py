@@ -40,6 +40,35 @@ def test_library_exports_every_declared_symbol():
     assert sorted(cm.declared_symbols()) == declared
 
 
+_RUNTIMES = r"""
+import sys
+sys.path.insert(0, %r)
+import cosmopp_amd as cm
+cm.lib()
+try:
+    import torch
+except ImportError:
+    pass
+names = set()
+for line in open("/proc/self/maps"):
+    path = line.split()[-1]
+    for key in ("libamdhip64", "libhsa-runtime64"):
+        if key in path:
+            names.add((key, path))
+for key in ("libamdhip64", "libhsa-runtime64"):
+    print(key, len([p for k, p in names if k == key]))
+"""
+
+
+def test_one_hip_runtime_when_torch_comes_second():
+    """Loading the library and then torch maps one HIP and one HSA runtime
+    (two HSA runtimes in a process leave the second without a GPU)."""
+    out = subprocess.run(["python3", "-c", _RUNTIMES % ROOT], stdout=subprocess.PIPE, text=True,
+                         check=True).stdout.split()
+    counts = dict(zip(out[0::2], map(int, out[1::2])))
+    assert counts == {"libamdhip64": 1, "libhsa-runtime64": 1}, counts
+
+
 def test_argument_errors_are_status_codes():
     with pytest.raises(cm.CmblError) as e:
         cm.Map(3, 8)  # nside not a power of two

@@ -1062,6 +1062,124 @@ __device__ __forceinline__ void fold_store(double2* Z, int k, int n, int logM, b
     Z[pow2 ? brev_m(k, logM) : k] = make_double2(gN.x - gS.y, gN.y + gS.x);
 }
 
+// Y_k of a split ring from its two half transforms: SE (global scratch) and
+// Z (LDS), Y_k = E_{k mod h} + e^{-2 pi i k/n} O_{k mod h}.
+__device__ __forceinline__ double2 split_bin(const double2* SE, const double2* Z, int k, int n, int h)
+{
+    double2 w = cis_pi_frac(2 * (long)k % (2L * n), n);
+    w.y = -w.y;
+    const int kk = k % h;
+    return cadd(SE[kk], cmul(Z[kk], w));
+}
+
+// A ring pair whose length-n Bluestein transform would not fit in LDS (n = 4i
+// with 2n - 1 > kMaxFFT): each direction splits into two length-h = n/2 DFTs
+// over the even and odd samples, each done by Bluestein in LDS, the first kept
+// in global scratch while the second runs (decimation in time):
+//   synthesis  s_j = E_j + w^j O_j,  s_{j+h} = E_j - w^j O_j,  w = e^{2 pi i/n},
+//              E, O = inverse DFT_h of the even / odd bins;
+//   analysis   Y_k = FE_{k mod h} + w^{-k} FO_{k mod h},
+//              FE, FO = DFT_h of the even / odd samples.
+// Scratch per ring: SE[h] (a half transform) and SF[n] (weighted residual).
+template <int MODE>
+__device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, bool shifted, long oN, long oS,
+                           double* __restrict__ map_out, const double* __restrict__ map_in,
+                           const double2* __restrict__ tw, double2* Z, double* sh)
+{
+    const int L = md.L;
+    const int tid = threadIdx.x;
+    const int h = n >> 1;
+    const double2* filt = md.bs_filt + md.ring_bsofs[p];
+    const double2* chirp = md.bs_chirp + md.ring_chofs[p];
+    double2* SE = md.ring_scr + md.ring_scrofs[p];
+    double2* SF = SE + h;
+
+    if (MODE != kRingAna) {
+        const double2* row = reinterpret_cast<const double2*>(md.P + (long)p * (L + 1) * 4);
+        for (int r = 0; r < 2; ++r) {
+            for (int kk = tid; kk < h; kk += kRingThreads) {
+                const int k = 2 * kk + r;
+                double2 gN, gS;
+                fold_aliases(row, L, n, k, 0, 1, shifted, gN, gS);
+                fold_store(Z, kk, n, 0, false, false, oS < 0, gN, gS);
+                if (shifted) Z[kk] = cmul(Z[kk], cis_pi_frac(k, n));
+            }
+            __syncthreads();
+            bluestein(Z, h, M, logM, +1, chirp, filt, tw);
+            if (r == 0) {
+                for (int j = tid; j < h; j += kRingThreads) SE[j] = Z[j];
+                __syncthreads();
+            }
+        }
+        double chi = 0.0;
+        for (int j = tid; j < h; j += kRingThreads) {
+            const double2 e = SE[j];
+            const double2 o = cmul(Z[j], cis_pi_frac(2 * j, n));
+            const double2 sv[2] = {cadd(e, o), csub(e, o)};
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const long t = j + half * h;
+                const double2 v = sv[half];
+                if (MODE == kRingSyn) {
+                    map_out[oN + t] = v.x;
+                    if (oS >= 0) map_out[oS + t] = v.y;
+                } else {
+                    const double rN = md.d[oN + t] - v.x;
+                    const double fN = md.w[oN + t] * rN;
+                    chi = fma(fN, rN, chi);
+                    double fS = 0.0;
+                    if (oS >= 0) {
+                        const double rS = md.d[oS + t] - v.y;
+                        fS = md.w[oS + t] * rS;
+                        chi = fma(fS, rS, chi);
+                    }
+                    SF[t] = make_double2(fN, fS);
+                }
+            }
+        }
+        if (MODE == kRingSyn) return;
+        const double t = block_sum<kRingThreads>(chi, sh);
+        if (tid == 0) md.chi[p] = t;
+    }
+
+    for (int r = 0; r < 2; ++r) {
+        for (int j = tid; j < h; j += kRingThreads) {
+            const long t = 2 * j + r;
+            Z[j] = (MODE == kRingAna) ? make_double2(map_in[oN + t], oS >= 0 ? map_in[oS + t] : 0.0) : SF[t];
+        }
+        __syncthreads();
+        bluestein(Z, h, M, logM, -1, chirp, filt, tw);
+        if (r == 0) {
+            for (int j = tid; j < h; j += kRingThreads) SE[j] = Z[j];
+            __syncthreads();
+        }
+    }
+
+    // unfold as in the unsplit path (n > kRingThreads, so one thread group)
+    double2* qrow = reinterpret_cast<double2*>(md.Q + (long)p * (L + 1) * 4);
+    for (int k = tid; k < n && k <= L; k += kRingThreads) {
+        const int nk = (k == 0) ? 0 : n - k;
+        const double2 y = split_bin(SE, Z, k, n, h);
+        const double2 z2 = split_bin(SE, Z, nk, n, h);
+        const double2 yc = make_double2(z2.x, -z2.y);
+        double2 hN = make_double2(0.5 * (y.x + yc.x), 0.5 * (y.y + yc.y));
+        double2 hS = make_double2(0.5 * (y.y - yc.y), -0.5 * (y.x - yc.x));
+        if (shifted) {
+            double2 ph = cis_pi_frac(k, n);
+            ph.y = -ph.y;
+            hN = cmul(hN, ph);
+            hS = cmul(hS, ph);
+        }
+        const double2 pe = cadd(hN, hS), po = csub(hN, hS);
+        for (int j = 0; k + (long)j * n <= L; ++j) {
+            const int mm = k + j * n;
+            const double sg = (shifted && (j & 1)) ? -1.0 : 1.0;
+            qrow[2 * mm] = make_double2(sg * pe.x, sg * pe.y);
+            qrow[2 * mm + 1] = make_double2(sg * po.x, sg * po.y);
+        }
+    }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restrict__ items,
                                                        const MapDesc* __restrict__ maps,
@@ -1083,6 +1201,10 @@ __global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restric
     const bool shifted = md.ring_shift[p] != 0;
     const long oN = md.ring_ofsN[p];
     const long oS = md.ring_ofsS[p];
+    if (md.ring_scrofs[p] >= 0) {
+        ring_split<MODE>(md, p, n, M, logM, shifted, oN, oS, map_out, map_in, tw, Z, sh);
+        return;
+    }
     const double2* filt = pow2 ? nullptr : md.bs_filt + md.ring_bsofs[p];
     const double2* chirp = pow2 ? nullptr : md.bs_chirp + md.ring_chofs[p];
     const int tid = threadIdx.x;
@@ -1191,9 +1313,10 @@ __global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restric
     }
 }
 
-// Bluestein set-up for one ring pair: chirp[j] = e^{-i pi j^2/n} (j < n) and
-// filt = FFT_M of b_j = e^{+i pi j^2/n} for |j| < n wrapped mod M, stored in
-// the bit-reversed order of the DIF transform.
+// Bluestein set-up for one ring pair of Bluestein length n (nph, or nph/2 for
+// a split ring): chirp[j] = e^{-i pi j^2/n} (j < n) and filt = FFT_M of
+// b_j = e^{+i pi j^2/n} for |j| < n wrapped mod M, stored in the bit-reversed
+// order of the DIF transform.
 __global__ __launch_bounds__(kRingThreads) void k_bluestein_setup(const int* __restrict__ pairs,
                                                                   const MapDesc* __restrict__ md_,
                                                                   double2* __restrict__ filt_out,
@@ -1203,7 +1326,7 @@ __global__ __launch_bounds__(kRingThreads) void k_bluestein_setup(const int* __r
     extern __shared__ double2 Z[];
     const MapDesc& md = *md_;
     const int p = pairs[blockIdx.x];
-    const int n = md.ring_nph[p];
+    const int n = md.ring_scrofs[p] >= 0 ? md.ring_nph[p] / 2 : md.ring_nph[p];  // Bluestein length
     const int M = md.ring_M[p];
     int logM = 0;
     while ((1 << logM) < M) ++logM;

@@ -278,9 +278,9 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     // ---- ring-pair tables
     std::vector<double> rx(npad, 0.0), rs(npad, 0.0);
     std::vector<int> nph(npad, 0), shift(npad, 0), Mfft(npad, 0);
-    std::vector<long> ofsN(npad, 0), ofsS(npad, -1), bsofs(npad, -1), chofs(npad, -1);
+    std::vector<long> ofsN(npad, 0), ofsS(npad, -1), bsofs(npad, -1), chofs(npad, -1), scrofs(npad, -1);
     std::vector<int> bs_pairs;
-    long bs_total = 0, ch_total = 0;
+    long bs_total = 0, ch_total = 0, scr_total = 0;
     int maxM = 1;
     for (int p = 0; p < mp->npair; ++p) {
         const RingGeom g = ring_geom(N, p + 1);
@@ -293,20 +293,28 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
         if (is_pow2(g.nph)) {
             Mfft[p] = g.nph;
         } else {
-            int M = 1;
-            while (M < 2 * g.nph - 1) M <<= 1;
+            // Bluestein length b (nph, or nph/2 for a split ring; nph = 4i is even)
+            int b = g.nph, M = 1;
+            while (M < 2 * b - 1) M <<= 1;
+            if (M > kMaxFFT) {
+                b = g.nph / 2;
+                M = 1;
+                while (M < 2 * b - 1) M <<= 1;
+                scrofs[p] = scr_total;
+                scr_total += 3L * b;
+            }
             Mfft[p] = M;
             bsofs[p] = bs_total;
             bs_total += M;
             chofs[p] = ch_total;
-            ch_total += g.nph;
+            ch_total += b;
             bs_pairs.push_back(p);
         }
         maxM = std::max(maxM, Mfft[p]);
     }
     if (maxM > kMaxFFT)
         return fail(CMBL_ERR_ARG, "ring transform length " + std::to_string(maxM) + " exceeds the LDS limit " +
-                                      std::to_string(kMaxFFT) + " (nside > 1024 is not supported yet)");
+                                      std::to_string(kMaxFFT) + " (nside > 2048 is not supported)");
     mp->lds_ring = maxM * (int)sizeof(double2);
 
     // ---- recurrence tables (long double on the host, rounded once)
@@ -390,6 +398,10 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     long* p_co;
     CMBL_TRY(dupload(mp, &p_co, chofs));
     d.ring_chofs = p_co;
+    long* p_so;
+    CMBL_TRY(dupload(mp, &p_so, scrofs));
+    d.ring_scrofs = p_so;
+    CMBL_TRY(dalloc(mp, &d.ring_scr, (size_t)std::max<long>(scr_total, 1)));
     CMBL_TRY(dupload(mp, &p_A, A));
     CMBL_TRY(dupload(mp, &p_N, Nrm));
     CMBL_TRY(dupload(mp, &p_sv, seed_v));

@@ -33,14 +33,31 @@ CONFIGS = {
     1: dict(nside=64, lmax=128, fwhm_deg=2.0, sigma=10.0, cut_deg=20.0),
     2: dict(nside=512, lmax=1024, fwhm_deg=20.0 / 60.0, sigma=30.0, cut_deg=20.0),
     4: dict(nside=1024, lmax=2048, fwhm_deg=10.0 / 60.0, sigma=20.0, cut_deg=20.0),
+    5: dict(nside=2048, lmax=4096, fwhm_deg=5.0 / 60.0, sigma=15.0, cut_deg=20.0),
 }
+
+
+def batch_configs(nmaps=64, seed=3):
+    """64 maps of uneven size: lmax ~ U{256, 512, 768, 1024, 1536}, nside =
+    2^round(log2(lmax/2)), FWHM = 5400/lmax arcmin, sigma ~ U[10, 50] uK,
+    cut ~ U[10, 30] deg, independent seeds."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nmaps):
+        lmax = int(rng.choice([256, 512, 768, 1024, 1536]))
+        nside = 1 << int(round(math.log2(lmax / 2)))
+        out.append(dict(nside=nside, lmax=lmax, fwhm_deg=5400.0 / lmax / 60.0, sigma=float(rng.uniform(10, 50)),
+                        cut_deg=float(rng.uniform(10, 30)), seed=1000 + 10 * i))
+    return out
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_case(cfg):
+def build_case(cfg, seed=101):
     """Inputs of the workload; the data map is synthesised on the GPU."""
     import numpy as np
 
@@ -49,17 +66,84 @@ def build_case(cfg):
     nside, lmax = cfg["nside"], cfg["lmax"]
     cl = cm.cl_shape(lmax)
     b = cm.beam(lmax, cfg["fwhm_deg"])
-    a_true = cm.simulate_alm(cl, lmax, 101)
+    a_true = cm.simulate_alm(cl, lmax, seed)
     mp = cm.Map(nside, lmax)
     mp.set_beam(b)
     mp.set_cl(cl)
     s = mp.forward(a_true)
-    d = s + cm.simulate_white_noise(nside, cfg["sigma"], 102)
+    d = s + cm.simulate_white_noise(nside, cfg["sigma"], seed + 1)
     mask = cm.mask_latitude(nside, math.radians(cfg["cut_deg"]))
     w = cm.weights_from_mask(mask, cfg["sigma"])
     mp.set_data(d, w)
-    x = a_true + 0.1 * cm.simulate_alm(cl, lmax, 103)
+    x = a_true + 0.1 * cm.simulate_alm(cl, lmax, seed + 2)
     return mp, dict(cl=cl, beam=b, d=d, w=w, x=x)
+
+
+def bench_batch(args):
+    """64 uneven maps: one batched evaluation (one launch per stage over all
+    maps) against the same maps evaluated one after another."""
+    import torch
+
+    import cosmopp_amd as cm
+
+    cfgs = batch_configs()
+    dev = torch.device("cuda:0")
+    maps, xs, gs = [], [], []
+    log("bench: building %d maps" % len(cfgs))
+    for c in cfgs:
+        mp, inp = build_case(c, c["seed"])
+        maps.append(mp)
+        xs.append(torch.from_numpy(inp["x"]).to(dev))
+        gs.append(torch.empty_like(xs[-1]))
+    f = torch.empty(len(maps), dtype=torch.float64, device=dev)
+    batch = cm.Batch(maps)
+    sp = torch.cuda.current_stream().cuda_stream
+    xp, gp = [x.data_ptr() for x in xs], [g.data_ptr() for g in gs]
+
+    def batched():
+        batch.value_and_grad_device(xp, f.data_ptr(), gp, sp)
+
+    def single():
+        for i, mp in enumerate(maps):
+            mp.value_and_grad_device(xp[i], f.data_ptr() + 8 * i, gp[i], sp)
+
+    rates = {}
+    for name, fn in (("batched", batched), ("per_map", single)):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        rates[name] = len(maps) * args.steps / (time.perf_counter() - t0)
+        log("bench: %s %.2f map-evals/s" % (name, rates[name]))
+    flop = sum(16.0 * 2 * c["nside"] * (c["lmax"] + 1) * (c["lmax"] + 2) / 2 for c in cfgs)
+    out = {
+        "metric": "map-level -2lnL + gradient evaluations per second over a batch of 64 uneven maps, FP64",
+        "value": round(rates["batched"], 3),
+        "unit": "map-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(len(maps) / rates["batched"] * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "64 maps, lmax ~ U{256,512,768,1024,1536}, nside = 2^round(log2(lmax/2)), "
+                               "FWHM = 5400/lmax arcmin, sigma ~ U[10,50] uK, cut ~ U[10,30] deg",
+                   "maps": [[c["nside"], c["lmax"]] for c in cfgs], "parallelism": "single GPU"},
+        "per_map_value": round(rates["per_map"], 3),
+        "batched_over_per_map": round(rates["batched"] / rates["per_map"], 3),
+        "legendre_flop_per_batch_eval": flop,
+        "legendre_tflops_batched": round(flop * rates["batched"] / len(maps) / 1e12, 3),
+    }
+    batch.close()
+    for mp in maps:
+        mp.close()
+    print(json.dumps(out), flush=True)
 
 
 def read_traffic(nside, lmax, kernel):
@@ -106,7 +190,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=4, choices=sorted(CONFIGS))
+    ap.add_argument("--config", type=int, default=4, choices=sorted(CONFIGS) + [3],
+                    help="workload: 4 (default, the headline), 1, 2, 5 (main map), 3 (64-map batch)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=40.0, help="seconds for the CPU baseline leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -137,6 +222,9 @@ def main():
 
     import cosmopp_amd as cm
 
+    if args.config == 3:
+        bench_batch(args)
+        return
     cfg = CONFIGS[args.config]
     nside, lmax = cfg["nside"], cfg["lmax"]
     log("bench: building inputs nside=%d lmax=%d" % (nside, lmax))

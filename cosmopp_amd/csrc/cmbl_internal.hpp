@@ -134,6 +134,7 @@ int hip_fail(hipError_t e, const char* what);
 int launch_pre(const MapDesc& md, const MapDesc* d_md, const double* d_x, hipStream_t s);
 int launch_synth(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, hipStream_t s);
 int launch_adj(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, hipStream_t s);
+int lm_tiles(int L);  // blocks of k_pre / k_post (prior partial sums)
 int launch_ring(int mode, const RingItem* items, int nitems, const MapDesc* d_maps, int lds_bytes,
                 double* d_map_out, const double* d_map_in, const double2* d_tw, hipStream_t s);
 int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, double* d_g, int like,

@@ -77,27 +77,106 @@ __device__ __forceinline__ double block_sum(double v, double* sh)
     return t;
 }
 
-// ================================================================== k_pre
-// grid: (ceil((L+1)/256), L+1), block 256.  m = blockIdx.y, l = m + j.
+// ================================================================== k_pre / k_post
+// The packed parameter vector (almToVector: l-major, index l^2 + 2m - 1 + c)
+// and the m-major device arrays (alm, Nrm, upart: index moff(L, m) + l - m)
+// are transposes of each other over (l, m).  Both kernels work on kLM x kLM
+// tiles of the triangle m <= l <= L, one tile per block, and transpose through
+// LDS so that each side is read or written in its own contiguous order.
+constexpr int kLM = 32;
+
+__device__ __forceinline__ void tile_of(int b, int& tl, int& tm)
+{
+    int t = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while (t * (t + 1) / 2 > b) --t;
+    while ((t + 1) * (t + 2) / 2 <= b) ++t;
+    tl = t;
+    tm = b - t * (t + 1) / 2;
+}
+
+// Packed index of (l, m, c) (c = 0 real, 1 imaginary), -1 for the absent Im a_l0.
+__device__ __forceinline__ long packed_index(int l, int m, int c)
+{
+    const long li = (long)l * l;
+    if (m == 0) return c == 0 ? li : -1;
+    return li + 2 * m - 1 + c;
+}
+
+// alm = B_l N_lm a_lm (m-major) from the packed x, and the prior partial sum
+// of the tile: sum invcl_l (|a_l0|^2 + 2 sum_m>0 |a_lm|^2).
 __global__ __launch_bounds__(256) void k_pre(const MapDesc* __restrict__ md_, const double* __restrict__ x)
 {
+    __shared__ double T[kLM][2 * kLM + 1];
     __shared__ double sh[4];
     const MapDesc& md = *md_;
     const int L = md.L;
-    const int m = blockIdx.y;
-    const int l = m + blockIdx.x * 256 + threadIdx.x;
+    int tl, tm;
+    tile_of(blockIdx.x, tl, tm);
+    const int l0 = tl * kLM, m0 = tm * kLM;
+    for (int i = threadIdx.x; i < kLM * 2 * kLM; i += 256) {
+        const int lr = i / (2 * kLM), c = i % (2 * kLM);
+        const int l = l0 + lr, m = m0 + (c >> 1);
+        const long pi = (l <= L && m <= l) ? packed_index(l, m, c & 1) : -1;
+        T[lr][c] = pi >= 0 ? x[pi] : 0.0;
+    }
+    __syncthreads();
     double pr = 0.0;
-    if (l <= L) {
-        const long li = (long)l * l;
-        const double re = (m == 0) ? x[li] : x[li + 2 * m - 1];
-        const double im = (m == 0) ? 0.0 : x[li + 2 * m];
-        const long k = moff(L, m) + (l - m);
-        const double s = md.beam[l] * md.Nrm[k];
-        md.alm[k] = make_double2(s * re, s * im);
-        pr = md.invcl[l] * ((m == 0) ? re * re : 2.0 * (re * re + im * im));
+    for (int i = threadIdx.x; i < kLM * kLM; i += 256) {
+        const int mr = i / kLM, lr = i % kLM;
+        const int l = l0 + lr, m = m0 + mr;
+        if (l <= L && m <= l) {
+            const double re = T[lr][2 * mr], im = T[lr][2 * mr + 1];
+            const long k = moff(L, m) + (l - m);
+            const double sc = md.beam[l] * md.Nrm[k];
+            md.alm[k] = make_double2(sc * re, sc * im);
+            pr += md.invcl[l] * ((m == 0) ? re * re : 2.0 * (re * re + im * im));
+        }
     }
     const double t = block_sum<256>(pr, sh);
-    if (threadIdx.x == 0) md.prior[blockIdx.y * gridDim.x + blockIdx.x] = t;
+    if (threadIdx.x == 0) md.prior[blockIdx.x] = t;
+}
+
+// u = N_lm B_l sum_rb upart (fixed order) -> packed gradient.
+// like = 1: g = -2 J^T (w r) + prior gradient;  like = 0: g = J^T v.
+__global__ __launch_bounds__(256) void k_post(const MapDesc* __restrict__ md_, const double* __restrict__ x,
+                                              double* __restrict__ g, int like)
+{
+    __shared__ double T[kLM][2 * kLM + 1];
+    const MapDesc& md = *md_;
+    const int L = md.L;
+    int tl, tm;
+    tile_of(blockIdx.x, tl, tm);
+    const int l0 = tl * kLM, m0 = tm * kLM;
+    for (int i = threadIdx.x; i < kLM * kLM; i += 256) {
+        const int mr = i / kLM, lr = i % kLM;
+        const int l = l0 + lr, m = m0 + mr;
+        double ur = 0.0, ui = 0.0;
+        if (l <= L && m <= l) {
+            const long k = moff(L, m) + (l - m);
+            for (int rb = 0; rb < md.nrb5; ++rb) {
+                const double2 v = reinterpret_cast<const double2*>(md.upart + 2 * (long)rb * md.nalm)[k];
+                ur += v.x;
+                ui += v.y;
+            }
+            const double sc = md.Nrm[k] * md.beam[l];
+            ur *= sc;
+            ui *= sc;
+        }
+        T[lr][2 * mr] = ur;
+        T[lr][2 * mr + 1] = ui;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kLM * 2 * kLM; i += 256) {
+        const int lr = i / (2 * kLM), c = i % (2 * kLM);
+        const int l = l0 + lr, m = m0 + (c >> 1);
+        const long pi = (l <= L && m <= l) ? packed_index(l, m, c & 1) : -1;
+        if (pi < 0) continue;
+        const double u = T[lr][c];
+        if (m == 0)
+            g[pi] = like ? -2.0 * u + 2.0 * md.invcl[l] * x[pi] : u;
+        else
+            g[pi] = like ? -4.0 * u + 4.0 * md.invcl[l] * x[pi] : 2.0 * u;
+    }
 }
 
 // ================================================================== Legendre stages
@@ -1350,44 +1429,6 @@ __global__ __launch_bounds__(kRingThreads) void k_bluestein_setup(const int* __r
     for (int j = threadIdx.x; j < M; j += blockDim.x) dst[j] = Z[j];
 }
 
-// ================================================================== k_post
-// grid (ceil((L+1)/256), L+1).  u = sum_rb upart (fixed order), lambda = N mu.
-// like = 1: g = -2 J^T (w r) + prior gradient;  like = 0: g = J^T v.
-__global__ __launch_bounds__(256) void k_post(const MapDesc* __restrict__ md_, const double* __restrict__ x,
-                                              double* __restrict__ g, int like)
-{
-    const MapDesc& md = *md_;
-    const int L = md.L;
-    const int m = blockIdx.y;
-    const int l = m + blockIdx.x * 256 + threadIdx.x;
-    if (l > L) return;
-    const long k = moff(L, m) + (l - m);
-    double ur = 0.0, ui = 0.0;
-    for (int rb = 0; rb < md.nrb5; ++rb) {
-        const double2 v = reinterpret_cast<const double2*>(md.upart + 2 * (long)rb * md.nalm)[k];
-        ur += v.x;
-        ui += v.y;
-    }
-    const double s = md.Nrm[k] * md.beam[l];
-    ur *= s;
-    ui *= s;
-    const long li = (long)l * l;
-    if (m == 0) {
-        if (like)
-            g[li] = -2.0 * ur + 2.0 * md.invcl[l] * x[li];
-        else
-            g[li] = ur;
-    } else {
-        if (like) {
-            g[li + 2 * m - 1] = -4.0 * ur + 4.0 * md.invcl[l] * x[li + 2 * m - 1];
-            g[li + 2 * m] = -4.0 * ui + 4.0 * md.invcl[l] * x[li + 2 * m];
-        } else {
-            g[li + 2 * m - 1] = 2.0 * ur;
-            g[li + 2 * m] = 2.0 * ui;
-        }
-    }
-}
-
 // ================================================================== k_final
 // One block: -2lnL = sum_pairs chi + sum_blocks prior, fixed order.
 __global__ __launch_bounds__(1024) void k_final(const MapDesc* __restrict__ md_, double* __restrict__ f)
@@ -1439,10 +1480,15 @@ __global__ __launch_bounds__(1024) void k_dot_final(const double* __restrict__ p
 
 // ================================================================== launchers
 
+int lm_tiles(int L)
+{
+    const int nt = (L + kLM) / kLM;
+    return nt * (nt + 1) / 2;
+}
+
 int launch_pre(const MapDesc& md, const MapDesc* d_md, const double* d_x, hipStream_t s)
 {
-    dim3 grid((md.L + 1 + 255) / 256, md.L + 1);
-    hipLaunchKernelGGL(k_pre, grid, dim3(256), 0, s, d_md, d_x);
+    hipLaunchKernelGGL(k_pre, dim3(lm_tiles(md.L)), dim3(256), 0, s, d_md, d_x);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }
@@ -1582,8 +1628,7 @@ int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_b
 
 int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, double* d_g, int like, hipStream_t s)
 {
-    dim3 grid((md.L + 1 + 255) / 256, md.L + 1);
-    hipLaunchKernelGGL(k_post, grid, dim3(256), 0, s, d_md, d_x, d_g, like);
+    hipLaunchKernelGGL(k_post, dim3(lm_tiles(md.L)), dim3(256), 0, s, d_md, d_x, d_g, like);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }

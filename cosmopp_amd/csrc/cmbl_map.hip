@@ -443,7 +443,7 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     CMBL_TRY(dalloc(mp, &d.upart, (size_t)mp->nrb5 * mp->nalm * 2));
     CMBL_HIP(hipMemset(d.upart, 0, sizeof(double) * (size_t)mp->nrb5 * mp->nalm * 2));
     CMBL_TRY(dalloc(mp, &d.chi, mp->npair));
-    d.nprior = ((L + 1 + 255) / 256) * (L + 1);
+    d.nprior = lm_tiles(L);
     CMBL_TRY(dalloc(mp, &d.prior, d.nprior));
     CMBL_TRY(dalloc(mp, &mp->d_f, 1));
     CMBL_TRY(dalloc(mp, &mp->d_x, mp->nparam));

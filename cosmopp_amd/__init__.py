@@ -27,6 +27,7 @@ CMBL_ERR_HIP = 2
 CMBL_ERR_ALLOC = 3
 CMBL_ERR_STATE = 4
 CMBL_ERR_NONFINITE = 5
+CMBL_ERR_NOCONV = 6
 
 
 class CmblError(RuntimeError):
@@ -66,6 +67,18 @@ _SIGS = {
     "cmbl_batch_create": ([ctypes.POINTER(_vp), _vp, ctypes.c_int], ctypes.c_int),
     "cmbl_batch_destroy": ([_vp], None),
     "cmbl_batch_value_and_grad_device": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_gibbs_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.c_double],
+                          ctypes.c_int),
+    "cmbl_gibbs_destroy": ([_vp], None),
+    "cmbl_gibbs_apply": ([_vp, _vp, _vp], ctypes.c_int),
+    "cmbl_gibbs_apply_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_gibbs_precondition": ([_vp, _vp, _vp], ctypes.c_int),
+    "cmbl_gibbs_precondition_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_gibbs_rhs": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_gibbs_rhs_device": ([_vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_gibbs_solve": ([_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "cmbl_gibbs_solve_device": ([_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp],
+                                ctypes.c_int),
     "cmbl_simulate_alm": ([_vp, ctypes.c_int, ctypes.c_long, _vp], ctypes.c_int),
     "cmbl_simulate_white_noise": ([ctypes.c_int, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
     "cmbl_beam": ([ctypes.c_int, ctypes.c_double, _vp], ctypes.c_int),
@@ -312,6 +325,64 @@ class Batch:
         gs = (_vp * n)(*g_ptrs)
         _check(lib().cmbl_batch_value_and_grad_device(self._h, ctypes.cast(xs, _vp), f_ptr,
                                                       ctypes.cast(gs, _vp), stream))
+
+
+class Gibbs:
+    """The Gibbs constrained-realisation system A x = b of
+    CMBGibbsSampler::generateSignal (source/cmb_gibbs.cpp:437-530), with the
+    CGTreats interface of Math::ConjugateGradient
+    (include/conjugate_gradient.hpp:13-19): multiply_by_matrix / preconditioner,
+    plus the right-hand side and the device CG solve."""
+
+    def __init__(self, nside, lmax, cl, beam, mask, pixel_noise):
+        self.nside, self.lmax = nside, lmax
+        self.n = (lmax + 1) ** 2
+        self.npix = 12 * nside * nside
+        h = _vp()
+        _check(lib().cmbl_gibbs_create(ctypes.byref(h), nside, lmax, _p(_arr(cl, lmax + 1)),
+                                       _p(_arr(beam, lmax + 1)), _p(_arr(mask, self.npix)), float(pixel_noise)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().cmbl_gibbs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def multiply_by_matrix(self, x):
+        x = _arr(x, self.n)
+        y = np.zeros(self.n)
+        _check(lib().cmbl_gibbs_apply(self._h, _p(x), _p(y)))
+        return y
+
+    def preconditioner(self, x):
+        x = _arr(x, self.n)
+        y = np.zeros(self.n)
+        _check(lib().cmbl_gibbs_precondition(self._h, _p(x), _p(y)))
+        return y
+
+    def rhs(self, d, omega0, omega1):
+        b = np.zeros(self.n)
+        _check(lib().cmbl_gibbs_rhs(self._h, _p(_arr(d, self.npix)), _p(_arr(omega0, self.npix)),
+                                    _p(_arr(omega1, self.npix)), _p(b)))
+        return b
+
+    def solve(self, b, precision=1e-7, max_iter=10000):
+        """(x, iterations); raises CmblError(CMBL_ERR_NOCONV) past max_iter."""
+        b = _arr(b, self.n)
+        x = np.zeros(self.n)
+        it = ctypes.c_int(0)
+        _check(lib().cmbl_gibbs_solve(self._h, _p(b), _p(x), float(precision), int(max_iter), ctypes.byref(it)))
+        return x, it.value
 
 
 class MapLikelihood:

@@ -116,7 +116,17 @@ struct RingItem {
     int pair;
 };
 
-enum RingMode { kRingLike = 0, kRingSyn = 1, kRingAna = 2 };
+// kRingLike: s -> f = w (d - s) and chi partials -> analysis;  kRingSyn: s -> map;
+// kRingAna: map -> analysis;  kRingWeight: s -> f = w s -> analysis (Gibbs operator).
+enum RingMode { kRingLike = 0, kRingSyn = 1, kRingAna = 2, kRingWeight = 3 };
+
+// k_post output modes (u = B_l N_lm sum of the ring-block partials, complex):
+//   kPostGrad    g = J^T v  (real-parameter derivative: 2u for m >= 1)
+//   kPostLike    g = -2 J^T (w r) + prior gradient
+//   kPostGibbs   y = x + u                 (I + D Y^T N^-1 Y D, HEALPix complex convention)
+//   kPostAlm     y = u                     (packed D Y^T v, complex convention)
+//   kPostAddAlm  y += c u / B_l            (packed c Y^T v without the l-scaling)
+enum PostMode { kPostGrad = 0, kPostLike = 1, kPostGibbs = 2, kPostAlm = 3, kPostAddAlm = 4 };
 
 // Thread-local error string.
 void set_error(const std::string& msg);
@@ -135,10 +145,11 @@ int launch_pre(const MapDesc& md, const MapDesc* d_md, const double* d_x, hipStr
 int launch_synth(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, hipStream_t s);
 int launch_adj(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, hipStream_t s);
 int lm_tiles(int L);  // blocks of k_pre / k_post (prior partial sums)
+int launch_ldiv(double* y, const double* x, const double* dl, int L, hipStream_t s);
 int launch_ring(int mode, const RingItem* items, int nitems, const MapDesc* d_maps, int lds_bytes,
                 double* d_map_out, const double* d_map_in, const double2* d_tw, hipStream_t s);
-int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, double* d_g, int like,
-                hipStream_t s);
+int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, double* d_g, int mode,
+                hipStream_t s, double c = 0.0);
 int launch_final(const MapDesc* d_md, double* d_f, hipStream_t s);
 int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_bs, double2* d_out,
                            double2* d_chirp, int lds_bytes, const double2* d_tw, hipStream_t s);
@@ -190,6 +201,20 @@ struct cmbl_map {
     double stage_ms[CMBL_NSTAGES] = {0, 0, 0, 0, 0, 0};
     long timed_evals = 0;
     static constexpr int kEvSets = 64;
+};
+
+// Gibbs constrained-realisation system: its own map whose "beam" is
+// D_l = B_l sqrt(C_l) and whose weights are mask/sigma^2.
+struct cmbl_gibbs {
+    cmbl_map* mp = nullptr;
+    int L = 0;
+    long n = 0, npix = 0, good = 0;
+    double sigma = 0.0;
+    double* d_dl = nullptr;  // [L+1] preconditioner divisors 1 + C_l B_l^2 / N_l
+    double *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_ap = nullptr;  // [n] CG vectors
+    double *d_b = nullptr, *d_x = nullptr;                                   // [n] host-call staging
+    double *d_m0 = nullptr, *d_m1 = nullptr, *d_m2 = nullptr;               // [npix] maps
+    std::vector<void*> allocs;
 };
 
 struct cmbl_batch {

@@ -136,10 +136,12 @@ __global__ __launch_bounds__(256) void k_pre(const MapDesc* __restrict__ md_, co
     if (threadIdx.x == 0) md.prior[blockIdx.x] = t;
 }
 
-// u = N_lm B_l sum_rb upart (fixed order) -> packed gradient.
-// like = 1: g = -2 J^T (w r) + prior gradient;  like = 0: g = J^T v.
+// u = N_lm B_l sum_rb upart (fixed order) -> the packed output of `mode`
+// (PostMode): the gradient forms kPostGrad / kPostLike, or the Gibbs forms in
+// the HEALPix complex-a_lm convention (no factor 2 for m >= 1).  kPostAddAlm
+// scales by c instead of B_l and adds to g.
 __global__ __launch_bounds__(256) void k_post(const MapDesc* __restrict__ md_, const double* __restrict__ x,
-                                              double* __restrict__ g, int like)
+                                              double* __restrict__ g, int mode, double c)
 {
     __shared__ double T[kLM][2 * kLM + 1];
     const MapDesc& md = *md_;
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(256) void k_post(const MapDesc* __restrict__ md_, c
                 ur += v.x;
                 ui += v.y;
             }
-            const double sc = md.Nrm[k] * md.beam[l];
+            const double sc = md.Nrm[k] * (mode == kPostAddAlm ? c : md.beam[l]);
             ur *= sc;
             ui *= sc;
         }
@@ -167,15 +169,34 @@ __global__ __launch_bounds__(256) void k_post(const MapDesc* __restrict__ md_, c
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kLM * 2 * kLM; i += 256) {
-        const int lr = i / (2 * kLM), c = i % (2 * kLM);
-        const int l = l0 + lr, m = m0 + (c >> 1);
-        const long pi = (l <= L && m <= l) ? packed_index(l, m, c & 1) : -1;
+        const int lr = i / (2 * kLM), c2 = i % (2 * kLM);
+        const int l = l0 + lr, m = m0 + (c2 >> 1);
+        const long pi = (l <= L && m <= l) ? packed_index(l, m, c2 & 1) : -1;
         if (pi < 0) continue;
-        const double u = T[lr][c];
-        if (m == 0)
-            g[pi] = like ? -2.0 * u + 2.0 * md.invcl[l] * x[pi] : u;
-        else
-            g[pi] = like ? -4.0 * u + 4.0 * md.invcl[l] * x[pi] : 2.0 * u;
+        const double u = T[lr][c2];
+        switch (mode) {
+        case kPostGrad: g[pi] = (m == 0) ? u : 2.0 * u; break;
+        case kPostLike:
+            g[pi] = (m == 0) ? -2.0 * u + 2.0 * md.invcl[l] * x[pi] : -4.0 * u + 4.0 * md.invcl[l] * x[pi];
+            break;
+        case kPostGibbs: g[pi] = x[pi] + u; break;
+        case kPostAlm: g[pi] = u; break;
+        default: g[pi] += u; break;  // kPostAddAlm
+        }
+    }
+}
+
+// y_p = x_p / d_l over the packed layout (l = floor(sqrt(p))): the Gibbs
+// preconditioner (source/cmb_gibbs.cpp:407-423 divides every a_lm by d_l).
+__global__ __launch_bounds__(256) void k_ldiv(double* __restrict__ y, const double* __restrict__ x,
+                                              const double* __restrict__ dl, int L)
+{
+    const long n = (long)(L + 1) * (L + 1);
+    for (long p = blockIdx.x * 256L + threadIdx.x; p < n; p += (long)gridDim.x * 256L) {
+        long l = (long)sqrt((double)p);
+        while (l * l > p) --l;
+        while ((l + 1) * (l + 1) <= p) ++l;
+        y[p] = x[p] / dl[l];
     }
 }
 
@@ -1202,6 +1223,8 @@ __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, boo
                 if (MODE == kRingSyn) {
                     map_out[oN + t] = v.x;
                     if (oS >= 0) map_out[oS + t] = v.y;
+                } else if (MODE == kRingWeight) {
+                    SF[t] = make_double2(md.w[oN + t] * v.x, oS >= 0 ? md.w[oS + t] * v.y : 0.0);
                 } else {
                     const double rN = md.d[oN + t] - v.x;
                     const double fN = md.w[oN + t] * rN;
@@ -1217,8 +1240,12 @@ __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, boo
             }
         }
         if (MODE == kRingSyn) return;
-        const double t = block_sum<kRingThreads>(chi, sh);
-        if (tid == 0) md.chi[p] = t;
+        if (MODE == kRingLike) {
+            const double t = block_sum<kRingThreads>(chi, sh);
+            if (tid == 0) md.chi[p] = t;
+        } else {
+            __syncthreads();
+        }
     }
 
     for (int r = 0; r < 2; ++r) {
@@ -1353,6 +1380,12 @@ __global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restric
         }
         const double t = block_sum<kRingThreads>(chi, sh);
         if (tid == 0) md.chi[p] = t;
+    } else if (MODE == kRingWeight) {
+        for (int j = tid; j < n; j += kRingThreads) {
+            const double2 v = Z[j];
+            Z[j] = make_double2(md.w[oN + j] * v.x, oS >= 0 ? md.w[oS + j] * v.y : 0.0);
+        }
+        __syncthreads();
     } else {  // kRingAna: load the map to analyse
         for (int j = tid; j < n; j += kRingThreads)
             Z[j] = make_double2(map_in[oN + j], oS >= 0 ? map_in[oS + j] : 0.0);
@@ -1606,6 +1639,7 @@ int launch_ring(int mode, const RingItem* items, int n, const MapDesc* d_maps, i
     case kRingLike: return ring_m<kRingLike>(items, n, d_maps, lds, out, in, tw, s);
     case kRingSyn: return ring_m<kRingSyn>(items, n, d_maps, lds, out, in, tw, s);
     case kRingAna: return ring_m<kRingAna>(items, n, d_maps, lds, out, in, tw, s);
+    case kRingWeight: return ring_m<kRingWeight>(items, n, d_maps, lds, out, in, tw, s);
     }
     return fail(CMBL_ERR_ARG, "bad ring mode");
 }
@@ -1626,9 +1660,19 @@ int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_b
     return CMBL_OK;
 }
 
-int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, double* d_g, int like, hipStream_t s)
+int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, double* d_g, int mode, hipStream_t s,
+                double c)
 {
-    hipLaunchKernelGGL(k_post, dim3(lm_tiles(md.L)), dim3(256), 0, s, d_md, d_x, d_g, like);
+    hipLaunchKernelGGL(k_post, dim3(lm_tiles(md.L)), dim3(256), 0, s, d_md, d_x, d_g, mode, c);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_ldiv(double* y, const double* x, const double* dl, int L, hipStream_t s)
+{
+    const long n = (long)(L + 1) * (L + 1);
+    const int blocks = (int)std::min<long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_ldiv, dim3(blocks), dim3(256), 0, s, y, x, dl, L);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }

@@ -584,7 +584,7 @@ static int eval_device(cmbl_map* mp, const double* x, double* f, double* g, hipS
     CMBL_TRY(t.mark(3));
     CMBL_TRY(launch_adj(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, s));
     CMBL_TRY(t.mark(4));
-    CMBL_TRY(launch_post(d, mp->d_desc, x, g, 1, s));
+    CMBL_TRY(launch_post(d, mp->d_desc, x, g, kPostLike, s));
     CMBL_TRY(t.mark(5));
     CMBL_TRY(launch_final(mp->d_desc, f, s));
     CMBL_TRY(t.mark(6));
@@ -750,7 +750,7 @@ int cmbl_backward_device(cmbl_map* mp, const double* v, double* g, void* stream)
     hipStream_t s = (hipStream_t)stream;
     CMBL_TRY(run_ring(kRingAna, mp->d_ring_items, mp->ring_classes, mp->d_desc, nullptr, v, mp->d_tw, s));
     CMBL_TRY(launch_adj(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, s));
-    CMBL_TRY(launch_post(mp->desc, mp->d_desc, nullptr, g, 0, s));
+    CMBL_TRY(launch_post(mp->desc, mp->d_desc, nullptr, g, kPostGrad, s));
     return CMBL_OK;
 }
 
@@ -762,6 +762,220 @@ int cmbl_backward(cmbl_map* mp, const double* v, double* g)
     CMBL_TRY(cmbl_backward_device(mp, mp->d_map, mp->d_g, nullptr));
     CMBL_HIP(hipMemcpy(g, mp->d_g, sizeof(double) * mp->nparam, hipMemcpyDeviceToHost));
     return CMBL_OK;
+}
+
+// ---------------------------------------------------------------- Gibbs CG
+void cmbl_gibbs_destroy(cmbl_gibbs* gb)
+{
+    if (!gb) return;
+    (void)hipDeviceSynchronize();
+    for (void* p : gb->allocs) (void)hipFree(p);
+    cmbl_map_destroy(gb->mp);
+    delete gb;
+}
+
+int cmbl_gibbs_create(cmbl_gibbs** out, int nside, int lmax, const double* cl, const double* beam,
+                      const double* mask, double pixel_noise)
+{
+    if (!out || !cl || !beam || !mask) return fail(CMBL_ERR_ARG, "null pointer");
+    *out = nullptr;
+    if (!(pixel_noise > 0.0) || !std::isfinite(pixel_noise)) return fail(CMBL_ERR_ARG, "pixel noise must be > 0");
+    for (int l = 0; l <= lmax && lmax >= 0; ++l)
+        if (!(cl[l] >= 0.0) || !std::isfinite(cl[l]) || !std::isfinite(beam[l]))
+            return fail(CMBL_ERR_ARG, "C_l must be finite and >= 0, B_l finite");
+    cmbl_gibbs* gb = new cmbl_gibbs();
+    auto bail = [&](int rc) {
+        const std::string msg = g_err;
+        cmbl_gibbs_destroy(gb);
+        g_err = msg;
+        return rc;
+    };
+    int rc = cmbl_map_create(&gb->mp, nside, lmax);
+    if (rc) return bail(rc);
+    gb->L = lmax;
+    gb->n = gb->mp->nparam;
+    gb->npix = gb->mp->npix;
+    gb->sigma = pixel_noise;
+    // D_l = B_l sqrt(C_l) (source/cmb_gibbs.cpp:370-375); w = mask / sigma^2 (:381-387)
+    std::vector<double> dl(lmax + 1), w(gb->npix), zero(gb->npix, 0.0);
+    for (int l = 0; l <= lmax; ++l) dl[l] = beam[l] * std::sqrt(cl[l]);
+    for (long i = 0; i < gb->npix; ++i) {
+        const bool good = mask[i] > 0.5;
+        w[i] = good ? 1.0 / (pixel_noise * pixel_noise) : 0.0;
+        gb->good += good;
+    }
+    if (gb->good == 0) {
+        g_err = "the mask has no pixel > 0.5";
+        return bail(CMBL_ERR_ARG);
+    }
+    if ((rc = cmbl_map_set_beam(gb->mp, dl.data())) || (rc = cmbl_map_set_data(gb->mp, zero.data(), w.data())))
+        return bail(rc);
+    // preconditioner divisors (source/cmb_gibbs.cpp:412-418)
+    const double nl = pixel_noise * pixel_noise * 4 * 3.141592653589793 / (double)gb->good;
+    for (int l = 0; l <= lmax; ++l) dl[l] = 1.0 + cl[l] * beam[l] * beam[l] / nl;
+    auto alloc = [&](double** p, long count) {
+        void* q = nullptr;
+        if (hipMalloc(&q, sizeof(double) * (size_t)std::max<long>(count, 1)) != hipSuccess) {
+            g_err = "hipMalloc failed for the Gibbs workspace";
+            return CMBL_ERR_ALLOC;
+        }
+        gb->allocs.push_back(q);
+        *p = static_cast<double*>(q);
+        return CMBL_OK;
+    };
+    for (double** v : {&gb->d_r, &gb->d_z, &gb->d_p, &gb->d_ap, &gb->d_b, &gb->d_x})
+        if ((rc = alloc(v, gb->n))) return bail(rc);
+    for (double** v : {&gb->d_m0, &gb->d_m1, &gb->d_m2})
+        if ((rc = alloc(v, gb->npix))) return bail(rc);
+    if ((rc = alloc(&gb->d_dl, lmax + 1))) return bail(rc);
+    if (hipMemcpy(gb->d_dl, dl.data(), sizeof(double) * (lmax + 1), hipMemcpyHostToDevice) != hipSuccess) {
+        g_err = "upload of the preconditioner failed";
+        return bail(CMBL_ERR_HIP);
+    }
+    *out = gb;
+    return CMBL_OK;
+}
+
+static int check_gibbs(const cmbl_gibbs* gb)
+{
+    if (!gb || !gb->mp) return fail(CMBL_ERR_ARG, "gibbs context is null");
+    return CMBL_OK;
+}
+
+int cmbl_gibbs_apply_device(cmbl_gibbs* gb, const double* x, double* y, void* stream)
+{
+    CMBL_TRY(check_gibbs(gb));
+    if (!x || !y) return fail(CMBL_ERR_ARG, "null pointer");
+    cmbl_map* mp = gb->mp;
+    hipStream_t s = (hipStream_t)stream;
+    // D x -> Y -> w s -> Y^T -> D -> + x, one pass per stage
+    CMBL_TRY(launch_pre(mp->desc, mp->d_desc, x, s));
+    CMBL_TRY(launch_synth(mp->K1, mp->d_items1, mp->nitems1, mp->d_desc, s));
+    CMBL_TRY(run_ring(kRingWeight, mp->d_ring_items, mp->ring_classes, mp->d_desc, nullptr, nullptr, mp->d_tw, s));
+    CMBL_TRY(launch_adj(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, s));
+    CMBL_TRY(launch_post(mp->desc, mp->d_desc, x, y, kPostGibbs, s));
+    return CMBL_OK;
+}
+
+int cmbl_gibbs_precondition_device(cmbl_gibbs* gb, const double* x, double* y, void* stream)
+{
+    CMBL_TRY(check_gibbs(gb));
+    if (!x || !y) return fail(CMBL_ERR_ARG, "null pointer");
+    return launch_ldiv(y, x, gb->d_dl, gb->L, (hipStream_t)stream);
+}
+
+int cmbl_gibbs_rhs_device(cmbl_gibbs* gb, const double* d, const double* omega0, const double* omega1, double* b,
+                          void* stream)
+{
+    CMBL_TRY(check_gibbs(gb));
+    if (!d || !omega0 || !omega1 || !b) return fail(CMBL_ERR_ARG, "null pointer");
+    cmbl_map* mp = gb->mp;
+    hipStream_t s = (hipStream_t)stream;
+    // w (d + sigma omega1) = N^-1 d + N^-1/2 omega1 on the good pixels
+    CMBL_TRY(cmbl_vec_copy(gb->d_m0, d, 1.0, gb->npix, s));
+    CMBL_TRY(cmbl_vec_axpy(gb->d_m0, omega1, gb->sigma, gb->npix, s));
+    CMBL_TRY(cmbl_vec_mul(gb->d_m0, mp->d_w, gb->npix, s));
+    CMBL_TRY(run_ring(kRingAna, mp->d_ring_items, mp->ring_classes, mp->d_desc, nullptr, gb->d_m0, mp->d_tw, s));
+    CMBL_TRY(launch_adj(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, s));
+    CMBL_TRY(launch_post(mp->desc, mp->d_desc, nullptr, b, kPostAlm, s));
+    // + sqrt(4pi/Npix) Y^T omega0 (map2alm with ring weight sqrt(Npix/4pi), :479)
+    CMBL_TRY(run_ring(kRingAna, mp->d_ring_items, mp->ring_classes, mp->d_desc, nullptr, omega0, mp->d_tw, s));
+    CMBL_TRY(launch_adj(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, s));
+    CMBL_TRY(launch_post(mp->desc, mp->d_desc, nullptr, b, kPostAddAlm, s,
+                         std::sqrt(4.0 * 3.141592653589793 / (double)gb->npix)));
+    return CMBL_OK;
+}
+
+int cmbl_gibbs_solve_device(cmbl_gibbs* gb, const double* b, double* x, double precision, int max_iter,
+                            int* iterations, void* stream)
+{
+    CMBL_TRY(check_gibbs(gb));
+    if (!b || !x) return fail(CMBL_ERR_ARG, "null pointer");
+    if (!(precision > 0.0)) return fail(CMBL_ERR_ARG, "precision must be > 0");
+    if (max_iter < 1) return fail(CMBL_ERR_ARG, "max_iter must be >= 1");
+    hipStream_t s = (hipStream_t)stream;
+    const long n = gb->n;
+    double *r = gb->d_r, *z = gb->d_z, *p = gb->d_p, *ap = gb->d_ap;
+    if (iterations) *iterations = 0;
+    // x = 0, r = b - A x, z = M^-1 r, p = z (include/conjugate_gradient.hpp:51-58)
+    CMBL_TRY(cmbl_vec_set(x, 0.0, n, s));
+    CMBL_TRY(cmbl_gibbs_apply_device(gb, x, r, s));
+    CMBL_TRY(cmbl_vec_copy(r, r, -1.0, n, s));
+    CMBL_TRY(cmbl_vec_axpy(r, b, 1.0, n, s));
+    CMBL_TRY(cmbl_gibbs_precondition_device(gb, r, z, s));
+    CMBL_TRY(cmbl_vec_copy(p, z, 1.0, n, s));
+    double eps = 0.0;
+    int it = 0;
+    do {  // :64-97
+        double rz = 0.0, pap = 0.0, rz_new = 0.0;
+        CMBL_TRY(cmbl_vec_dot(r, z, n, &rz, s));
+        CMBL_TRY(cmbl_gibbs_apply_device(gb, p, ap, s));
+        CMBL_TRY(cmbl_vec_dot(p, ap, n, &pap, s));
+        if (pap == 0.0) return fail(CMBL_ERR_STATE, "CG breakdown: p.Ap = 0");
+        const double alpha = rz / pap;
+        CMBL_TRY(cmbl_vec_axpy(x, p, alpha, n, s));
+        CMBL_TRY(cmbl_vec_axpy(r, ap, -alpha, n, s));
+        CMBL_TRY(cmbl_vec_dot(r, r, n, &eps, s));
+        CMBL_TRY(cmbl_gibbs_precondition_device(gb, r, z, s));
+        CMBL_TRY(cmbl_vec_dot(r, z, n, &rz_new, s));
+        if (rz == 0.0) return fail(CMBL_ERR_STATE, "CG breakdown: r.z = 0");
+        const double beta = rz_new / rz;
+        CMBL_TRY(cmbl_vec_copy(p, p, beta, n, s));
+        CMBL_TRY(cmbl_vec_axpy(p, z, 1.0, n, s));
+        ++it;
+        if (iterations) *iterations = it;
+        if (!std::isfinite(eps)) return fail(CMBL_ERR_NONFINITE, "CG residual is not finite");
+    } while (eps > precision * precision && it < max_iter);
+    if (eps > precision * precision)
+        return fail(CMBL_ERR_NOCONV, "CG reached " + std::to_string(max_iter) + " iterations, |r|^2 = " +
+                                         std::to_string(eps));
+    return CMBL_OK;
+}
+
+int cmbl_gibbs_apply(cmbl_gibbs* gb, const double* x, double* y)
+{
+    CMBL_TRY(check_gibbs(gb));
+    if (!x || !y) return fail(CMBL_ERR_ARG, "null pointer");
+    CMBL_HIP(hipMemcpy(gb->d_x, x, sizeof(double) * gb->n, hipMemcpyHostToDevice));
+    CMBL_TRY(cmbl_gibbs_apply_device(gb, gb->d_x, gb->d_b, nullptr));
+    CMBL_HIP(hipMemcpy(y, gb->d_b, sizeof(double) * gb->n, hipMemcpyDeviceToHost));
+    return CMBL_OK;
+}
+
+int cmbl_gibbs_precondition(cmbl_gibbs* gb, const double* x, double* y)
+{
+    CMBL_TRY(check_gibbs(gb));
+    if (!x || !y) return fail(CMBL_ERR_ARG, "null pointer");
+    CMBL_HIP(hipMemcpy(gb->d_x, x, sizeof(double) * gb->n, hipMemcpyHostToDevice));
+    CMBL_TRY(cmbl_gibbs_precondition_device(gb, gb->d_x, gb->d_b, nullptr));
+    CMBL_HIP(hipMemcpy(y, gb->d_b, sizeof(double) * gb->n, hipMemcpyDeviceToHost));
+    return CMBL_OK;
+}
+
+int cmbl_gibbs_rhs(cmbl_gibbs* gb, const double* d, const double* omega0, const double* omega1, double* b)
+{
+    CMBL_TRY(check_gibbs(gb));
+    if (!d || !omega0 || !omega1 || !b) return fail(CMBL_ERR_ARG, "null pointer");
+    const size_t bytes = sizeof(double) * gb->npix;
+    CMBL_HIP(hipMemcpy(gb->d_m1, d, bytes, hipMemcpyHostToDevice));
+    CMBL_HIP(hipMemcpy(gb->d_m2, omega1, bytes, hipMemcpyHostToDevice));
+    CMBL_HIP(hipMemcpy(gb->mp->d_map, omega0, bytes, hipMemcpyHostToDevice));
+    CMBL_TRY(cmbl_gibbs_rhs_device(gb, gb->d_m1, gb->mp->d_map, gb->d_m2, gb->d_b, nullptr));
+    CMBL_HIP(hipMemcpy(b, gb->d_b, sizeof(double) * gb->n, hipMemcpyDeviceToHost));
+    return CMBL_OK;
+}
+
+int cmbl_gibbs_solve(cmbl_gibbs* gb, const double* b, double* x, double precision, int max_iter, int* iterations)
+{
+    CMBL_TRY(check_gibbs(gb));
+    if (!b || !x) return fail(CMBL_ERR_ARG, "null pointer");
+    CMBL_HIP(hipMemcpy(gb->d_b, b, sizeof(double) * gb->n, hipMemcpyHostToDevice));
+    const int rc = cmbl_gibbs_solve_device(gb, gb->d_b, gb->d_x, precision, max_iter, iterations, nullptr);
+    if (rc != CMBL_OK && rc != CMBL_ERR_NOCONV) return rc;
+    const std::string msg = g_err;
+    CMBL_HIP(hipMemcpy(x, gb->d_x, sizeof(double) * gb->n, hipMemcpyDeviceToHost));
+    g_err = msg;
+    return rc;
 }
 
 // ---------------------------------------------------------------- batch
@@ -858,7 +1072,7 @@ int cmbl_batch_value_and_grad_device(cmbl_batch* b, const double* const* x, doub
     CMBL_TRY(run_ring(kRingLike, b->d_ring_items, b->ring_classes, b->d_descs, nullptr, nullptr, b->d_tw, s));
     for (int k = 0; k < 4; ++k) CMBL_TRY(launch_adj(1 << k, b->d_items5[k], b->n5[k], b->d_descs, s));
     for (int i = 0; i < n; ++i) {
-        CMBL_TRY(launch_post(b->maps[i]->desc, b->d_descs + i, x[i], g[i], 1, s));
+        CMBL_TRY(launch_post(b->maps[i]->desc, b->d_descs + i, x[i], g[i], kPostLike, s));
         CMBL_TRY(launch_final(b->d_descs + i, f + i, s));
     }
     return CMBL_OK;

@@ -35,9 +35,11 @@ extern "C" {
 #define CMBL_ERR_ALLOC 3     /* device or host allocation failed */
 #define CMBL_ERR_STATE 4     /* call out of order (e.g. no data set yet) */
 #define CMBL_ERR_NONFINITE 5 /* NaN/Inf in the likelihood value */
+#define CMBL_ERR_NOCONV 6    /* iteration limit reached before convergence */
 
 typedef struct cmbl_map cmbl_map;     /* one map: geometry, tables, data, workspace */
 typedef struct cmbl_batch cmbl_batch; /* many maps evaluated in one launch per stage */
+typedef struct cmbl_gibbs cmbl_gibbs; /* Gibbs constrained-realisation system */
 
 const char* cmbl_last_error(void);
 /* Library version string. */
@@ -108,6 +110,41 @@ void cmbl_batch_destroy(cmbl_batch* batch);
 /* d_x[i], d_g[i]: device pointers for map i; d_f: device array of nmaps. */
 int cmbl_batch_value_and_grad_device(cmbl_batch* batch, const double* const* d_x, double* d_f,
                                      double* const* d_g, void* stream);
+
+/* ---------------------------------------------------------------- Gibbs CG */
+/* The constrained-realisation system of CMBGibbsSampler::generateSignal,
+ * A x = b with A = I + D Y^T N^-1 Y D, D = B_l sqrt(C_l), N^-1 = mask/sigma^2
+ * (mask > 0.5), Y^T = map2alm with ring weight Npix/4pi, in the HEALPix
+ * complex-a_lm convention and the packed layout: the operator of
+ * CmbGibbsCGTreats (source/cmb_gibbs.cpp:303-434).  C_l >= 0, sigma > 0,
+ * at least one pixel with mask > 0.5. */
+int cmbl_gibbs_create(cmbl_gibbs** out, int nside, int lmax, const double* cl, const double* beam,
+                      const double* mask, double pixel_noise);
+void cmbl_gibbs_destroy(cmbl_gibbs* gibbs);
+/* y = A x: CmbGibbsCGTreats::multiplyByMatrix (source/cmb_gibbs.cpp:359-405). */
+int cmbl_gibbs_apply(cmbl_gibbs* gibbs, const double* x, double* y);
+int cmbl_gibbs_apply_device(cmbl_gibbs* gibbs, const double* d_x, double* d_y, void* stream);
+/* y_lm = x_lm / (1 + C_l B_l^2 / N_l), N_l = sigma^2 4pi / (pixels with mask > 0.5):
+ * CmbGibbsCGTreats::preconditioner (source/cmb_gibbs.cpp:407-423). */
+int cmbl_gibbs_precondition(cmbl_gibbs* gibbs, const double* x, double* y);
+int cmbl_gibbs_precondition_device(cmbl_gibbs* gibbs, const double* d_x, double* d_y, void* stream);
+/* Right-hand side of generateSignal (source/cmb_gibbs.cpp:437-487):
+ * b = D Y^T (N^-1 d + N^-1/2 omega1) + sqrt(4pi/Npix) Y^T omega0, for the
+ * data map d (monopole and dipole already removed, as :449) and the two
+ * white-noise maps omega0, omega1 (npix each). */
+int cmbl_gibbs_rhs(cmbl_gibbs* gibbs, const double* d, const double* omega0, const double* omega1, double* b);
+int cmbl_gibbs_rhs_device(cmbl_gibbs* gibbs, const double* d_d, const double* d_omega0, const double* d_omega1,
+                          double* d_b, void* stream);
+/* Math::ConjugateGradient::solve (include/conjugate_gradient.hpp:45-102) on
+ * the device: x0 = 0, preconditioned CG, stop when |r|^2 <= precision^2.
+ * max_iter > 0 bounds the iterations (the reference has no bound):
+ * CMBL_ERR_NOCONV if reached, x then holds the last iterate.  *iterations
+ * (may be NULL) receives the count.  The device variant blocks on each
+ * iteration's scalars, as the reference's host loop does. */
+int cmbl_gibbs_solve(cmbl_gibbs* gibbs, const double* b, double* x, double precision, int max_iter,
+                     int* iterations);
+int cmbl_gibbs_solve_device(cmbl_gibbs* gibbs, const double* d_b, double* d_x, double precision, int max_iter,
+                            int* iterations, void* stream);
 
 /* ---------------------------------------------------------------- inputs (host) */
 /* Simulate::simulateAlm (source/simulate.cpp:162-192): mt19937(seed) +

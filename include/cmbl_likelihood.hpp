@@ -266,6 +266,71 @@ private:
     long evals_ = 0;
 };
 
+/// The Gibbs constrained-realisation system of CMBGibbsSampler::generateSignal
+/// (source/cmb_gibbs.cpp:437-530) on the GPU.  multiplyByMatrix and
+/// preconditioner make it a CGTreats for the reference's
+/// Math::ConjugateGradient (include/conjugate_gradient.hpp:13-19), the way
+/// CmbGibbsCGTreats is (source/cmb_gibbs.cpp:303-434); solve() runs the same
+/// CG on the device without moving vectors to the host.
+class GibbsSystem
+{
+public:
+    GibbsSystem(int nside, int lmax, const std::vector<double>& cl, const std::vector<double>& beam,
+                const std::vector<double>& mask, double pixelNoise)
+        : n_((long)(lmax + 1) * (lmax + 1)), npix_(12L * nside * nside)
+    {
+        if ((long)cl.size() != lmax + 1 || (long)beam.size() != lmax + 1 || (long)mask.size() != npix_)
+            throw Error(CMBL_ERR_ARG, "GibbsSystem: cl, beam or mask has the wrong size");
+        raise_on_error(cmbl_gibbs_create(&g_, nside, lmax, cl.data(), beam.data(), mask.data(), pixelNoise));
+    }
+    ~GibbsSystem() { cmbl_gibbs_destroy(g_); }
+    GibbsSystem(const GibbsSystem&) = delete;
+    GibbsSystem& operator=(const GibbsSystem&) = delete;
+
+    long size() const { return n_; }
+    cmbl_gibbs* get() { return g_; }
+
+    void multiplyByMatrix(const std::vector<double>& original, std::vector<double>& result) const
+    {
+        need(original.size());
+        result.resize(n_);
+        raise_on_error(cmbl_gibbs_apply(g_, original.data(), result.data()));
+    }
+    void preconditioner(const std::vector<double>& original, std::vector<double>& result) const
+    {
+        need(original.size());
+        result.resize(n_);
+        raise_on_error(cmbl_gibbs_precondition(g_, original.data(), result.data()));
+    }
+    /// b of generateSignal from the data map (monopole and dipole removed)
+    /// and the two white-noise maps.
+    void rhs(const std::vector<double>& d, const std::vector<double>& omega0, const std::vector<double>& omega1,
+             std::vector<double>& b) const
+    {
+        if ((long)d.size() != npix_ || (long)omega0.size() != npix_ || (long)omega1.size() != npix_)
+            throw Error(CMBL_ERR_ARG, "GibbsSystem::rhs: maps must have npix values");
+        b.resize(n_);
+        raise_on_error(cmbl_gibbs_rhs(g_, d.data(), omega0.data(), omega1.data(), b.data()));
+    }
+    /// Device CG from x = 0; returns the iteration count.
+    int solve(const std::vector<double>& b, std::vector<double>& x, double precision, int maxIter) const
+    {
+        need(b.size());
+        x.resize(n_);
+        int it = 0;
+        raise_on_error(cmbl_gibbs_solve(g_, b.data(), x.data(), precision, maxIter, &it));
+        return it;
+    }
+
+private:
+    void need(size_t have) const
+    {
+        if ((long)have != n_) throw Error(CMBL_ERR_ARG, "GibbsSystem: vector size " + std::to_string(have));
+    }
+    long n_, npix_;
+    cmbl_gibbs* g_ = nullptr;
+};
+
 }  // namespace cmbl
 
 #endif

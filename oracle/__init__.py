@@ -153,6 +153,96 @@ def num_threads():
     return lib().orc_num_threads()
 
 
+# ---------------------------------------------------------------- Gibbs CG system
+# CmbGibbsCGTreats (source/cmb_gibbs.cpp:303-434) over the oracle's synthesis
+# and adjoint.  HEALPix map2alm returns complex a_lm (u = Y^T_c f); in the
+# packed real layout that is the real-parameter transpose (adjoint(), which has
+# the factor 2 for m >= 1) halved for m >= 1.
+
+def m_half(L):
+    """1 for the packed entries with m = 0, 1/2 for m >= 1."""
+    p = np.arange((L + 1) ** 2)
+    l = np.floor(np.sqrt(p)).astype(np.int64)
+    l -= (l * l > p)
+    l += ((l + 1) * (l + 1) <= p)
+    return np.where(p == l * l, 1.0, 0.5)
+
+
+def packed_l(L):
+    p = np.arange((L + 1) ** 2)
+    l = np.floor(np.sqrt(p)).astype(np.int64)
+    l -= (l * l > p)
+    l += ((l + 1) * (l + 1) <= p)
+    return l
+
+
+def gibbs_weights(mask, sigma):
+    """N^-1 = 1/sigma^2 where mask > 0.5 (source/cmb_gibbs.cpp:381-387)."""
+    mask = np.asarray(mask, dtype=np.float64)
+    return np.where(mask > 0.5, 1.0 / (sigma * sigma), 0.0)
+
+
+def gibbs_apply(nside, L, x, cl, beam, mask, sigma):
+    """multiplyByMatrix (source/cmb_gibbs.cpp:359-405): x + D Y^T_c N^-1 Y D x, D = B sqrt(C)."""
+    D = np.asarray(beam, dtype=np.float64) * np.sqrt(np.asarray(cl, dtype=np.float64))
+    s = synthesis(nside, L, x, D)
+    u = adjoint(nside, L, gibbs_weights(mask, sigma) * s, D) * m_half(L)
+    return np.asarray(x, dtype=np.float64) + u
+
+
+def gibbs_precondition(L, x, cl, beam, mask, sigma):
+    """preconditioner (source/cmb_gibbs.cpp:407-423): x_lm / (1 + C_l B_l^2 / N_l)."""
+    good = int(np.count_nonzero(np.asarray(mask) > 0.5))
+    nl = sigma * sigma * 4 * np.pi / good
+    cl = np.asarray(cl, dtype=np.float64)
+    b = np.asarray(beam, dtype=np.float64)
+    d = 1.0 + cl * b * b / nl
+    return np.asarray(x, dtype=np.float64) / d[packed_l(L)]
+
+
+def gibbs_rhs(nside, L, d, omega0, omega1, cl, beam, mask, sigma):
+    """generateSignal's b (source/cmb_gibbs.cpp:444-487):
+    D Y^T_c (N^-1 d + N^-1/2 omega1) + sqrt(4 pi / Npix) Y^T_c omega0."""
+    good = np.asarray(mask) > 0.5
+    m1 = np.where(good, np.asarray(d) / (sigma * sigma), 0.0)
+    o1 = np.where(good, np.asarray(omega1) / sigma, 0.0)
+    D = np.asarray(beam, dtype=np.float64) * np.sqrt(np.asarray(cl, dtype=np.float64))
+    half = m_half(L)
+    npix = 12 * nside * nside
+    b = adjoint(nside, L, m1 + o1, D) * half
+    b += np.sqrt(4 * np.pi / npix) * adjoint(nside, L, np.asarray(omega0, dtype=np.float64)) * half
+    return b
+
+
+def cg_solve(apply, precond, b, precision=1e-7, max_iter=None):
+    """Math::ConjugateGradient::solve (include/conjugate_gradient.hpp:45-102):
+    x0 = 0, r = b - A x0, z = M^-1 r, p = z; loop until |r|^2 <= precision^2.
+    Returns (x, iterations)."""
+    b = np.asarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    r = b - apply(x)
+    z = precond(r)
+    p = z.copy()
+    it = 0
+    while True:
+        rz = float(np.dot(r, z))
+        ap = apply(p)
+        pap = float(np.dot(p, ap))
+        if pap == 0:
+            raise ZeroDivisionError("p.Ap = 0")
+        alpha = rz / pap
+        x += alpha * p
+        r -= alpha * ap
+        eps = float(np.dot(r, r))
+        z = precond(r)
+        rz_new = float(np.dot(r, z))
+        beta = rz_new / rz
+        p = z + beta * p
+        it += 1
+        if eps <= precision * precision or (max_iter is not None and it >= max_iter):
+            return x, it
+
+
 # ---------------------------------------------------------------- reference shim
 _ref = None
 

@@ -10,6 +10,8 @@
 //   lbfgs_device   Math::LBFGS_General on cmbl::DeviceLargeVector / DeviceMapFunction
 //   hmc            Math::HMC on cmbl::MapLikelihood (LikelihoodWithDerivs)
 //   hmc_device     Math::HMCGeneral on device vectors
+//   cg             Math::ConjugateGradient on cmbl::GibbsSystem (the Gibbs
+//                  constrained-realisation system), against the device solve
 // Prints one JSON object on stdout.
 #include <cmath>
 #include <cstdio>
@@ -24,6 +26,7 @@
 #include <lbfgs_general.hpp>
 #include <hmc.hpp>
 #include <hmc_general.hpp>
+#include <conjugate_gradient.hpp>
 
 #include <cmbl_likelihood.hpp>
 
@@ -85,7 +88,7 @@ struct ChainStats {
 int main(int argc, char** argv)
 {
     if (argc < 4) {
-        std::fprintf(stderr, "usage: %s <lbfgs|lbfgs_device|hmc|hmc_device> <nside> <lmax> [iters]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <lbfgs|lbfgs_device|hmc|hmc_device|cg> <nside> <lmax> [iters]\n", argv[0]);
         return 2;
     }
     const std::string mode = argv[1];
@@ -156,6 +159,28 @@ int main(int argc, char** argv)
             std::printf("{\"mode\": \"hmc_device\", \"samples\": %ld, \"evals\": %ld, \"mean_like\": %.17g, ", cb.n,
                         func.evaluations(), cb.like_sum / cb.n);
             print_vec("mean", cb.mean);
+            std::printf("}\n");
+        } else if (mode == "cg") {
+            // Gibbs system on the same data: mask 20 deg cut, sigma 10, white
+            // noise maps omega0, omega1 of unit variance (seeds 601, 602)
+            const long npix = 12L * nside * nside;
+            std::vector<double> mask(npix, 1.0), w0(npix), w1(npix), b, xcg, xdev;
+            cmbl::raise_on_error(cmbl_mask_latitude(nside, 20.0 * M_PI / 180.0, mask.data()));
+            cmbl::raise_on_error(cmbl_simulate_white_noise(nside, 1.0, 601, w0.data()));
+            cmbl::raise_on_error(cmbl_simulate_white_noise(nside, 1.0, 602, w1.data()));
+            cmbl::GibbsSystem sys(nside, lmax, in.cl, in.beam, mask, 10.0);
+            sys.rhs(in.d, w0, w1, b);
+            const double precision = 1e-8;
+            Math::ConjugateGradient<cmbl::GibbsSystem> cg((int)b.size(), &sys, b);
+            int it_cg = 0;
+            xcg = cg.solve(precision, &it_cg);
+            const int it_dev = sys.solve(b, xdev, precision, 100000);
+            std::printf("{\"mode\": \"cg\", \"iter_cg\": %d, \"iter_dev\": %d, ", it_cg, it_dev);
+            print_vec("b", b);
+            std::printf(", ");
+            print_vec("x_cg", xcg);
+            std::printf(", ");
+            print_vec("x_dev", xdev);
             std::printf("}\n");
         } else {
             std::fprintf(stderr, "unknown mode %s\n", mode.c_str());

@@ -1,10 +1,13 @@
-"""GPU: the reference's own optimisers and samplers drive the MI355X likelihood.
+"""GPU: the reference's own optimisers, samplers and CG solver drive the
+MI355X likelihood and Gibbs system.
 
 oracle/_ref/ref_consumers is compiled from the reference sources
 (source/lbfgs.cpp, source/hmc.cpp, include/lbfgs_general.hpp,
-include/hmc_general.hpp, ...) plus include/cmbl_likelihood.hpp.  The L-BFGS
-results are checked against the exact minimiser of the quadratic -2lnL,
-built column by column from the oracle's gradients.
+include/hmc_general.hpp, include/conjugate_gradient.hpp, ...) plus
+include/cmbl_likelihood.hpp.  The L-BFGS results are checked against the
+exact minimiser of the quadratic -2lnL, built column by column from the
+oracle's gradients; the reference's ConjugateGradient on cmbl::GibbsSystem
+against the device CG and the oracle's.
 """
 import json
 import math
@@ -93,3 +96,21 @@ def test_reference_hmc_general_on_device_vectors(problem):
     assert out["samples"] == 41
     assert out["evals"] <= 1 + 40 * (10 + 2)
     assert np.linalg.norm(out["mean"]) > 0
+
+
+@needs_bin
+def test_reference_conjugate_gradient_on_gibbs_system(problem):
+    out = run("cg", 0)
+    mask = cm.mask_latitude(NSIDE, math.radians(20.0))
+    w0 = cm.simulate_white_noise(NSIDE, 1.0, 601)
+    w1 = cm.simulate_white_noise(NSIDE, 1.0, 602)
+    b = oracle.gibbs_rhs(NSIDE, LMAX, problem["d"], w0, w1, problem["cl"], problem["b"], mask, 10.0)
+    assert np.linalg.norm(np.array(out["b"]) - b) <= 1e-10 * np.linalg.norm(b)
+    A = lambda v: oracle.gibbs_apply(NSIDE, LMAX, v, problem["cl"], problem["b"], mask, 10.0)
+    M = lambda v: oracle.gibbs_precondition(LMAX, v, problem["cl"], problem["b"], mask, 10.0)
+    xo, ito = oracle.cg_solve(A, M, b, precision=1e-8)
+    for key in ("x_cg", "x_dev"):
+        x = np.array(out[key])
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+        assert np.linalg.norm(A(x) - b) <= 1e-7
+    assert abs(out["iter_cg"] - ito) <= 2 and abs(out["iter_dev"] - ito) <= 2

@@ -5,7 +5,9 @@
 // include/cmbl_likelihood.hpp.  TEST INFRASTRUCTURE ONLY: built by
 // oracle/Makefile (target `consumers`) into oracle/_ref/.
 //
-// Usage: ref_consumers <mode> <nside> <lmax> [iters]
+// Usage: ref_consumers <mode> <nside> <lmax> [iters] [fwhm_deg] [sigma] [vectors]
+// (defaults 200, 5, 10, 1; vectors = 0 leaves the result vectors out of the
+// output, for timing runs at large sizes).
 //   lbfgs          Math::LBFGS on cmbl::MapLikelihood (host vectors)
 //   lbfgs_device   Math::LBFGS_General on cmbl::DeviceLargeVector / DeviceMapFunction
 //   hmc            Math::HMC on cmbl::MapLikelihood (LikelihoodWithDerivs)
@@ -13,6 +15,7 @@
 //   cg             Math::ConjugateGradient on cmbl::GibbsSystem (the Gibbs
 //                  constrained-realisation system), against the device solve
 // Prints one JSON object on stdout.
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -36,26 +39,33 @@ struct Inputs {
     std::vector<double> cl, beam, d, w, x0;
 };
 
-Inputs make_inputs(cmbl::MapContext& ctx, int nside, int lmax)
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+bool g_vectors = true;
+
+Inputs make_inputs(cmbl::MapContext& ctx, int nside, int lmax, double fwhm_deg, double sigma)
 {
     Inputs in;
     const long npix = 12L * nside * nside;
     in.cl.assign(lmax + 1, 0.0);
     for (int l = 2; l <= lmax; ++l) in.cl[l] = 1e3 / (l * (l + 1.0));
     in.beam.resize(lmax + 1);
-    cmbl::raise_on_error(cmbl_beam(lmax, 5.0, in.beam.data()));
+    cmbl::raise_on_error(cmbl_beam(lmax, fwhm_deg, in.beam.data()));
     std::vector<double> a((size_t)(lmax + 1) * (lmax + 1));
     cmbl::raise_on_error(cmbl_simulate_alm(in.cl.data(), lmax, 501, a.data()));
     ctx.setBeam(in.beam);
     ctx.setCl(in.cl);
     std::vector<double> s(npix), noise(npix), mask(npix, 1.0);
     cmbl::raise_on_error(cmbl_forward(ctx.get(), a.data(), s.data()));
-    cmbl::raise_on_error(cmbl_simulate_white_noise(nside, 10.0, 502, noise.data()));
+    cmbl::raise_on_error(cmbl_simulate_white_noise(nside, sigma, 502, noise.data()));
     in.d.resize(npix);
     for (long i = 0; i < npix; ++i) in.d[i] = s[i] + noise[i];
     cmbl::raise_on_error(cmbl_mask_latitude(nside, 20.0 * M_PI / 180.0, mask.data()));
     in.w.resize(npix);
-    cmbl::raise_on_error(cmbl_weights_from_mask(npix, mask.data(), 10.0, in.w.data()));
+    cmbl::raise_on_error(cmbl_weights_from_mask(npix, mask.data(), sigma, in.w.data()));
     ctx.setData(in.d, in.w);
     in.x0.assign(a.size(), 0.0);
     return in;
@@ -63,6 +73,10 @@ Inputs make_inputs(cmbl::MapContext& ctx, int nside, int lmax)
 
 void print_vec(const char* name, const std::vector<double>& v)
 {
+    if (!g_vectors) {
+        std::printf("\"%s\": []", name);
+        return;
+    }
     std::printf("\"%s\": [", name);
     for (size_t i = 0; i < v.size(); ++i) std::printf("%s%.17g", i ? "," : "", v[i]);
     std::printf("]");
@@ -95,9 +109,12 @@ int main(int argc, char** argv)
     const int nside = std::atoi(argv[2]);
     const int lmax = std::atoi(argv[3]);
     const int iters = argc > 4 ? std::atoi(argv[4]) : 200;
+    const double fwhm = argc > 5 ? std::atof(argv[5]) : 5.0;
+    const double sigma = argc > 6 ? std::atof(argv[6]) : 10.0;
+    g_vectors = argc > 7 ? std::atoi(argv[7]) != 0 : true;
     try {
         cmbl::MapContext ctx(nside, lmax);
-        Inputs in = make_inputs(ctx, nside, lmax);
+        Inputs in = make_inputs(ctx, nside, lmax, fwhm, sigma);
         const long n = ctx.nParams();
         if (mode == "lbfgs") {
             cmbl::MapLikelihood like(ctx);
@@ -120,13 +137,15 @@ int main(int argc, char** argv)
             Math::LBFGS_General<cmbl::DeviceLargeVector, cmbl::DeviceLargeVectorFactory, cmbl::DeviceMapFunction> lbfgs(
                 &factory, &func, start, 10);
             cmbl::DeviceLargeVector res(n);
+            const double t0 = now_s();
             const double f = lbfgs.minimize(&res, 1e-14, 1e-8, iters);
+            const double t1 = now_s();
             cmbl::DeviceLargeVector g(n);
             lbfgs.getGradient(&g);
             std::vector<double> h;
             res.download(&h);
-            std::printf("{\"mode\": \"lbfgs_device\", \"f\": %.17g, \"gnorm\": %.17g, \"evals\": %ld, ", f, g.norm(),
-                        func.evaluations());
+            std::printf("{\"mode\": \"lbfgs_device\", \"f\": %.17g, \"gnorm\": %.17g, \"evals\": %ld, "
+                        "\"seconds\": %.6f, ", f, g.norm(), func.evaluations(), t1 - t0);
             print_vec("x", h);
             std::printf("}\n");
         } else if (mode == "hmc") {
@@ -154,10 +173,12 @@ int main(int argc, char** argv)
             Math::HMCGeneral<cmbl::DeviceLargeVector, cmbl::DeviceLargeVectorFactory, cmbl::DeviceMapFunction> hmc(
                 &factory, &func, start, masses, 0.5, 10, 778);
             ChainStats cb;
+            const double t0 = now_s();
             hmc.run(iters, &cb);
+            const double t1 = now_s();
             for (double& v : cb.mean) v /= (double)cb.n;
-            std::printf("{\"mode\": \"hmc_device\", \"samples\": %ld, \"evals\": %ld, \"mean_like\": %.17g, ", cb.n,
-                        func.evaluations(), cb.like_sum / cb.n);
+            std::printf("{\"mode\": \"hmc_device\", \"samples\": %ld, \"evals\": %ld, \"mean_like\": %.17g, "
+                        "\"seconds\": %.6f, ", cb.n, func.evaluations(), cb.like_sum / cb.n, t1 - t0);
             print_vec("mean", cb.mean);
             std::printf("}\n");
         } else if (mode == "cg") {
@@ -168,14 +189,21 @@ int main(int argc, char** argv)
             cmbl::raise_on_error(cmbl_mask_latitude(nside, 20.0 * M_PI / 180.0, mask.data()));
             cmbl::raise_on_error(cmbl_simulate_white_noise(nside, 1.0, 601, w0.data()));
             cmbl::raise_on_error(cmbl_simulate_white_noise(nside, 1.0, 602, w1.data()));
-            cmbl::GibbsSystem sys(nside, lmax, in.cl, in.beam, mask, 10.0);
+            cmbl::GibbsSystem sys(nside, lmax, in.cl, in.beam, mask, sigma);
             sys.rhs(in.d, w0, w1, b);
-            const double precision = 1e-8;
+            double bn = 0;
+            for (double v : b) bn += v * v;
+            // absolute precision of the reference's solve(); 1e-10 |b| relative
+            const double precision = 1e-10 * std::sqrt(bn);
             Math::ConjugateGradient<cmbl::GibbsSystem> cg((int)b.size(), &sys, b);
             int it_cg = 0;
+            const double t0 = now_s();
             xcg = cg.solve(precision, &it_cg);
+            const double t1 = now_s();
             const int it_dev = sys.solve(b, xdev, precision, 100000);
-            std::printf("{\"mode\": \"cg\", \"iter_cg\": %d, \"iter_dev\": %d, ", it_cg, it_dev);
+            const double t2 = now_s();
+            std::printf("{\"mode\": \"cg\", \"iter_cg\": %d, \"iter_dev\": %d, \"precision\": %.17g, "
+                        "\"seconds_cg\": %.6f, \"seconds_dev\": %.6f, ", it_cg, it_dev, precision, t1 - t0, t2 - t1);
             print_vec("b", b);
             std::printf(", ");
             print_vec("x_cg", xcg);

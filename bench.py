@@ -193,6 +193,8 @@ def main():
     ap.add_argument("--config", type=int, default=4, choices=sorted(CONFIGS) + [3],
                     help="workload: 4 (default, the headline), 1, 2, 5 (main map), 3 (64-map batch)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--host-buffers", action="store_true",
+                    help="time cmbl_value_and_grad with x and g in host memory (PCIe-inclusive rate)")
     ap.add_argument("--cpu-budget", type=float, default=40.0, help="seconds for the CPU baseline leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates HBM byte counters)")
@@ -212,9 +214,10 @@ def main():
         return
 
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box in this setup has a single MI355X, and the "
-              "multi-GPU layout (independent maps per GPU, m-sharded single maps) is described in "
-              "DESIGN.md but not run.  Use --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: every box in this setup has a single MI355X.  Several GPUs "
+              "would take whole maps each (independent maps, LPT by Legendre cost) or m-slices of one map "
+              "with two all-to-all transposes around the ring stage; that layout is not run here.  "
+              "Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -235,6 +238,20 @@ def main():
     f = torch.empty(1, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
+
+    if args.host_buffers:
+        xh = inputs["x"]
+        for _ in range(args.warmup):
+            mp.value_and_grad(xh)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fh, gh = mp.value_and_grad(xh)
+        rate = args.steps / (time.perf_counter() - t0)
+        print(json.dumps({"metric": "host-buffer value_and_grad evals/s (x in, g out over PCIe)",
+                          "value": round(rate, 3), "unit": "evals/s", "nside": nside, "lmax": lmax,
+                          "steps": args.steps, "neg2lnL": fh}), flush=True)
+        mp.close()
+        return
 
     log("bench: warmup %d" % args.warmup)
     for _ in range(args.warmup):

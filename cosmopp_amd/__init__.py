@@ -87,6 +87,14 @@ _SIGS = {
     "cmbl_weights_from_mask": ([ctypes.c_long, _vp, ctypes.c_double, _vp], ctypes.c_int),
     "cmbl_pix2ang_ring": ([ctypes.c_int, ctypes.c_long, _dp, _dp], ctypes.c_int),
     "cmbl_whitenoise": ([ctypes.c_int, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
+    "cmbl_fits_map_info": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                            ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "cmbl_fits_read_map": ([ctypes.c_char_p, ctypes.c_int, _vp, ctypes.c_long], ctypes.c_int),
+    "cmbl_fits_write_map": ([ctypes.c_char_p, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "cmbl_nest2ring": ([ctypes.c_int, ctypes.c_long, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+    "cmbl_ring2nest": ([ctypes.c_int, ctypes.c_long, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+    "cmbl_reorder": ([ctypes.c_int, _vp, _vp, ctypes.c_int], ctypes.c_int),
+    "cmbl_reorder_device": ([ctypes.c_int, _vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
     "cmbl_device_alloc": ([ctypes.POINTER(_vp), ctypes.c_ulong], ctypes.c_int),
     "cmbl_device_free": ([_vp], ctypes.c_int),
     "cmbl_copy_to_device": ([_vp, _vp, ctypes.c_ulong], ctypes.c_int),
@@ -213,6 +221,45 @@ def cl_shape(lmax, amp=1e3):
 
 
 # ------------------------------------------------------------------ map context
+
+# ------------------------------------------------------------------ map files
+def read_map(path, column=1):
+    """(map, nside, nested) from a HEALPix FITS file, in the file's ordering
+    (read_Healpix_map_from_fits)."""
+    nside, nested, npix, ncol = ctypes.c_int(), ctypes.c_int(), ctypes.c_long(), ctypes.c_int()
+    b = os.fsencode(path)
+    _check(lib().cmbl_fits_map_info(b, ctypes.byref(nside), ctypes.byref(nested), ctypes.byref(npix),
+                                    ctypes.byref(ncol)))
+    out = np.zeros(npix.value)
+    _check(lib().cmbl_fits_read_map(b, column, _p(out), npix.value))
+    return out, nside.value, bool(nested.value)
+
+
+def write_map(path, m, nside, nested=False, overwrite=True):
+    """Write a HEALPix FITS map (write_Healpix_map_to_fits), float64."""
+    _check(lib().cmbl_fits_write_map(os.fsencode(path), _p(_arr(m, 12 * nside * nside)), nside, int(nested),
+                                     int(overwrite)))
+
+
+def nest2ring(nside, ip):
+    out = ctypes.c_long()
+    _check(lib().cmbl_nest2ring(nside, int(ip), ctypes.byref(out)))
+    return out.value
+
+
+def ring2nest(nside, ip):
+    out = ctypes.c_long()
+    _check(lib().cmbl_ring2nest(nside, int(ip), ctypes.byref(out)))
+    return out.value
+
+
+def reorder(m, nside, to_ring):
+    """NESTED -> RING (to_ring) or RING -> NESTED on the GPU."""
+    m = _arr(m, 12 * nside * nside)
+    out = np.zeros_like(m)
+    _check(lib().cmbl_reorder(nside, _p(m), _p(out), int(bool(to_ring))))
+    return out
+
 
 class Map:
     """One HEALPix map with its likelihood tables on the current GPU (cmbl_map)."""

@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libcosmopp_amd.so")
 TEST_BIN = os.path.join(HERE, "cmbl_cpp_tests")
 
 HIP_SOURCES = ["cmbl_kernels.hip", "cmbl_map.hip"]
-CXX_SOURCES = ["cmbl_likelihood.cpp"]
+CXX_SOURCES = ["cmbl_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CMBL_OFFLOAD_ARCH", "gfx950")

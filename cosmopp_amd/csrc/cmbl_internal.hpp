@@ -26,6 +26,117 @@ namespace cmbl {
 
 __host__ __device__ inline long moff(int L, int m) { return (long)m * (L + 1) - (long)m * (m - 1) / 2; }
 
+// ---------------------------------------------------------------- HEALPix NEST <-> RING
+// The 12 base pixels (faces) of the HEALPix sphere (Gorski et al. 2005): face
+// f spans rings around jrll[f] * nside and starts at longitude index
+// jpll[f] (in units of pi/4).  Inside a face, NEST numbers pixel (ix, iy) by
+// interleaving the bits of ix (even positions) and iy (odd positions).
+__host__ __device__ inline int face_ring(int f) { return 2 + f / 4; }                            // jrll
+__host__ __device__ inline int face_phi(int f) { return f < 4 || f >= 8 ? 2 * (f & 3) + 1 : 2 * (f & 3); }  // jpll
+
+// Morton bit spreading: bit b of x moves to bit 2b (x < 2^32).
+__host__ __device__ inline unsigned long spread_bits(unsigned long x)
+{
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFUL;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFUL;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0FUL;
+    x = (x | (x << 2)) & 0x3333333333333333UL;
+    x = (x | (x << 1)) & 0x5555555555555555UL;
+    return x;
+}
+__host__ __device__ inline long interleave_bits(long x, long y)
+{
+    return (long)(spread_bits((unsigned long)x) | (spread_bits((unsigned long)y) << 1));
+}
+// Inverse: the even-position bits of v, packed.
+__host__ __device__ inline long even_bits(long v)
+{
+    unsigned long x = (unsigned long)v & 0x5555555555555555UL;
+    x = (x | (x >> 1)) & 0x3333333333333333UL;
+    x = (x | (x >> 2)) & 0x0F0F0F0F0F0F0F0FUL;
+    x = (x | (x >> 4)) & 0x00FF00FF00FF00FFUL;
+    x = (x | (x >> 8)) & 0x0000FFFF0000FFFFUL;
+    x = (x | (x >> 16)) & 0x00000000FFFFFFFFUL;
+    return (long)x;
+}
+__host__ __device__ inline long isqrt_hd(long v)
+{
+    long r = (long)sqrt((double)v);
+    while (r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return r;
+}
+
+// NEST index -> RING index.  Pixel (ix, iy) of face f lies on ring
+// jr = jrll nside - ix - iy - 1 (counted from the north pole, 1 .. 4 nside - 1)
+// at position jp = (jpll nr + ix - iy + 1 + shift) / 2 along that ring of nr
+// pixels per quarter.
+__host__ __device__ inline long nest_to_ring(long nside, long ipnest)
+{
+    const long fpix = nside * nside, npix = 12 * fpix, ncap = 2 * nside * (nside - 1);
+    const int f = (int)(ipnest / fpix);
+    const long in = ipnest - f * fpix;
+    const long ix = even_bits(in), iy = even_bits(in >> 1);
+    const long jr = face_ring(f) * nside - ix - iy - 1;
+    long nr, first, shift;
+    if (jr < nside) {
+        nr = jr;
+        first = 2 * nr * (nr - 1);
+        shift = 0;
+    } else if (jr > 3 * nside) {
+        nr = 4 * nside - jr;
+        first = npix - 2 * nr * (nr + 1);
+        shift = 0;
+    } else {
+        nr = nside;
+        first = ncap + (jr - nside) * 4 * nside;
+        shift = (jr - nside) & 1;
+    }
+    long jp = (face_phi(f) * nr + ix - iy + 1 + shift) / 2;
+    if (jp > 4 * nr) jp -= 4 * nr;
+    if (jp < 1) jp += 4 * nr;
+    return first + jp - 1;
+}
+
+// RING index -> NEST index: ring number and position give the face and, by
+// inverting the two relations above, (ix, iy).
+__host__ __device__ inline long ring_to_nest(long nside, long ipring)
+{
+    const long fpix = nside * nside, npix = 12 * fpix, ncap = 2 * nside * (nside - 1);
+    long jr, jp, nr, shift;
+    int f;
+    if (ipring < ncap) {
+        jr = (1 + isqrt_hd(1 + 2 * ipring)) >> 1;
+        jp = ipring - 2 * jr * (jr - 1) + 1;
+        nr = jr;
+        shift = 0;
+        f = (int)((jp - 1) / nr);
+    } else if (ipring < npix - ncap) {
+        const long ip = ipring - ncap;
+        jr = ip / (4 * nside) + nside;
+        jp = ip % (4 * nside) + 1;
+        nr = nside;
+        shift = (jr - nside) & 1;
+        // the two diagonals through the pixel name the face
+        const long ire = jr - nside + 1, irm = 2 * nside + 2 - ire;
+        const long ifm = (jp - ire / 2 + nside - 1) / nside, ifp = (jp - irm / 2 + nside - 1) / nside;
+        f = (int)(ifp == ifm ? (ifp | 4) : (ifp < ifm ? ifp : ifm + 8));
+    } else {
+        const long ip = npix - ipring;
+        const long r = (1 + isqrt_hd(2 * ip - 1)) >> 1;
+        jp = 4 * r + 1 - (ip - 2 * r * (r - 1));
+        nr = r;
+        shift = 0;
+        jr = 4 * nside - r;
+        f = (int)(8 + (jp - 1) / nr);
+    }
+    const long irt = jr - face_ring(f) * nside + 1;
+    long ipt = 2 * jp - face_phi(f) * nr - shift - 1;
+    if (ipt >= 2 * nside) ipt -= 8 * nside;
+    const long ix = (ipt - irt) >> 1, iy = (-ipt - irt) >> 1;
+    return f * fpix + interleave_bits(ix, iy);
+}
+
 // Legendre scaling: true value = v * 2^(-1000 k).  A lane whose scale count k
 // is > 0 has |true| <= 2^-100 and contributes nothing; it is rescaled when
 // |v| exceeds 2^900.
@@ -133,6 +244,12 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
+#define CMBL_TRY(expr)                \
+    do {                              \
+        int rc_ = (expr);             \
+        if (rc_ != CMBL_OK) return rc_; \
+    } while (0)
+
 #define CMBL_HIP(call)                                  \
     do {                                                \
         hipError_t e_ = (call);                         \
@@ -146,6 +263,7 @@ int launch_synth(int K, const WorkItem* items, int nitems, const MapDesc* d_maps
 int launch_adj(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, hipStream_t s);
 int lm_tiles(int L);  // blocks of k_pre / k_post (prior partial sums)
 int launch_ldiv(double* y, const double* x, const double* dl, int L, hipStream_t s);
+int launch_reorder(const double* in, double* out, long nside, int to_ring, hipStream_t s);
 int launch_ring(int mode, const RingItem* items, int nitems, const MapDesc* d_maps, int lds_bytes,
                 double* d_map_out, const double* d_map_in, const double2* d_tw, hipStream_t s);
 int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, double* d_g, int mode,

@@ -200,6 +200,16 @@ __global__ __launch_bounds__(256) void k_ldiv(double* __restrict__ y, const doub
     }
 }
 
+// NEST <-> RING reordering of a map: out[i] = in[pixel of i in the other
+// scheme].  Writes are coalesced, reads gathered.
+__global__ __launch_bounds__(256) void k_reorder(const double* __restrict__ in, double* __restrict__ out, long nside,
+                                                 int to_ring)
+{
+    const long npix = 12 * nside * nside;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < npix; i += (long)gridDim.x * 256L)
+        out[i] = in[to_ring ? ring_to_nest(nside, i) : nest_to_ring(nside, i)];
+}
+
 // ================================================================== Legendre stages
 //
 // One wavefront (= one workgroup) per work item: degree m of one ring block
@@ -1664,6 +1674,15 @@ int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, doubl
                 double c)
 {
     hipLaunchKernelGGL(k_post, dim3(lm_tiles(md.L)), dim3(256), 0, s, d_md, d_x, d_g, mode, c);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_reorder(const double* in, double* out, long nside, int to_ring, hipStream_t s)
+{
+    const long npix = 12 * nside * nside;
+    const int blocks = (int)std::min<long>((npix + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_reorder, dim3(blocks), dim3(256), 0, s, in, out, nside, to_ring);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }

@@ -125,11 +125,7 @@ static int dupload(cmbl_map* mp, T** p, const std::vector<T>& h)
     return CMBL_OK;
 }
 
-#define CMBL_TRY(expr)                \
-    do {                              \
-        int rc_ = (expr);             \
-        if (rc_ != CMBL_OK) return rc_; \
-    } while (0)
+
 
 // Work lists for a Legendre stage with K rings per lane.  One item per live
 // (m, ring block); lfirst is the first l at which any of its pairs is live.
@@ -1267,6 +1263,35 @@ int cmbl_vec_dot_device(const double* x, const double* y, long n, double* d_out,
     double* w = dot_work();
     if (!w) return fail(CMBL_ERR_ALLOC, "dot workspace");
     return launch_dot(x, y, n, d_out, w, (hipStream_t)s);
+}
+
+int cmbl_reorder_device(int nside, const double* in, double* out, int to_ring, void* stream)
+{
+    if (nside < 1 || (nside & (nside - 1))) return fail(CMBL_ERR_ARG, "nside must be a power of two");
+    if (!in || !out || in == out) return fail(CMBL_ERR_ARG, "null or aliased buffers");
+    return launch_reorder(in, out, nside, to_ring, (hipStream_t)stream);
+}
+
+int cmbl_reorder(int nside, const double* in, double* out, int to_ring)
+{
+    if (nside < 1 || (nside & (nside - 1))) return fail(CMBL_ERR_ARG, "nside must be a power of two");
+    if (!in || !out) return fail(CMBL_ERR_ARG, "null pointer");
+    const size_t bytes = sizeof(double) * 12 * (size_t)nside * nside;
+    double *d_in = nullptr, *d_out = nullptr;
+    CMBL_HIP(hipMalloc(&d_in, bytes));
+    if (hipMalloc(&d_out, bytes) != hipSuccess) {
+        (void)hipFree(d_in);
+        return fail(CMBL_ERR_ALLOC, "hipMalloc failed for the reorder buffers");
+    }
+    int rc = CMBL_OK;
+    if (hipMemcpy(d_in, in, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(CMBL_ERR_HIP, "copy to device failed");
+    if (rc == CMBL_OK) rc = launch_reorder(d_in, d_out, nside, to_ring, nullptr);
+    if (rc == CMBL_OK && hipMemcpy(out, d_out, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(CMBL_ERR_HIP, "copy to host failed");
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    return rc;
 }
 
 int cmbl_vec_dot(const double* x, const double* y, long n, double* out, void* s)

@@ -170,6 +170,25 @@ int cmbl_pix2ang_ring(int nside, long pix, double* theta, double* phi);
  * amplitude * GaussianGenerator(seed', 0, 1), into a host array. */
 int cmbl_whitenoise(int seed, double amplitude, long n, double* out);
 
+/* ---------------------------------------------------------------- map files, NEST/RING */
+/* HEALPix FITS maps as read and written by HEALPix read_Healpix_map_from_fits /
+ * write_Healpix_map_to_fits, which the reference uses for its maps, noise
+ * maps and masks (source/cmb_gibbs.cpp:56-58, source/utils.cpp:26-52): a
+ * BINTABLE extension with PIXTYPE 'HEALPIX', ORDERING, NSIDE and implicit
+ * pixel indexing; columns of float64/float32/int types (TSCAL/TZERO applied).
+ * Values are returned in the file's ordering (see *nested). */
+int cmbl_fits_map_info(const char* path, int* nside, int* nested, long* npix, int* ncolumns);
+int cmbl_fits_read_map(const char* path, int column, double* map, long npix); /* column from 1 */
+int cmbl_fits_write_map(const char* path, const double* map, int nside, int nested, int overwrite);
+/* HEALPix pixel index conversions (nest2ring / ring2nest of Healpix_Base,
+ * as Healpix_Map::Import uses them to change a map's ordering). */
+int cmbl_nest2ring(int nside, long ipnest, long* ipring);
+int cmbl_ring2nest(int nside, long ipring, long* ipnest);
+/* Reorder a whole map on the GPU: to_ring != 0 maps NESTED -> RING, else
+ * RING -> NESTED.  in and out must not alias. */
+int cmbl_reorder(int nside, const double* in, double* out, int to_ring);
+int cmbl_reorder_device(int nside, const double* d_in, double* d_out, int to_ring, void* stream);
+
 /* ---------------------------------------------------------------- device memory */
 /* Plain device buffers for callers that hold no HIP headers (the C++ wrapper
  * include/cmbl_likelihood.hpp).  Copies block until complete. */

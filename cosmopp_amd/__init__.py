@@ -87,6 +87,18 @@ _SIGS = {
     "cmbl_weights_from_mask": ([ctypes.c_long, _vp, ctypes.c_double, _vp], ctypes.c_int),
     "cmbl_pix2ang_ring": ([ctypes.c_int, ctypes.c_long, _dp, _dp], ctypes.c_int),
     "cmbl_whitenoise": ([ctypes.c_int, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
+    "cmbl_map2alm_quadrature": ([_vp, _vp, _vp], ctypes.c_int),
+    "cmbl_map2alm_quadrature_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_pseudo_cl": ([_vp, ctypes.c_int, _vp], ctypes.c_int),
+    "cmbl_coupling_matrix": ([_vp, ctypes.c_int, _vp], ctypes.c_int),
+    "cmbl_coupling_matrix_device": ([_vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_master_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int],
+                           ctypes.c_int),
+    "cmbl_master_destroy": ([_vp], None),
+    "cmbl_master_nbins": ([_vp], ctypes.c_int),
+    "cmbl_master_mask_spectrum": ([_vp, _vp], ctypes.c_int),
+    "cmbl_master_coupling": ([_vp, _vp], ctypes.c_int),
+    "cmbl_master_calculate": ([_vp, _vp, _vp, _vp], ctypes.c_int),
     "cmbl_fits_map_info": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                             ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "cmbl_fits_read_map": ([ctypes.c_char_p, ctypes.c_int, _vp, ctypes.c_long], ctypes.c_int),
@@ -322,6 +334,13 @@ class Map:
     def forward_device(self, x_ptr, map_ptr, stream=0):
         _check(lib().cmbl_forward_device(self._h, x_ptr, map_ptr, stream))
 
+    def map2alm_quadrature(self, m):
+        """HEALPix map2alm with ring weight 1: packed complex a_lm, no beam."""
+        m = _arr(m, self.npix)
+        out = np.zeros(self.nparams)
+        _check(lib().cmbl_map2alm_quadrature(self._h, _p(m), _p(out)))
+        return out
+
     def backward(self, v):
         v = _arr(v, self.npix)
         out = np.zeros(self.nparams)
@@ -372,6 +391,65 @@ class Batch:
         gs = (_vp * n)(*g_ptrs)
         _check(lib().cmbl_batch_value_and_grad_device(self._h, ctypes.cast(xs, _vp), f_ptr,
                                                       ctypes.cast(gs, _vp), stream))
+
+
+def pseudo_cl(alm, lmax):
+    """ps(alm, l) (source/master.cpp:62-73) of a packed complex a_lm vector."""
+    out = np.zeros(lmax + 1)
+    _check(lib().cmbl_pseudo_cl(_p(_arr(alm, (lmax + 1) ** 2)), lmax, _p(out)))
+    return out
+
+
+def coupling_matrix(w, lmax):
+    """MASTER mode-coupling matrix from the mask pseudo spectrum w (GPU)."""
+    M = np.zeros((lmax + 1, lmax + 1))
+    _check(lib().cmbl_coupling_matrix(_p(_arr(w, lmax + 1)), lmax, _p(M)))
+    return M
+
+
+class Master:
+    """MASTER estimator (class Master, source/master.cpp): mask and beam fixed,
+    optional bin edges; calculate(map) -> (spectrum per bin or per l, pseudo spectrum)."""
+
+    def __init__(self, nside, mask, beam, bins=None, lmax=0):
+        h = _vp()
+        edges = None if bins is None else np.ascontiguousarray(bins, dtype=np.int32)
+        L = int(edges[-1]) - 1 if edges is not None else lmax
+        self.lmax = L
+        _check(lib().cmbl_master_create(ctypes.byref(h), nside, 0 if edges is not None else lmax,
+                                        _p(_arr(mask, 12 * nside * nside)), _p(_arr(beam)[:L + 1].copy()),
+                                        None if edges is None else edges.ctypes.data_as(_vp),
+                                        0 if edges is None else len(edges)))
+        self._h = h
+        self.nbins = lib().cmbl_master_nbins(h)
+        self.npix = 12 * nside * nside
+
+    def close(self):
+        if self._h:
+            lib().cmbl_master_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def mask_spectrum(self):
+        w = np.zeros(self.lmax + 1)
+        _check(lib().cmbl_master_mask_spectrum(self._h, _p(w)))
+        return w
+
+    def coupling(self):
+        M = np.zeros((self.lmax + 1, self.lmax + 1))
+        _check(lib().cmbl_master_coupling(self._h, _p(M)))
+        return M
+
+    def calculate(self, m):
+        ps = np.zeros(self.nbins)
+        pseudo = np.zeros(self.lmax + 1)
+        _check(lib().cmbl_master_calculate(self._h, _p(_arr(m, self.npix)), _p(ps), _p(pseudo)))
+        return ps, pseudo
 
 
 class Gibbs:

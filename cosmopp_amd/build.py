@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libcosmopp_amd.so")
 TEST_BIN = os.path.join(HERE, "cmbl_cpp_tests")
 
 HIP_SOURCES = ["cmbl_kernels.hip", "cmbl_map.hip"]
-CXX_SOURCES = ["cmbl_io.cpp"]
+CXX_SOURCES = ["cmbl_io.cpp", "cmbl_master.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CMBL_OFFLOAD_ARCH", "gfx950")
@@ -50,8 +50,7 @@ def build(verbose=False):
         if not os.path.exists(path):
             continue
         obj = os.path.join(obj_dir, src + ".o")
-        _run([HIPCC, "-O3", "-std=c++17", "-fPIC", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
-              "-c", path, "-o", obj])
+        _run([HIPCC] + HIP_FLAGS + ["-c", path, "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)

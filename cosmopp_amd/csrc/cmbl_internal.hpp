@@ -237,7 +237,8 @@ enum RingMode { kRingLike = 0, kRingSyn = 1, kRingAna = 2, kRingWeight = 3 };
 //   kPostGibbs   y = x + u                 (I + D Y^T N^-1 Y D, HEALPix complex convention)
 //   kPostAlm     y = u                     (packed D Y^T v, complex convention)
 //   kPostAddAlm  y += c u / B_l            (packed c Y^T v without the l-scaling)
-enum PostMode { kPostGrad = 0, kPostLike = 1, kPostGibbs = 2, kPostAlm = 3, kPostAddAlm = 4 };
+//   kPostQuad    y = c u / B_l             (quadrature analysis, c = 4 pi / Npix)
+enum PostMode { kPostGrad = 0, kPostLike = 1, kPostGibbs = 2, kPostAlm = 3, kPostAddAlm = 4, kPostQuad = 5 };
 
 // Thread-local error string.
 void set_error(const std::string& msg);
@@ -264,6 +265,7 @@ int launch_adj(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, 
 int lm_tiles(int L);  // blocks of k_pre / k_post (prior partial sums)
 int launch_ldiv(double* y, const double* x, const double* dl, int L, hipStream_t s);
 int launch_reorder(const double* in, double* out, long nside, int to_ring, hipStream_t s);
+int launch_coupling(const double* w, int L, double* s0, double* M, hipStream_t s);
 int launch_ring(int mode, const RingItem* items, int nitems, const MapDesc* d_maps, int lds_bytes,
                 double* d_map_out, const double* d_map_in, const double2* d_tw, hipStream_t s);
 int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, double* d_g, int mode,

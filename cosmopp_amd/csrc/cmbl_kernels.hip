@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void k_post(const MapDesc* __restrict__ md_, c
                 ur += v.x;
                 ui += v.y;
             }
-            const double sc = md.Nrm[k] * (mode == kPostAddAlm ? c : md.beam[l]);
+            const double sc = md.Nrm[k] * (mode == kPostAddAlm || mode == kPostQuad ? c : md.beam[l]);
             ur *= sc;
             ui *= sc;
         }
@@ -180,7 +180,8 @@ __global__ __launch_bounds__(256) void k_post(const MapDesc* __restrict__ md_, c
             g[pi] = (m == 0) ? -2.0 * u + 2.0 * md.invcl[l] * x[pi] : -4.0 * u + 4.0 * md.invcl[l] * x[pi];
             break;
         case kPostGibbs: g[pi] = x[pi] + u; break;
-        case kPostAlm: g[pi] = u; break;
+        case kPostAlm:
+        case kPostQuad: g[pi] = u; break;
         default: g[pi] += u; break;  // kPostAddAlm
         }
     }
@@ -197,6 +198,56 @@ __global__ __launch_bounds__(256) void k_ldiv(double* __restrict__ y, const doub
         while (l * l > p) --l;
         while ((l + 1) * (l + 1) <= p) ++l;
         y[p] = x[p] / dl[l];
+    }
+}
+
+// ================================================================== MASTER coupling
+// Mode-coupling matrix of a mask with pseudo spectrum W (source/master.cpp:180-215):
+//   M[l1][l2] = (2 l2 + 1) / (4 pi) sum_{l3 <= L} (2 l3 + 1) W_l3 (l1 l2 l3; 0 0 0)^2.
+// Only squares of the 3j symbols enter, and with g = (l1 + l2 + l3) / 2,
+// a = g - l1, b = g - l2, c = g - l3 their closed form gives two rational
+// recursions (no square roots):
+//   (l1 l2 l1-l2)^2:  1 / (2 l1 + 1) at l2 = 0, times (2c+1) b / ((c+1)(2b-1)) per l2 step;
+//   l3 -> l3 + 2:     times (2a+1)(2b+1) c (g+1) / ((a+1)(b+1)(2c-1)(2g+3)).
+// k_w3j_start tabulates the first, one thread per l1 (s0[l1 (L+1) + l2], l2 <= l1).
+__global__ __launch_bounds__(256) void k_w3j_start(int L, double* __restrict__ s0)
+{
+    const int l1 = blockIdx.x * 256 + threadIdx.x;
+    if (l1 > L) return;
+    double s = 1.0 / (2.0 * l1 + 1.0);
+    double* row = s0 + (long)l1 * (L + 1);
+    row[0] = s;
+    for (int k = 0; k < l1; ++k) {
+        const double b = l1 - k, c = k;
+        s *= (2.0 * c + 1.0) * b / ((c + 1.0) * (2.0 * b - 1.0));
+        row[k + 1] = s;
+    }
+}
+
+// One thread per pair l2 <= l1 (consecutive threads: consecutive l2, so the
+// l3 loops of a wavefront have similar lengths); writes M[l1][l2] and M[l2][l1].
+__global__ __launch_bounds__(256) void k_coupling(int L, const double* __restrict__ W, const double* __restrict__ s0,
+                                                  double* __restrict__ M)
+{
+    const long np = (long)(L + 1) * (L + 2) / 2;
+    for (long p = blockIdx.x * 256L + threadIdx.x; p < np; p += (long)gridDim.x * 256L) {
+        long l1 = (long)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+        while (l1 * (l1 + 1) / 2 > p) --l1;
+        while ((l1 + 1) * (l1 + 2) / 2 <= p) ++l1;
+        const int i1 = (int)l1, i2 = (int)(p - l1 * (l1 + 1) / 2);
+        double s = s0[l1 * (L + 1) + i2];
+        double acc = 0.0;
+        const int top = min(i1 + i2, L);
+        for (int l3 = i1 - i2; l3 <= top; l3 += 2) {
+            acc = fma((2.0 * l3 + 1.0) * W[l3], s, acc);
+            const int g = (i1 + i2 + l3) >> 1;
+            const double a = g - i1, b = g - i2, c = g - l3;
+            s *= (2.0 * a + 1.0) * (2.0 * b + 1.0) * c * (g + 1.0) /
+                 ((a + 1.0) * (b + 1.0) * (2.0 * c - 1.0) * (2.0 * g + 3.0));
+        }
+        const double inv4pi = 0.25 / 3.141592653589793;
+        M[l1 * (L + 1) + i2] = (2.0 * i2 + 1.0) * inv4pi * acc;
+        M[(long)i2 * (L + 1) + l1] = (2.0 * i1 + 1.0) * inv4pi * acc;
     }
 }
 
@@ -1674,6 +1725,17 @@ int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, doubl
                 double c)
 {
     hipLaunchKernelGGL(k_post, dim3(lm_tiles(md.L)), dim3(256), 0, s, d_md, d_x, d_g, mode, c);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_coupling(const double* w, int L, double* s0, double* M, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_w3j_start, dim3((L + 256) / 256), dim3(256), 0, s, L, s0);
+    CMBL_HIP(hipGetLastError());
+    const long np = (long)(L + 1) * (L + 2) / 2;
+    const int blocks = (int)std::min<long>((np + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_coupling, dim3(blocks), dim3(256), 0, s, L, w, s0, M);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }

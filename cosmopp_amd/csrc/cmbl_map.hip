@@ -760,6 +760,28 @@ int cmbl_backward(cmbl_map* mp, const double* v, double* g)
     return CMBL_OK;
 }
 
+// ---------------------------------------------------------------- quadrature analysis
+int cmbl_map2alm_quadrature_device(cmbl_map* mp, const double* map, double* alm, void* stream)
+{
+    CMBL_TRY(check_map(mp));
+    if (!map || !alm) return fail(CMBL_ERR_ARG, "null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    CMBL_TRY(run_ring(kRingAna, mp->d_ring_items, mp->ring_classes, mp->d_desc, nullptr, map, mp->d_tw, s));
+    CMBL_TRY(launch_adj(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, s));
+    CMBL_TRY(launch_post(mp->desc, mp->d_desc, nullptr, alm, kPostQuad, s, 4.0 * 3.141592653589793 / (double)mp->npix));
+    return CMBL_OK;
+}
+
+int cmbl_map2alm_quadrature(cmbl_map* mp, const double* map, double* alm)
+{
+    CMBL_TRY(check_map(mp));
+    if (!map || !alm) return fail(CMBL_ERR_ARG, "null pointer");
+    CMBL_HIP(hipMemcpy(mp->d_map, map, sizeof(double) * mp->npix, hipMemcpyHostToDevice));
+    CMBL_TRY(cmbl_map2alm_quadrature_device(mp, mp->d_map, mp->d_g, nullptr));
+    CMBL_HIP(hipMemcpy(alm, mp->d_g, sizeof(double) * mp->nparam, hipMemcpyDeviceToHost));
+    return CMBL_OK;
+}
+
 // ---------------------------------------------------------------- Gibbs CG
 void cmbl_gibbs_destroy(cmbl_gibbs* gb)
 {

@@ -170,6 +170,37 @@ int cmbl_pix2ang_ring(int nside, long pix, double* theta, double* phi);
  * amplitude * GaussianGenerator(seed', 0, 1), into a host array. */
 int cmbl_whitenoise(int seed, double amplitude, long n, double* out);
 
+/* ---------------------------------------------------------------- quadrature analysis, MASTER */
+/* HEALPix map2alm with ring weight 1 (quadrature weight 4pi/Npix per pixel, no
+ * iteration), as Master uses it for the mask and the masked map
+ * (source/master.cpp:112-117, 143-147): packed complex a_lm (almToVector
+ * layout), no beam. */
+int cmbl_map2alm_quadrature(cmbl_map* map, const double* m, double* alm);
+int cmbl_map2alm_quadrature_device(cmbl_map* map, const double* d_m, double* d_alm, void* stream);
+/* ps(alm, l) = (|a_l0|^2 + 2 sum_m |a_lm|^2) / (2l+1) (source/master.cpp:62-73), l = 0..lmax. */
+int cmbl_pseudo_cl(const double* alm, int lmax, double* cl);
+/* Mode-coupling matrix of Master::calculateCouplingKernel (source/master.cpp:180-249):
+ * M[l1][l2] = (2 l2 + 1)/(4pi) sum_{l3 <= lmax} (2 l3 + 1) W_l3 (l1 l2 l3; 0 0 0)^2,
+ * row-major (lmax+1)^2, from the mask pseudo spectrum W (lmax+1 values).  The
+ * device variant needs a workspace of (lmax+1)^2 doubles. */
+int cmbl_coupling_matrix(const double* w, int lmax, double* M);
+int cmbl_coupling_matrix_device(const double* d_w, int lmax, double* d_work, double* d_M, void* stream);
+
+/* MASTER estimator (class Master, source/master.cpp:28-155, 353-471): mask and
+ * beam fixed at creation (RING order); bins = nedges increasing l edges (bin b
+ * covers [bins[b], bins[b+1]), lmax = bins[nedges-1] - 1, pass lmax 0) or
+ * nedges = 0 for every l up to lmax.  The K matrix is factorised once. */
+typedef struct cmbl_master cmbl_master;
+int cmbl_master_create(cmbl_master** out, int nside, int lmax, const double* mask, const double* beam,
+                       const int* bins, int nedges);
+void cmbl_master_destroy(cmbl_master* master);
+int cmbl_master_nbins(const cmbl_master* master);
+int cmbl_master_mask_spectrum(const cmbl_master* master, double* w);   /* W_l, lmax+1 values */
+int cmbl_master_coupling(const cmbl_master* master, double* M);        /* (lmax+1)^2 */
+/* Master::calculate(map): ps[b] per bin (or per l), pseudo (may be NULL) the
+ * masked map's pseudo spectrum, lmax+1 values. */
+int cmbl_master_calculate(cmbl_master* master, const double* map, double* ps, double* pseudo);
+
 /* ---------------------------------------------------------------- map files, NEST/RING */
 /* HEALPix FITS maps as read and written by HEALPix read_Healpix_map_from_fits /
  * write_Healpix_map_to_fits, which the reference uses for its maps, noise

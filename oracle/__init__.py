@@ -214,6 +214,80 @@ def gibbs_rhs(nside, L, d, omega0, omega1, cl, beam, mask, sigma):
     return b
 
 
+# ---------------------------------------------------------------- MASTER (source/master.cpp)
+
+def quadrature(nside, L, m):
+    """HEALPix map2alm with ring weight 1: (4 pi / Npix) Y^T_c m, packed complex a_lm."""
+    return 4 * np.pi / (12 * nside * nside) * adjoint(nside, L, m) * m_half(L)
+
+
+def pseudo_cl(alm, L):
+    """ps(alm, l) (source/master.cpp:62-73): (|a_l0|^2 + 2 sum_m |a_lm|^2) / (2l + 1)."""
+    a = np.asarray(alm, dtype=np.float64)
+    out = np.zeros(L + 1)
+    for l in range(L + 1):
+        seg = a[l * l:(l + 1) * (l + 1)]
+        out[l] = (seg[0] ** 2 + 2 * np.sum(seg[1:] ** 2)) / (2 * l + 1)
+    return out
+
+
+def w3j2_table(L):
+    """(l1 l2 l3; 0 0 0)^2, dense (L+1)^3, from the closed form
+    (2g-2l1)!(2g-2l2)!(2g-2l3)!/(2g+1)! [g!/((g-l1)!(g-l2)!(g-l3)!)]^2, 2g = l1+l2+l3 even."""
+    from math import lgamma
+    l = np.arange(L + 1)
+    l1, l2, l3 = np.meshgrid(l, l, l, indexing="ij")
+    J = l1 + l2 + l3
+    ok = (J % 2 == 0) & (l3 >= np.abs(l1 - l2)) & (l3 <= l1 + l2)
+    g = J // 2
+    lg = np.vectorize(lgamma, otypes=[np.float64])
+    out = np.zeros(l1.shape)
+    gg, a, b, c = g[ok], (g - l1)[ok], (g - l2)[ok], (g - l3)[ok]
+    val = (lg(2 * a + 1) + lg(2 * b + 1) + lg(2 * c + 1) - lg(2 * gg + 2)
+           + 2 * (lg(gg + 1) - lg(a + 1) - lg(b + 1) - lg(c + 1)))
+    out[ok] = np.exp(val)
+    return out
+
+
+def coupling(w, L, table=None):
+    """M[l1][l2] = (2 l2 + 1)/(4 pi) sum_{l3 <= L} (2 l3 + 1) W_l3 (l1 l2 l3; 000)^2
+    (Wigner3jTreats::process, source/master.cpp:190-201)."""
+    t = w3j2_table(L) if table is None else table
+    l = np.arange(L + 1)
+    s = np.einsum("abc,c->ab", t, (2 * l + 1) * np.asarray(w, dtype=np.float64)[:L + 1])
+    return s * (2 * l[None, :] + 1) / (4 * np.pi)
+
+
+def lfactor(l):
+    l = np.asarray(l, dtype=np.float64)
+    return np.where(l == 0, 1.0 / (2 * np.pi), l * (l + 1) / (2 * np.pi))
+
+
+def master(nside, mask, beam, m, bins=None, lmax=None):
+    """Master::construct + calculate (source/master.cpp:76-153, 353-436):
+    returns (spectrum per bin or per l, pseudo spectrum, W, M)."""
+    L = bins[-1] - 1 if bins is not None else lmax
+    W = pseudo_cl(quadrature(nside, L, mask), L)
+    M = coupling(W, L)
+    b2 = np.asarray(beam, dtype=np.float64)[:L + 1] ** 2
+    lf = lfactor(np.arange(L + 1))
+    c = pseudo_cl(quadrature(nside, L, np.asarray(m) * np.asarray(mask)), L)
+    if bins is None:
+        K = M * b2[None, :] * lf[:, None] / lf[None, :]
+        v = c * lf
+    else:
+        nb = len(bins) - 1
+        P = np.zeros((nb, L + 1))
+        Q = np.zeros((L + 1, nb))
+        for b in range(nb):
+            ls = np.arange(bins[b], bins[b + 1])
+            P[b, ls] = lf[ls] / (bins[b + 1] - bins[b])
+            Q[ls, b] = 1.0 / lf[ls]
+        K = P @ (M * b2[None, :]) @ Q
+        v = P @ c
+    return np.linalg.solve(K, v), c, W, M
+
+
 def cg_solve(apply, precond, b, precision=1e-7, max_iter=None):
     """Math::ConjugateGradient::solve (include/conjugate_gradient.hpp:45-102):
     x0 = 0, r = b - A x0, z = M^-1 r, p = z; loop until |r|^2 <= precision^2.
@@ -265,6 +339,7 @@ def ref():
         r.ref_gauss_draws.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_long, _dp]
         r.ref_uniform_draws.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_long, _dp]
         r.ref_lambda_table.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
+        r.ref_w3j2_table.argtypes = [ctypes.c_int, _dp]
         r.ref_brute_synthesis.argtypes = [ctypes.c_int, _dp, ctypes.c_void_p, ctypes.c_long, _dp, _dp, _dp]
         _ref = r
     return _ref
@@ -290,6 +365,13 @@ def ref_brute_synthesis(L, x, beam, theta, phi):
     ref().ref_brute_synthesis(L, np.ascontiguousarray(x, dtype=np.float64), _ptr(b), len(theta),
                               theta, phi, out)
     return out
+
+
+def ref_w3j2_table(L):
+    """(l1 l2 l3; 0 0 0)^2 for l's <= L from the reference's Wigner3JZeroM, dense (L+1)^3."""
+    out = np.zeros((L + 1) ** 3)
+    ref().ref_w3j2_table(L, out)
+    return out.reshape(L + 1, L + 1, L + 1)
 
 
 def ref_gauss(seed, mean, sigma, n):

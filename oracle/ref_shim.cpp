@@ -15,6 +15,7 @@
 #include <spherical_harmonics.hpp>
 #include <legendre.hpp>
 #include <random.hpp>
+#include <wigner_3j.hpp>
 
 extern "C" {
 
@@ -90,6 +91,26 @@ void ref_brute_synthesis(int L, const double* x_packed, const double* beam, long
         }
         out[p] = (double)s;
     }
+}
+
+// (l1 l2 l3; 0 0 0)^2 from the reference's Math::Wigner3JZeroM for every
+// l1, l2, l3 <= L, into out[(l1 (L+1) + l2) (L+1) + l3] (zero where the
+// symbol vanishes).
+struct W3jCollect {
+    int L;
+    double* out;
+    void process(int l1, int l2, int l3, double val)
+    {
+        if (l1 > L || l2 > L || l3 > L) return;
+        out[((long)l1 * (L + 1) + l2) * (L + 1) + l3] = val * val;
+    }
+};
+
+void ref_w3j2_table(int L, double* out)
+{
+    W3jCollect c{L, out};
+    Math::Wigner3JZeroM<W3jCollect> w(c);
+    w.calculate(L * 3 / 2 + 1);
 }
 
 }  // extern "C"

@@ -11,6 +11,7 @@ Outputs (small .npz files, data only):
   brute_n8_l16.npz   brute-force map sum_lm a_lm Y_lm(theta_p, phi_p), nside 8, lmax 16
   brute_n16_l40.npz  same, nside 16, lmax 40, with a Gaussian beam
   rng.npz            GaussianGenerator / UniformRealGenerator draws for fixed seeds
+  w3j.npz            (l1 l2 l3; 0 0 0)^2 for l's <= 24 from Math::Wigner3JZeroM
 """
 import os
 import sys
@@ -55,6 +56,7 @@ def main():
         gauss_sigma_seed=np.array([777]), gauss_sigma=oracle.ref_gauss(777, 0.0, 20.0, 1000),
         uni_seed=np.array([1000000]), uni=oracle.ref_uniform(1000000, np.pi / 50, np.pi - np.pi / 50, 200),
     )
+    np.savez_compressed(os.path.join(HERE, "w3j.npz"), lmax=np.array([24]), w3j2=oracle.ref_w3j2_table(24))
     print("fixtures written to", HERE)
 
 

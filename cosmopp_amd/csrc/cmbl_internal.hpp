@@ -148,6 +148,10 @@ constexpr double kScaleLo = 0x1p-100;
 constexpr int kWave = 64;
 constexpr int kRingThreads = 512;
 constexpr int kMaxFFT = 8192;          // largest LDS FFT (complex doubles)
+// LDS bytes of a sequence of n complex doubles in the ring stage's padded
+// layout (one spare slot per 16 values).
+constexpr int lds_bytes(int n) { return (n + n / 16) * 16; }
+constexpr int kMaxLds = lds_bytes(kMaxFFT);  // 136 KiB
 constexpr int kRedChunk = 8;           // l-steps per cross-lane reduction in the adjoint
 constexpr int kRedStride = 17;         // doubles per lane row in the reduction buffer
 constexpr int kChunk = 64;             // l-steps per LDS-staged coefficient chunk

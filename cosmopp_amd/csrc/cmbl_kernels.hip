@@ -1050,10 +1050,17 @@ __global__ __launch_bounds__(64) void k_capture(const WorkItem* __restrict__ ite
 // complex sequence of length n held in LDS:
 //   synthesis: Z_k = G_N,k + i G_S,k  ->  inverse DFT  ->  s_N + i s_S
 //   adjoint:   f_N + i f_S  ->  forward DFT  ->  Y, split into H_N, H_S.
-// Power-of-two n uses a radix-2^2 FFT directly; other n (the polar rings,
-// n = 4i) use Bluestein's chirp-z with a power-of-two length M >= 2n-1.
-// The phase of the first pixel, phi0 = pi/n, enters per Fourier bin:
-// e^{i pi m/n} = e^{i pi k/n} (-1)^j for m = k + j n.
+// Power-of-two n uses the FFT directly; other n (the polar rings, n = 4i)
+// use Bluestein's chirp-z with a power-of-two length M >= 2n-1.  All
+// transforms take and return natural order.  The phase of the first pixel,
+// phi0 = pi/n, enters per Fourier bin: e^{i pi m/n} = e^{i pi k/n} (-1)^j for
+// m = k + j n.
+//
+// LDS layout: element e of a sequence lives at zp(e) = e + e/16, one spare
+// slot per 16 values, so the stride-16 accesses of the FFT passes fall on
+// distinct banks.
+
+__device__ __forceinline__ int zp(int e) { return e + (e >> 4); }
 
 __device__ __forceinline__ double2 tw_get(const double2* __restrict__ tw, int t, int sign)
 {
@@ -1061,124 +1068,169 @@ __device__ __forceinline__ double2 tw_get(const double2* __restrict__ tw, int t,
     if (sign > 0) w.y = -w.y;
     return w;
 }
-// w * (sign i)
-__device__ __forceinline__ double2 mul_si(double2 w, int sign)
-{
-    return sign < 0 ? make_double2(w.y, -w.x) : make_double2(-w.y, w.x);
-}
-__device__ __forceinline__ int brev_m(int k, int logM) { return (int)(__brev((unsigned)k) >> (32 - logM)); }
 
-// Decimation in frequency, natural order in, bit-reversed order out.
-// A pass with inner span q applies the stages of span 2q and q to groups of
-// four points (one twiddle lookup per group); an odd log2(M) ends with one
-// span-1 stage.
-__device__ void fft_dif(double2* z, int M, int logM, int sign, const double2* __restrict__ tw)
+// x e^{SIGN 2 pi i j / 16}, j = 0..7 (cos, sin of pi j / 8 as literals).
+template <int SIGN>
+__device__ __forceinline__ double2 tw16(double2 x, int j)
 {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int q = M >> 2; q >= 1; q >>= 2) {
-        const int ts = kMaxFFT / (4 * q);
-        for (int g = tid; g < (M >> 2); g += nt) {
-            const int k = g & (q - 1);
-            const int p0 = ((g - k) << 2) + k;
-            const double2 w4 = tw_get(tw, k * ts, sign);
-            const double2 w2 = cmul(w4, w4);
-            const double2 w4q = mul_si(w4, sign);
-            const double2 a = z[p0], b = z[p0 + q], c = z[p0 + 2 * q], d = z[p0 + 3 * q];
-            const double2 a1 = cadd(a, c), c1 = cmul(csub(a, c), w4);
-            const double2 b1 = cadd(b, d), d1 = cmul(csub(b, d), w4q);
-            z[p0] = cadd(a1, b1);
-            z[p0 + q] = cmul(csub(a1, b1), w2);
-            z[p0 + 2 * q] = cadd(c1, d1);
-            z[p0 + 3 * q] = cmul(csub(c1, d1), w2);
-        }
-        __syncthreads();
+    constexpr double C1 = 0.9238795325112867, S1 = 0.3826834323650898, R2 = 0.7071067811865476;
+    double c, sn;
+    switch (j) {
+    case 0: return x;
+    case 4: return SIGN > 0 ? make_double2(-x.y, x.x) : make_double2(x.y, -x.x);
+    case 1: c = C1; sn = S1; break;
+    case 2: c = R2; sn = R2; break;
+    case 3: c = S1; sn = C1; break;
+    case 5: c = -S1; sn = C1; break;
+    case 6: c = -R2; sn = R2; break;
+    default: c = -C1; sn = S1; break;
     }
-    if (logM & 1) {
-        for (int b = tid; b < (M >> 1); b += nt) {
-            const double2 u = z[2 * b], v = z[2 * b + 1];
-            z[2 * b] = cadd(u, v);
-            z[2 * b + 1] = csub(u, v);
+    sn *= SIGN;
+    return make_double2(x.x * c - x.y * sn, x.x * sn + x.y * c);
+}
+
+// In-register R-point DFT (R <= 16), radix-2 decimation in frequency:
+// natural order in, u[bitrev(q)] = sum_r u_r e^{SIGN 2 pi i r q / R} out.
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_reg(double2 (&u)[R])
+{
+#pragma unroll
+    for (int s = R / 2; s >= 1; s /= 2) {
+#pragma unroll
+        for (int blk = 0; blk < R; blk += 2 * s) {
+#pragma unroll
+            for (int a = 0; a < s; ++a) {
+                const double2 x0 = u[blk + a], x1 = u[blk + a + s];
+                u[blk + a] = cadd(x0, x1);
+                u[blk + a + s] = tw16<SIGN>(csub(x0, x1), a * (8 / s));
+            }
         }
-        __syncthreads();
     }
 }
 
-// Decimation in time, bit-reversed order in, natural order out.
-__device__ void fft_dit(double2* z, int M, int logM, int sign, const double2* __restrict__ tw)
+template <int LR>
+__device__ __forceinline__ constexpr int brev_c(int q)
 {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    int q = 1;
-    if (logM & 1) {
-        for (int b = tid; b < (M >> 1); b += nt) {
-            const double2 u = z[2 * b], v = z[2 * b + 1];
-            z[2 * b] = cadd(u, v);
-            z[2 * b + 1] = csub(u, v);
-        }
-        __syncthreads();
-        q = 2;
-    }
-    for (; 4 * q <= M; q <<= 2) {
-        const int ts = kMaxFFT / (4 * q);
-        for (int g = tid; g < (M >> 2); g += nt) {
-            const int k = g & (q - 1);
-            const int p0 = ((g - k) << 2) + k;
-            const double2 w4 = tw_get(tw, k * ts, sign);
-            const double2 w2 = cmul(w4, w4);
-            const double2 w4q = mul_si(w4, sign);
-            const double2 a = z[p0];
-            const double2 b = cmul(z[p0 + q], w2);
-            const double2 c = z[p0 + 2 * q];
-            const double2 d = cmul(z[p0 + 3 * q], w2);
-            const double2 a1 = cadd(a, b), b1 = csub(a, b);
-            const double2 c1 = cmul(cadd(c, d), w4), d1 = cmul(csub(c, d), w4q);
-            z[p0] = cadd(a1, c1);
-            z[p0 + 2 * q] = csub(a1, c1);
-            z[p0 + q] = cadd(b1, d1);
-            z[p0 + 3 * q] = csub(b1, d1);
-        }
-        __syncthreads();
-    }
+    int r = 0;
+    for (int b = 0; b < LR; ++b) r |= ((q >> b) & 1) << (LR - 1 - b);
+    return r;
 }
 
-// Bluestein DFT of z[0..n) (natural order, z zero-padded by this call to M):
+// One Stockham pass of radix R over M points with p = product of the radices
+// already applied: butterfly i (k = i mod p) reads x[i + r M/R], twiddles by
+// e^{SIGN 2 pi i r k/(p R)}, transforms, and writes y[(i - k) R + k + q p].
+// Every thread loads all its butterflies before the first barrier and stores
+// after it, so the pass works in place.
+template <int R, int LR, int SIGN, int MAXM>
+__device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, const double2* __restrict__ tw)
+{
+    constexpr int B = (MAXM / R + kRingThreads - 1) / kRingThreads;  // butterflies per thread, at most
+    const int tid = threadIdx.x;
+    const int nb = M / R;
+    double2 u[B][R];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int i = tid + b * kRingThreads;
+        if (i < nb) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) u[b][r] = Z[zp(i + r * nb)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int i = tid + b * kRingThreads;
+        if (i < nb) {
+            const int k = i & (p - 1);
+            if (k) {
+                const double2 w1 = tw_get(tw, k * (kMaxFFT / (p * R)), SIGN);
+                double2 w = w1;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    u[b][r] = cmul(u[b][r], w);
+                    if (r + 1 < R) w = cmul(w, w1);
+                }
+            }
+            dft_reg<R, SIGN>(u[b]);
+            const int j = (i - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) Z[zp(j + q * p)] = u[b][brev_c<LR>(q)];
+        }
+    }
+    __syncthreads();
+}
+
+// Unnormalised DFT of Z[0..M) in place, natural order in and out:
+// Z_q <- sum_r Z_r e^{SIGN 2 pi i r q / M}, M = 2^logM <= MAXM <= kMaxFFT, M >= 2.
+// Radix-16 passes, then one radix-8/4/2 pass for the remaining factor.  MAXM
+// bounds the butterflies per thread and so the registers.
+template <int SIGN, int MAXM>
+__device__ void fft(double2* Z, int M, int logM, const double2* __restrict__ tw)
+{
+    // radix 16 where the class runs one workgroup per CU anyway (LDS-bound),
+    // radix 8 below, whose halved register set lets two workgroups share a CU
+    constexpr int LR = MAXM > 4096 ? 4 : 3;
+    int p = 1, rem = logM;
+    for (; rem >= LR; rem -= LR, p <<= LR) stockham_pass<1 << LR, LR, SIGN, MAXM>(Z, M, p, tw);
+    if (rem == 3)
+        stockham_pass<8, 3, SIGN, MAXM>(Z, M, p, tw);
+    else if (rem == 2)
+        stockham_pass<4, 2, SIGN, MAXM>(Z, M, p, tw);
+    else if (rem == 1)
+        stockham_pass<2, 1, SIGN, MAXM>(Z, M, p, tw);
+}
+
+// Bluestein DFT of z[0..n) (natural order, zero-padded by this call to M):
 // result in natural order in z[0..n).  chirp[j] = e^{-i pi j^2/n};
-// filt = FFT_M of e^{+i pi j^2/n} (wrapped) stored in bit-reversed order.
-__device__ void bluestein(double2* z, int n, int M, int logM, int sign, const double2* __restrict__ chirp,
+// filt = FFT_M of e^{+i pi j^2/n} (wrapped), natural order.  For SIGN > 0 the
+// chirps are conjugated and the filter is read reversed and conjugated.
+template <int SIGN, int MAXM>
+__device__ void bluestein(double2* z, int n, int M, int logM, const double2* __restrict__ chirp,
                           const double2* __restrict__ filt, const double2* __restrict__ tw)
 {
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int j = tid; j < M; j += nt) {
         if (j < n) {
             double2 c = chirp[j];
-            if (sign > 0) c.y = -c.y;
-            z[j] = cmul(z[j], c);
+            if (SIGN > 0) c.y = -c.y;
+            z[zp(j)] = cmul(z[zp(j)], c);
         } else {
-            z[j] = make_double2(0.0, 0.0);
+            z[zp(j)] = make_double2(0.0, 0.0);
         }
     }
     __syncthreads();
-    fft_dif(z, M, logM, -1, tw);
+    fft<-1, MAXM>(z, M, logM, tw);
     for (int r = tid; r < M; r += nt) {
         double2 f;
-        if (sign < 0) {
+        if (SIGN < 0) {
             f = filt[r];
         } else {
-            const int k = brev_m(r, logM);
-            const double2 g = filt[brev_m((M - k) & (M - 1), logM)];
+            const double2 g = filt[(M - r) & (M - 1)];
             f = make_double2(g.x, -g.y);
         }
-        z[r] = cmul(z[r], f);
+        z[zp(r)] = cmul(z[zp(r)], f);
     }
     __syncthreads();
-    fft_dit(z, M, logM, +1, tw);
+    fft<+1, MAXM>(z, M, logM, tw);
     const double inv = 1.0 / (double)M;
     for (int k = tid; k < n; k += nt) {
         double2 c = chirp[k];
-        if (sign > 0) c.y = -c.y;
-        const double2 v = z[k];
-        z[k] = cmul(make_double2(v.x * inv, v.y * inv), c);
+        if (SIGN > 0) c.y = -c.y;
+        const double2 v = z[zp(k)];
+        z[zp(k)] = cmul(make_double2(v.x * inv, v.y * inv), c);
     }
     __syncthreads();
+}
+
+// Length-n DFT in LDS (natural order): the FFT for power-of-two n, Bluestein otherwise.
+template <int SIGN, int MAXM>
+__device__ __forceinline__ void ring_dft(double2* Z, int n, int M, int logM, bool pow2, const double2* chirp,
+                                         const double2* filt, const double2* __restrict__ tw)
+{
+    if (pow2)
+        fft<SIGN, MAXM>(Z, n, logM, tw);
+    else
+        bluestein<SIGN, MAXM>(Z, n, M, logM, chirp, filt, tw);
 }
 
 // Alias sums of Fourier bin k over the parity rows (E, O) of a ring pair:
@@ -1209,9 +1261,9 @@ __device__ __forceinline__ void fold_aliases(const double2* __restrict__ row, in
     }
 }
 
-// Bin k of the packed pair spectrum: phase e^{i pi k/n} for shifted rings,
-// no south ring at the equator, bit-reversed position for the power-of-two FFT.
-__device__ __forceinline__ void fold_store(double2* Z, int k, int n, int logM, bool pow2, bool shifted, bool no_south,
+// Bin k of the packed pair spectrum (natural order): phase e^{i pi k/n} for
+// shifted rings, no south ring at the equator.
+__device__ __forceinline__ void fold_store(double2* Z, int slot, int k, int n, bool shifted, bool no_south,
                                            double2 gN, double2 gS)
 {
     if (no_south) gS = make_double2(0.0, 0.0);
@@ -1220,7 +1272,30 @@ __device__ __forceinline__ void fold_store(double2* Z, int k, int n, int logM, b
         gN = cmul(gN, ph);
         gS = cmul(gS, ph);
     }
-    Z[pow2 ? brev_m(k, logM) : k] = make_double2(gN.x - gS.y, gN.y + gS.x);
+    Z[zp(slot)] = make_double2(gN.x - gS.y, gN.y + gS.x);
+}
+
+// Unfold bin k and its mirror n - k of Y into Q (the aliases m = k + j n,
+// j = r, r + R, ...): Psi_X,m = e^{-i pi k/n} (-1)^j H_X,k.
+__device__ __forceinline__ void unfold_bin(double2* qrow, int L, int n, int k, int r, int R, bool shifted, double2 y,
+                                           double2 z2)
+{
+    const double2 yc = make_double2(z2.x, -z2.y);
+    double2 hN = make_double2(0.5 * (y.x + yc.x), 0.5 * (y.y + yc.y));
+    double2 hS = make_double2(0.5 * (y.y - yc.y), -0.5 * (y.x - yc.x));
+    if (shifted) {
+        double2 ph = cis_pi_frac(k, n);
+        ph.y = -ph.y;
+        hN = cmul(hN, ph);
+        hS = cmul(hS, ph);
+    }
+    const double2 pe = cadd(hN, hS), po = csub(hN, hS);
+    for (int j = r; k + (long)j * n <= L; j += R) {
+        const int mm = k + j * n;
+        const double sg = (shifted && (j & 1)) ? -1.0 : 1.0;
+        qrow[2 * mm] = make_double2(sg * pe.x, sg * pe.y);
+        qrow[2 * mm + 1] = make_double2(sg * po.x, sg * po.y);
+    }
 }
 
 // Y_k of a split ring from its two half transforms: SE (global scratch) and
@@ -1230,7 +1305,7 @@ __device__ __forceinline__ double2 split_bin(const double2* SE, const double2* Z
     double2 w = cis_pi_frac(2 * (long)k % (2L * n), n);
     w.y = -w.y;
     const int kk = k % h;
-    return cadd(SE[kk], cmul(Z[kk], w));
+    return cadd(SE[kk], cmul(Z[zp(kk)], w));
 }
 
 // A ring pair whose length-n Bluestein transform would not fit in LDS (n = 4i
@@ -1242,7 +1317,7 @@ __device__ __forceinline__ double2 split_bin(const double2* SE, const double2* Z
 //   analysis   Y_k = FE_{k mod h} + w^{-k} FO_{k mod h},
 //              FE, FO = DFT_h of the even / odd samples.
 // Scratch per ring: SE[h] (a half transform) and SF[n] (weighted residual).
-template <int MODE>
+template <int MODE, int MAXM>
 __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, bool shifted, long oN, long oS,
                            double* __restrict__ map_out, const double* __restrict__ map_in,
                            const double2* __restrict__ tw, double2* Z, double* sh)
@@ -1262,20 +1337,19 @@ __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, boo
                 const int k = 2 * kk + r;
                 double2 gN, gS;
                 fold_aliases(row, L, n, k, 0, 1, shifted, gN, gS);
-                fold_store(Z, kk, n, 0, false, false, oS < 0, gN, gS);
-                if (shifted) Z[kk] = cmul(Z[kk], cis_pi_frac(k, n));
+                fold_store(Z, kk, k, n, shifted, oS < 0, gN, gS);
             }
             __syncthreads();
-            bluestein(Z, h, M, logM, +1, chirp, filt, tw);
+            bluestein<+1, MAXM>(Z, h, M, logM, chirp, filt, tw);
             if (r == 0) {
-                for (int j = tid; j < h; j += kRingThreads) SE[j] = Z[j];
+                for (int j = tid; j < h; j += kRingThreads) SE[j] = Z[zp(j)];
                 __syncthreads();
             }
         }
         double chi = 0.0;
         for (int j = tid; j < h; j += kRingThreads) {
             const double2 e = SE[j];
-            const double2 o = cmul(Z[j], cis_pi_frac(2 * j, n));
+            const double2 o = cmul(Z[zp(j)], cis_pi_frac(2 * j, n));
             const double2 sv[2] = {cadd(e, o), csub(e, o)};
 #pragma unroll
             for (int half = 0; half < 2; ++half) {
@@ -1312,43 +1386,28 @@ __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, boo
     for (int r = 0; r < 2; ++r) {
         for (int j = tid; j < h; j += kRingThreads) {
             const long t = 2 * j + r;
-            Z[j] = (MODE == kRingAna) ? make_double2(map_in[oN + t], oS >= 0 ? map_in[oS + t] : 0.0) : SF[t];
+            Z[zp(j)] = (MODE == kRingAna) ? make_double2(map_in[oN + t], oS >= 0 ? map_in[oS + t] : 0.0) : SF[t];
         }
         __syncthreads();
-        bluestein(Z, h, M, logM, -1, chirp, filt, tw);
+        bluestein<-1, MAXM>(Z, h, M, logM, chirp, filt, tw);
         if (r == 0) {
-            for (int j = tid; j < h; j += kRingThreads) SE[j] = Z[j];
+            for (int j = tid; j < h; j += kRingThreads) SE[j] = Z[zp(j)];
             __syncthreads();
         }
     }
 
-    // unfold as in the unsplit path (n > kRingThreads, so one thread group)
     double2* qrow = reinterpret_cast<double2*>(md.Q + (long)p * (L + 1) * 4);
     for (int k = tid; k < n && k <= L; k += kRingThreads) {
         const int nk = (k == 0) ? 0 : n - k;
-        const double2 y = split_bin(SE, Z, k, n, h);
-        const double2 z2 = split_bin(SE, Z, nk, n, h);
-        const double2 yc = make_double2(z2.x, -z2.y);
-        double2 hN = make_double2(0.5 * (y.x + yc.x), 0.5 * (y.y + yc.y));
-        double2 hS = make_double2(0.5 * (y.y - yc.y), -0.5 * (y.x - yc.x));
-        if (shifted) {
-            double2 ph = cis_pi_frac(k, n);
-            ph.y = -ph.y;
-            hN = cmul(hN, ph);
-            hS = cmul(hS, ph);
-        }
-        const double2 pe = cadd(hN, hS), po = csub(hN, hS);
-        for (int j = 0; k + (long)j * n <= L; ++j) {
-            const int mm = k + j * n;
-            const double sg = (shifted && (j & 1)) ? -1.0 : 1.0;
-            qrow[2 * mm] = make_double2(sg * pe.x, sg * pe.y);
-            qrow[2 * mm + 1] = make_double2(sg * po.x, sg * po.y);
-        }
+        unfold_bin(qrow, L, n, k, 0, 1, shifted, split_bin(SE, Z, k, n, h), split_bin(SE, Z, nk, n, h));
     }
 }
 
-template <int MODE>
-__global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restrict__ items,
+// Classes up to M = 4096 fit two workgroups per CU in LDS: ask for the
+// registers that allow it (the second bound counts waves per SIMD: two
+// workgroups of kRingThreads over the CU's four SIMDs).
+template <int MODE, int MAXM>
+__global__ __launch_bounds__(kRingThreads, MAXM > 4096 ? 1 : 2 * kRingThreads / 256) void k_ring(const RingItem* __restrict__ items,
                                                        const MapDesc* __restrict__ maps,
                                                        double* __restrict__ map_out,
                                                        const double* __restrict__ map_in,
@@ -1369,55 +1428,51 @@ __global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restric
     const long oN = md.ring_ofsN[p];
     const long oS = md.ring_ofsS[p];
     if (md.ring_scrofs[p] >= 0) {
-        ring_split<MODE>(md, p, n, M, logM, shifted, oN, oS, map_out, map_in, tw, Z, sh);
+        ring_split<MODE, MAXM>(md, p, n, M, logM, shifted, oN, oS, map_out, map_in, tw, Z, sh);
         return;
     }
     const double2* filt = pow2 ? nullptr : md.bs_filt + md.ring_bsofs[p];
     const double2* chirp = pow2 ? nullptr : md.bs_chirp + md.ring_chofs[p];
     const int tid = threadIdx.x;
+    // for n < kRingThreads the aliases of each bin are split over R thread
+    // groups (reads stay coalesced over k), partial sums added in fixed order
+    const int R = n < kRingThreads ? kRingThreads / n : 1;
 
     if (MODE != kRingAna) {
         const double2* row = reinterpret_cast<const double2*>(md.P + (long)p * (L + 1) * 4);
-        // fold: Z_k = G_N,k + i G_S,k (natural order for Bluestein, bit-reversed
-        // for the power-of-two FFT).  For n < kRingThreads the aliases of bin k
-        // are split over R = kRingThreads / n thread groups (reads stay
-        // coalesced over k) and the R partial sums are added in a fixed order.
-        const int R = n < kRingThreads ? kRingThreads / n : 1;
+        // fold: Z_k = G_N,k + i G_S,k
         if (R == 1) {
             for (int k = tid; k < n; k += kRingThreads) {
                 double2 gN, gS;
                 fold_aliases(row, L, n, k, 0, 1, shifted, gN, gS);
-                fold_store(Z, k, n, logM, pow2, shifted, oS < 0, gN, gS);
+                fold_store(Z, k, k, n, shifted, oS < 0, gN, gS);
             }
         } else {
             double2 gN = make_double2(0.0, 0.0), gS = make_double2(0.0, 0.0);
             if (tid < R * n) {
                 fold_aliases(row, L, n, tid % n, tid / n, R, shifted, gN, gS);
-                Z[2 * tid] = gN;
-                Z[2 * tid + 1] = gS;
+                Z[zp(2 * tid)] = gN;
+                Z[zp(2 * tid + 1)] = gS;
             }
             __syncthreads();
             if (tid < n) {
                 gN = make_double2(0.0, 0.0);
                 gS = make_double2(0.0, 0.0);
                 for (int r = 0; r < R; ++r) {
-                    gN = cadd(gN, Z[2 * (r * n + tid)]);
-                    gS = cadd(gS, Z[2 * (r * n + tid) + 1]);
+                    gN = cadd(gN, Z[zp(2 * (r * n + tid))]);
+                    gS = cadd(gS, Z[zp(2 * (r * n + tid) + 1)]);
                 }
             }
             __syncthreads();
-            if (tid < n) fold_store(Z, tid, n, logM, pow2, shifted, oS < 0, gN, gS);
+            if (tid < n) fold_store(Z, tid, tid, n, shifted, oS < 0, gN, gS);
         }
         __syncthreads();
-        if (pow2)
-            fft_dit(Z, n, logM, +1, tw);
-        else
-            bluestein(Z, n, M, logM, +1, chirp, filt, tw);
+        ring_dft<+1, MAXM>(Z, n, M, logM, pow2, chirp, filt, tw);
     }
 
     if (MODE == kRingSyn) {
         for (int j = tid; j < n; j += kRingThreads) {
-            const double2 v = Z[j];
+            const double2 v = Z[zp(j)];
             map_out[oN + j] = v.x;
             if (oS >= 0) map_out[oS + j] = v.y;
         }
@@ -1427,7 +1482,7 @@ __global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restric
     if (MODE == kRingLike) {
         double chi = 0.0;
         for (int j = tid; j < n; j += kRingThreads) {
-            const double2 v = Z[j];
+            const double2 v = Z[zp(j)];
             const double rN = md.d[oN + j] - v.x;
             const double fN = md.w[oN + j] * rN;
             chi = fma(fN, rN, chi);
@@ -1437,59 +1492,36 @@ __global__ __launch_bounds__(kRingThreads) void k_ring(const RingItem* __restric
                 fS = md.w[oS + j] * rS;
                 chi = fma(fS, rS, chi);
             }
-            Z[j] = make_double2(fN, fS);
+            Z[zp(j)] = make_double2(fN, fS);
         }
         const double t = block_sum<kRingThreads>(chi, sh);
         if (tid == 0) md.chi[p] = t;
     } else if (MODE == kRingWeight) {
         for (int j = tid; j < n; j += kRingThreads) {
-            const double2 v = Z[j];
-            Z[j] = make_double2(md.w[oN + j] * v.x, oS >= 0 ? md.w[oS + j] * v.y : 0.0);
+            const double2 v = Z[zp(j)];
+            Z[zp(j)] = make_double2(md.w[oN + j] * v.x, oS >= 0 ? md.w[oS + j] * v.y : 0.0);
         }
         __syncthreads();
     } else {  // kRingAna: load the map to analyse
         for (int j = tid; j < n; j += kRingThreads)
-            Z[j] = make_double2(map_in[oN + j], oS >= 0 ? map_in[oS + j] : 0.0);
+            Z[zp(j)] = make_double2(map_in[oN + j], oS >= 0 ? map_in[oS + j] : 0.0);
         __syncthreads();
     }
-    if (pow2)
-        fft_dif(Z, n, logM, -1, tw);
-    else
-        bluestein(Z, n, M, logM, -1, chirp, filt, tw);
+    ring_dft<-1, MAXM>(Z, n, M, logM, pow2, chirp, filt, tw);
 
-    // unfold: Psi_X,m = e^{-i pi k/n} (-1)^j H_X,k for m = k + j n, with the
-    // aliases of bin k split over R thread groups as in the fold
+    // unfold, with the aliases of bin k split over R thread groups as in the fold
     double2* qrow = reinterpret_cast<double2*>(md.Q + (long)p * (L + 1) * 4);
-    const int R = n < kRingThreads ? kRingThreads / n : 1;
     for (int t = tid; t < R * n; t += kRingThreads) {
         const int k = t % n, r = t / n;
         if (k > L) continue;
         const int nk = (k == 0) ? 0 : n - k;
-        const double2 y = Z[pow2 ? brev_m(k, logM) : k];
-        const double2 z2 = Z[pow2 ? brev_m(nk, logM) : nk];
-        const double2 yc = make_double2(z2.x, -z2.y);
-        double2 hN = make_double2(0.5 * (y.x + yc.x), 0.5 * (y.y + yc.y));
-        double2 hS = make_double2(0.5 * (y.y - yc.y), -0.5 * (y.x - yc.x));
-        if (shifted) {
-            double2 ph = cis_pi_frac(k, n);
-            ph.y = -ph.y;
-            hN = cmul(hN, ph);
-            hS = cmul(hS, ph);
-        }
-        const double2 pe = cadd(hN, hS), po = csub(hN, hS);
-        for (int j = r; k + (long)j * n <= L; j += R) {
-            const int mm = k + j * n;
-            const double sg = (shifted && (j & 1)) ? -1.0 : 1.0;
-            qrow[2 * mm] = make_double2(sg * pe.x, sg * pe.y);
-            qrow[2 * mm + 1] = make_double2(sg * po.x, sg * po.y);
-        }
+        unfold_bin(qrow, L, n, k, r, R, shifted, Z[zp(k)], Z[zp(nk)]);
     }
 }
 
 // Bluestein set-up for one ring pair of Bluestein length n (nph, or nph/2 for
 // a split ring): chirp[j] = e^{-i pi j^2/n} (j < n) and filt = FFT_M of
-// b_j = e^{+i pi j^2/n} for |j| < n wrapped mod M, stored in the bit-reversed
-// order of the DIF transform.
+// b_j = e^{+i pi j^2/n} for |j| < n wrapped mod M, natural order.
 __global__ __launch_bounds__(kRingThreads) void k_bluestein_setup(const int* __restrict__ pairs,
                                                                   const MapDesc* __restrict__ md_,
                                                                   double2* __restrict__ filt_out,
@@ -1509,7 +1541,7 @@ __global__ __launch_bounds__(kRingThreads) void k_bluestein_setup(const int* __r
             jj = j;
         else if (j > M - n)
             jj = M - j;
-        Z[j] = jj >= 0 ? cis_pi_frac(((long)jj * jj) % (2L * n), n) : make_double2(0.0, 0.0);
+        Z[zp(j)] = jj >= 0 ? cis_pi_frac(((long)jj * jj) % (2L * n), n) : make_double2(0.0, 0.0);
     }
     double2* ch = chirp_out + md.ring_chofs[p];
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
@@ -1518,9 +1550,9 @@ __global__ __launch_bounds__(kRingThreads) void k_bluestein_setup(const int* __r
         ch[j] = c;
     }
     __syncthreads();
-    fft_dif(Z, M, logM, -1, tw);
+    fft<-1, kMaxFFT>(Z, M, logM, tw);
     double2* dst = filt_out + md.ring_bsofs[p];
-    for (int j = threadIdx.x; j < M; j += blockDim.x) dst[j] = Z[j];
+    for (int j = threadIdx.x; j < M; j += blockDim.x) dst[j] = Z[zp(j)];
 }
 
 // ================================================================== k_final
@@ -1677,20 +1709,32 @@ int launch_scan(int K, const WorkItem* items, int n, const MapDesc* d_maps, int*
     return fail(CMBL_ERR_ARG, "unsupported rings-per-lane K");
 }
 
+template <int MODE, int MAXM>
+static int ring_mm(const RingItem* items, int n, const MapDesc* d_maps, int lds, double* out, const double* in,
+                   const double2* tw, hipStream_t s)
+{
+    static bool attr_set = false;
+    if (!attr_set) {
+        CMBL_HIP(hipFuncSetAttribute((const void*)k_ring<MODE, MAXM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     kMaxLds));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_ring<MODE, MAXM>), dim3(n), dim3(kRingThreads), lds, s, items, d_maps, out, in, tw);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+// One instantiation per transform-length class (the classes' LDS sizes):
+// the bound on the transform length bounds the registers of the FFT passes.
 template <int MODE>
 static int ring_m(const RingItem* items, int n, const MapDesc* d_maps, int lds, double* out, const double* in,
                   const double2* tw, hipStream_t s)
 {
     if (n <= 0) return CMBL_OK;
-    static bool attr_set = false;
-    if (!attr_set) {
-        CMBL_HIP(hipFuncSetAttribute((const void*)k_ring<MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     kMaxFFT * (int)sizeof(double2)));
-        attr_set = true;
-    }
-    hipLaunchKernelGGL(k_ring<MODE>, dim3(n), dim3(kRingThreads), lds, s, items, d_maps, out, in, tw);
-    CMBL_HIP(hipGetLastError());
-    return CMBL_OK;
+    if (lds > kMaxLds) return fail(CMBL_ERR_ARG, "ring transform longer than the LDS FFT");
+    if (lds > lds_bytes(4096)) return ring_mm<MODE, 8192>(items, n, d_maps, lds, out, in, tw, s);
+    if (lds > lds_bytes(2048)) return ring_mm<MODE, 4096>(items, n, d_maps, lds, out, in, tw, s);
+    return ring_mm<MODE, 2048>(items, n, d_maps, lds, out, in, tw, s);
 }
 
 int launch_ring(int mode, const RingItem* items, int n, const MapDesc* d_maps, int lds, double* out,
@@ -1712,7 +1756,7 @@ int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_b
     static bool attr_set = false;
     if (!attr_set) {
         CMBL_HIP(hipFuncSetAttribute((const void*)k_bluestein_setup, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     kMaxFFT * (int)sizeof(double2)));
+                                     kMaxLds));
         attr_set = true;
     }
     hipLaunchKernelGGL(k_bluestein_setup, dim3(npairs_bs), dim3(kRingThreads), lds, s, d_pairs, d_md, d_out, d_chirp,

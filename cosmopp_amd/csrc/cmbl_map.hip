@@ -219,7 +219,7 @@ static std::vector<RingClass> ring_classes(std::vector<RingItem>* items, MOf Mof
         const int M = Mof((*items)[i]);
         if (out.empty() || cls(Mof((*items)[out.back().first])) != cls(M))
             // at least 2 * kRingThreads entries: the fold's partial sums live in LDS
-            out.push_back(RingClass{i, 0, std::max(M, 2 * kRingThreads) * (int)sizeof(double2)});
+            out.push_back(RingClass{i, 0, lds_bytes(std::max(M, 2 * kRingThreads))});
         ++out.back().count;
     }
     return out;
@@ -311,7 +311,7 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     if (maxM > kMaxFFT)
         return fail(CMBL_ERR_ARG, "ring transform length " + std::to_string(maxM) + " exceeds the LDS limit " +
                                       std::to_string(kMaxFFT) + " (nside > 2048 is not supported)");
-    mp->lds_ring = maxM * (int)sizeof(double2);
+    mp->lds_ring = lds_bytes(std::max(maxM, 2 * kRingThreads));
 
     // ---- recurrence tables (long double on the host, rounded once)
     std::vector<double> A(mp->nalm, 0.0), Nrm(mp->nalm, 1.0);

@@ -146,6 +146,57 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_pol(args, cfg):
+    """Spin-2 synthesis (HEALPix alm2map_pol: T, E, B -> T, Q, U) on device
+    buffers at the configuration's nside / lmax; one step = one call."""
+    import numpy as np
+    import torch
+
+    import cosmopp_amd as cm
+
+    nside, lmax = cfg["nside"], cfg["lmax"]
+    log("bench: pol nside=%d lmax=%d" % (nside, lmax))
+    mp = cm.Map(nside, lmax)
+    rng = np.random.default_rng(17)
+    n = (lmax + 1) ** 2
+    dev = torch.device("cuda:0")
+    ins = [torch.from_numpy(rng.standard_normal(n)).to(dev) for _ in range(3)]
+    outs = [torch.empty(mp.npix, dtype=torch.float64, device=dev) for _ in range(3)]
+    sp = torch.cuda.current_stream().cuda_stream
+    ptrs = [v.data_ptr() for v in ins] + [o.data_ptr() for o in outs]
+    for _ in range(args.warmup):
+        mp.alm2map_pol_device(*ptrs, sp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mp.alm2map_pol_device(*ptrs, sp)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    R, nalm = 2 * nside, (lmax + 1) * (lmax + 2) // 2
+    # algorithmic: T 8 R Nalm; Q/U 4 (recurrence) + 16 (a^E F1, a^B F2, a^B F1, a^E F2) per (l, m, pair)
+    flop = 28.0 * R * nalm
+    rate = args.steps / elapsed
+    out = {
+        "metric": "spin-2 synthesis (alm2map_pol T,E,B -> T,Q,U) calls/sec, FP64",
+        "value": round(rate, 3),
+        "unit": "calls/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "alm2map_pol nside=%d lmax=%d" % (nside, lmax), "nside": nside, "lmax": lmax},
+        "algorithmic_flop_per_call": flop,
+        "algorithmic_tflops": round(flop * rate / 1e12, 3),
+        "frac_fp64_peak": round(flop * rate / 1e12 / FP64_PEAK_TFLOPS, 4),
+        "executed_legendre_flop_per_call": 9 * 8.0 * R * nalm,
+    }
+    mp.close()
+    print(json.dumps(out), flush=True)
+
+
 def read_traffic(nside, lmax, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if any."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
@@ -192,6 +243,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=4, choices=sorted(CONFIGS) + [3],
                     help="workload: 4 (default, the headline), 1, 2, 5 (main map), 3 (64-map batch)")
+    ap.add_argument("--pol", action="store_true",
+                    help="time the spin-2 synthesis (cmbl_alm2map_pol_device) at the config's nside/lmax")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--host-buffers", action="store_true",
                     help="time cmbl_value_and_grad with x and g in host memory (PCIe-inclusive rate)")
@@ -229,6 +282,9 @@ def main():
         bench_batch(args)
         return
     cfg = CONFIGS[args.config]
+    if args.pol:
+        bench_pol(args, cfg)
+        return
     nside, lmax = cfg["nside"], cfg["lmax"]
     log("bench: building inputs nside=%d lmax=%d" % (nside, lmax))
     mp, inputs = build_case(cfg)

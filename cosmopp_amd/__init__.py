@@ -89,6 +89,8 @@ _SIGS = {
     "cmbl_whitenoise": ([ctypes.c_int, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
     "cmbl_map2alm_quadrature": ([_vp, _vp, _vp], ctypes.c_int),
     "cmbl_map2alm_quadrature_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_alm2map_pol": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_alm2map_pol_device": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "cmbl_pseudo_cl": ([_vp, ctypes.c_int, _vp], ctypes.c_int),
     "cmbl_coupling_matrix": ([_vp, ctypes.c_int, _vp], ctypes.c_int),
     "cmbl_coupling_matrix_device": ([_vp, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
@@ -340,6 +342,16 @@ class Map:
         out = np.zeros(self.nparams)
         _check(lib().cmbl_map2alm_quadrature(self._h, _p(m), _p(out)))
         return out
+
+    def alm2map_pol(self, t, e, b):
+        """HEALPix alm2map_pol: packed T, E, B a_lm (no beam) -> RING maps (T, Q, U)."""
+        t, e, b = (_arr(v, self.nparams) for v in (t, e, b))
+        out = [np.zeros(self.npix) for _ in range(3)]
+        _check(lib().cmbl_alm2map_pol(self._h, _p(t), _p(e), _p(b), *(_p(o) for o in out)))
+        return tuple(out)
+
+    def alm2map_pol_device(self, t_ptr, e_ptr, b_ptr, mt_ptr, mq_ptr, mu_ptr, stream=0):
+        _check(lib().cmbl_alm2map_pol_device(self._h, t_ptr, e_ptr, b_ptr, mt_ptr, mq_ptr, mu_ptr, stream))
 
     def backward(self, v):
         v = _arr(v, self.npix)

@@ -163,6 +163,7 @@ struct MapDesc {
     long npix, nalm;
     // ring-pair tables [npair_pad]
     const double* ring_x;     // cos(theta) of the north ring
+    const double* ring_s2;    // sin^2(theta), from 1 - z without cancellation
     const int* ring_nph;      // pixels per ring
     const int* ring_shift;    // 1 if phi0 = pi/nph, 0 if phi0 = 0
     const long* ring_ofsN;    // first pixel of the north ring
@@ -170,7 +171,7 @@ struct MapDesc {
     const int* ring_M;        // FFT length used (nph if a power of two, else Bluestein length)
     const long* ring_bsofs;   // offset of the Bluestein filter in bs_filt (-1 if none)
     const long* ring_chofs;   // offset of the Bluestein chirp in bs_chirp (-1 if none)
-    const double2* bs_filt;   // FFT_M of e^{+i pi j^2/b}, bit-reversed order, per non-power-of-two ring
+    const double2* bs_filt;   // FFT_M of e^{+i pi j^2/b}, natural order, per non-power-of-two ring
     const double2* bs_chirp;  // e^{-i pi j^2/b}, j < b, per non-power-of-two ring
     // Rings whose Bluestein length for nph would exceed kMaxFFT are split into
     // two transforms of b = nph/2 (even and odd samples); the others use b = nph.
@@ -268,6 +269,11 @@ int launch_synth(int K, const WorkItem* items, int nitems, const MapDesc* d_maps
 int launch_adj(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, hipStream_t s);
 int lm_tiles(int L);  // blocks of k_pre / k_post (prior partial sums)
 int launch_ldiv(double* y, const double* x, const double* dl, int L, hipStream_t s);
+// spin-2 synthesis: d_mds holds kPolVectors descriptors (one per coefficient vector)
+constexpr int kPolVectors = 9;
+int launch_pre_pol(const MapDesc& md, const MapDesc* d_mds, const double* t, const double* e, const double* b,
+                   double2* d_alm0, hipStream_t s);
+int launch_pol_combine(const MapDesc& md, const MapDesc* d_mds, hipStream_t s);
 int launch_reorder(const double* in, double* out, long nside, int to_ring, hipStream_t s);
 int launch_coupling(const double* w, int L, double* s0, double* M, hipStream_t s);
 int launch_ring(int mode, const RingItem* items, int nitems, const MapDesc* d_maps, int lds_bytes,
@@ -313,6 +319,12 @@ struct cmbl_map {
     double* d_x = nullptr;  // staging for the host entry points
     double* d_g = nullptr;
     double* d_map = nullptr;
+    // spin-2 synthesis (allocated at its first call): nine coefficient
+    // vectors and their parity sums, one descriptor per vector
+    cmbl::MapDesc* d_desc_pol = nullptr;
+    double2* d_pol_alm = nullptr;  // [kPolVectors * nalm]
+    double* d_pol_in = nullptr;   // [3 * nparam] host-call staging
+    double* d_pol_out = nullptr;  // [3 * npix]
     // mutable views of the workspace / data owned by this map
     double* d_beam = nullptr;
     double* d_invcl = nullptr;

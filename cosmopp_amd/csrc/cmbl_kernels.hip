@@ -201,6 +201,114 @@ __global__ __launch_bounds__(256) void k_ldiv(double* __restrict__ y, const doub
     }
 }
 
+// ================================================================== spin-2 synthesis
+// alm2map_pol (HEALPix; called at source/likelihood.cpp:585 and
+// source/c_matrix_generator.cpp:577-578): T = sum a^T Y, and with
+// W_lm = (2Y_lm + -2Y_lm)/2 = F1_lm(theta) e^{i m phi},
+// X_lm = (2Y_lm - -2Y_lm)/2 = F2_lm(theta) e^{i m phi},
+//   Q = -sum (a^E W + i a^B X),   U = -sum (a^B W - i a^E X).
+// With kappa_l = 2 / sqrt((l-1) l (l+1) (l+2)), q_lm = sqrt((2l+1)/(2l-1) (l^2 - m^2)),
+// c = cos(theta), s2 = sin^2(theta) and lambda_lm = Y_lm(theta, 0):
+//   F1_lm = kappa_l [-((l - m^2)/s2 + l(l-1)/2) lambda_lm + c q_lm lambda_{l-1,m} / s2]
+//   F2_lm = kappa_l (m/s2) [-(l-1) c lambda_lm + q_lm lambda_{l-1,m}].
+// For one field the sums over l split into four scalar Legendre sums
+//   D1 = sum kappa (l - m^2) a lambda_l,  D2 = sum kappa l(l-1)/2 a lambda_l,
+//   D3 = sum kappa q a lambda_{l-1},       D4 = sum kappa (l-1) a lambda_l,
+// giving sum_l a F1 = -D1/s2 - D2 + c D3/s2 and sum_l a F2 = m (D3 - c D4)/s2.
+// D3 is a scalar sum after the shift l -> l+1 of its coefficients.  The
+// nine vectors (a^T, D1..D4 of E, D1..D4 of B) go through the scalar
+// synthesis kernels; k_pol_combine then forms Q_m and U_m per ring.
+
+// Vector v of k_pre_pol lives at alm0 + v * nalm (m-major, times N_lm).
+__global__ __launch_bounds__(256) void k_pre_pol(const MapDesc* __restrict__ md_, const double* __restrict__ t,
+                                                 const double* __restrict__ e, const double* __restrict__ b,
+                                                 double2* __restrict__ alm0)
+{
+    __shared__ double T[3][kLM + 1][2 * kLM + 1];
+    const MapDesc& md = *md_;
+    const int L = md.L;
+    const long nalm = md.nalm;
+    int tl, tm;
+    tile_of(blockIdx.x, tl, tm);
+    const int l0 = tl * kLM, m0 = tm * kLM;
+    // rows l0 .. l0 + kLM: the extra row feeds the shifted D3 coefficients
+    for (int i = threadIdx.x; i < (kLM + 1) * 2 * kLM; i += 256) {
+        const int lr = i / (2 * kLM), c = i % (2 * kLM);
+        const int l = l0 + lr, m = m0 + (c >> 1);
+        const long pi = (l <= L && m <= l) ? packed_index(l, m, c & 1) : -1;
+        T[0][lr][c] = pi >= 0 ? t[pi] : 0.0;
+        T[1][lr][c] = pi >= 0 ? e[pi] : 0.0;
+        T[2][lr][c] = pi >= 0 ? b[pi] : 0.0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kLM * kLM; i += 256) {
+        const int mr = i / kLM, lr = i % kLM;
+        const int l = l0 + lr, m = m0 + mr;
+        if (l > L || m > l) continue;
+        const long k = moff(L, m) + (l - m);
+        const double nr = md.Nrm[k];
+        alm0[k] = make_double2(nr * T[0][lr][2 * mr], nr * T[0][lr][2 * mr + 1]);
+        const double fl = l, m2 = (double)m * m;
+        const double kap = l >= 2 ? 2.0 / sqrt((fl - 1.0) * fl * (fl + 1.0) * (fl + 2.0)) : 0.0;
+        const double l1 = fl + 1.0;
+        const double kq1 = (l + 1 <= L && l + 1 >= 2)
+                               ? 2.0 / sqrt((l1 - 1.0) * l1 * (l1 + 1.0) * (l1 + 2.0)) *
+                                     sqrt((2.0 * l1 + 1.0) / (2.0 * l1 - 1.0) * (l1 * l1 - m2))
+                               : 0.0;
+        const double c1 = kap * (fl - m2) * nr, c2 = kap * 0.5 * fl * (fl - 1.0) * nr;
+        const double c3 = kq1 * nr, c4 = kap * (fl - 1.0) * nr;
+#pragma unroll
+        for (int f = 1; f <= 2; ++f) {
+            const double re = T[f][lr][2 * mr], im = T[f][lr][2 * mr + 1];
+            const double re1 = T[f][lr + 1][2 * mr], im1 = T[f][lr + 1][2 * mr + 1];
+            double2* o = alm0 + (4 * f - 3) * nalm + k;
+            o[0] = make_double2(c1 * re, c1 * im);
+            o[nalm] = make_double2(c2 * re, c2 * im);
+            o[2 * nalm] = make_double2(c3 * re1, c3 * im1);
+            o[3 * nalm] = make_double2(c4 * re, c4 * im);
+        }
+    }
+}
+
+// Q_m, U_m of both rings of pair blockIdx.y from the parity sums of vectors
+// 1..8 (on a ring: north E + O at c, south E - O at -c).  Q overwrites
+// vector 1's sums and U vector 5's, as ((north + south)/2, (north - south)/2),
+// the form the ring stage reads.  One thread per m.
+__global__ __launch_bounds__(256) void k_pol_combine(const MapDesc* __restrict__ mds)
+{
+    const MapDesc& md = mds[0];
+    const int L = md.L;
+    const int m = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y;
+    if (m > L) return;
+    const double c = md.ring_x[p], is2 = 1.0 / md.ring_s2[p], fm = m;
+    const long o = ((long)p * (L + 1) + m) * 4;
+    double2 Wn[2], Ws[2], Xn[2], Xs[2];
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+        double2 dn[4], ds[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double4 q = *reinterpret_cast<const double4*>(mds[1 + 4 * f + v].P + o);
+            dn[v] = make_double2(q.x + q.z, q.y + q.w);
+            ds[v] = make_double2(q.x - q.z, q.y - q.w);
+        }
+        Wn[f] = make_double2(-is2 * dn[0].x - dn[1].x + c * is2 * dn[2].x, -is2 * dn[0].y - dn[1].y + c * is2 * dn[2].y);
+        Ws[f] = make_double2(-is2 * ds[0].x - ds[1].x - c * is2 * ds[2].x, -is2 * ds[0].y - ds[1].y - c * is2 * ds[2].y);
+        Xn[f] = make_double2(fm * is2 * (dn[2].x - c * dn[3].x), fm * is2 * (dn[2].y - c * dn[3].y));
+        Xs[f] = make_double2(fm * is2 * (ds[2].x + c * ds[3].x), fm * is2 * (ds[2].y + c * ds[3].y));
+    }
+    // Q = -(W_E + i X_B), U = -(W_B - i X_E)
+    const double2 Qn = make_double2(-(Wn[0].x - Xn[1].y), -(Wn[0].y + Xn[1].x));
+    const double2 Qs = make_double2(-(Ws[0].x - Xs[1].y), -(Ws[0].y + Xs[1].x));
+    const double2 Un = make_double2(-(Wn[1].x + Xn[0].y), -(Wn[1].y - Xn[0].x));
+    const double2 Us = make_double2(-(Ws[1].x + Xs[0].y), -(Ws[1].y - Xs[0].x));
+    *reinterpret_cast<double4*>(mds[1].P + o) =
+        make_double4(0.5 * (Qn.x + Qs.x), 0.5 * (Qn.y + Qs.y), 0.5 * (Qn.x - Qs.x), 0.5 * (Qn.y - Qs.y));
+    *reinterpret_cast<double4*>(mds[5].P + o) =
+        make_double4(0.5 * (Un.x + Us.x), 0.5 * (Un.y + Us.y), 0.5 * (Un.x - Us.x), 0.5 * (Un.y - Us.y));
+}
+
 // ================================================================== MASTER coupling
 // Mode-coupling matrix of a mask with pseudo spectrum W (source/master.cpp:180-215):
 //   M[l1][l2] = (2 l2 + 1) / (4 pi) sum_{l3 <= L} (2 l3 + 1) W_l3 (l1 l2 l3; 0 0 0)^2.
@@ -1780,6 +1888,21 @@ int launch_coupling(const double* w, int L, double* s0, double* M, hipStream_t s
     const long np = (long)(L + 1) * (L + 2) / 2;
     const int blocks = (int)std::min<long>((np + 255) / 256, 16384);
     hipLaunchKernelGGL(k_coupling, dim3(blocks), dim3(256), 0, s, L, w, s0, M);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_pre_pol(const MapDesc& md, const MapDesc* d_mds, const double* t, const double* e, const double* b,
+                   double2* d_alm0, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pre_pol, dim3(lm_tiles(md.L)), dim3(256), 0, s, d_mds, t, e, b, d_alm0);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_pol_combine(const MapDesc& md, const MapDesc* d_mds, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pol_combine, dim3((md.L + 1 + 255) / 256, md.npair), dim3(256), 0, s, d_mds);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }

@@ -32,7 +32,7 @@ static const long double kPiL = 3.141592653589793238462643383279502884L;
 // ------------------------------------------------------------------ geometry
 // HEALPix RING scheme (Gorski et al. 2005); pair p = north ring p+1.
 struct RingGeom {
-    double z, sint;
+    double z, sint, s2;
     int nph;
     int shift;  // phi0 = pi/nph
     long ofs;
@@ -47,13 +47,15 @@ static RingGeom ring_geom(int nside, long i /* 1-based ring */)
     if (in < N) {
         const double t = (double)(in * in) / (3.0 * (double)N * (double)N);
         g.z = 1.0 - t;
-        g.sint = std::sqrt(t * (2.0 - t));
+        g.s2 = t * (2.0 - t);
+        g.sint = std::sqrt(g.s2);
         g.nph = (int)(4 * in);
         g.shift = 1;
         g.ofs = 2 * in * (in - 1);
     } else {
         g.z = (double)(2 * N - in) * 2.0 / (3.0 * (double)N);
-        g.sint = std::sqrt((1.0 - g.z) * (1.0 + g.z));
+        g.s2 = (1.0 - g.z) * (1.0 + g.z);
+        g.sint = std::sqrt(g.s2);
         g.nph = (int)(4 * N);
         g.shift = ((in - N) % 2 == 0) ? 1 : 0;
         g.ofs = 2 * N * (N - 1) + (in - N) * 4 * N;
@@ -272,7 +274,7 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     const int npad = mp->npair_pad;
 
     // ---- ring-pair tables
-    std::vector<double> rx(npad, 0.0), rs(npad, 0.0);
+    std::vector<double> rx(npad, 0.0), rs(npad, 0.0), rs2(npad, 1.0);
     std::vector<int> nph(npad, 0), shift(npad, 0), Mfft(npad, 0);
     std::vector<long> ofsN(npad, 0), ofsS(npad, -1), bsofs(npad, -1), chofs(npad, -1), scrofs(npad, -1);
     std::vector<int> bs_pairs;
@@ -282,6 +284,7 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
         const RingGeom g = ring_geom(N, p + 1);
         rx[p] = g.z;
         rs[p] = g.sint;
+        rs2[p] = g.s2;
         nph[p] = g.nph;
         shift[p] = g.shift;
         ofsN[p] = g.ofs;
@@ -385,6 +388,9 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     int *p_nph, *p_sh, *p_M, *p_sk;
     long *p_oN, *p_oS, *p_bo;
     CMBL_TRY(dupload(mp, &p_rx, rx));
+    double* p_rs2;
+    CMBL_TRY(dupload(mp, &p_rs2, rs2));
+    d.ring_s2 = p_rs2;
     CMBL_TRY(dupload(mp, &p_nph, nph));
     CMBL_TRY(dupload(mp, &p_sh, shift));
     CMBL_TRY(dupload(mp, &p_oN, ofsN));
@@ -779,6 +785,71 @@ int cmbl_map2alm_quadrature(cmbl_map* mp, const double* map, double* alm)
     CMBL_HIP(hipMemcpy(mp->d_map, map, sizeof(double) * mp->npix, hipMemcpyHostToDevice));
     CMBL_TRY(cmbl_map2alm_quadrature_device(mp, mp->d_map, mp->d_g, nullptr));
     CMBL_HIP(hipMemcpy(alm, mp->d_g, sizeof(double) * mp->nparam, hipMemcpyDeviceToHost));
+    return CMBL_OK;
+}
+
+// ---------------------------------------------------------------- spin-2 synthesis
+// Workspace of cmbl_alm2map_pol, allocated at its first call: kPolVectors
+// m-major coefficient vectors and parity-sum buffers (zeroed once: entries of
+// (m, ring block) items with no live l are never written), one descriptor
+// per vector that differs from the map's only in alm and P.
+static int pol_setup(cmbl_map* mp)
+{
+    if (mp->d_desc_pol) return CMBL_OK;
+    const size_t pstride = (size_t)mp->npair * (mp->L + 1) * 4;
+    double2* alm;
+    double* P;
+    CMBL_TRY(dalloc(mp, &alm, (size_t)kPolVectors * mp->nalm));
+    CMBL_TRY(dalloc(mp, &P, (size_t)kPolVectors * pstride));
+    CMBL_HIP(hipMemset(P, 0, sizeof(double) * kPolVectors * pstride));
+    std::vector<MapDesc> v(kPolVectors, mp->desc);
+    for (int i = 0; i < kPolVectors; ++i) {
+        v[i].alm = alm + (size_t)i * mp->nalm;
+        v[i].P = P + (size_t)i * pstride;
+    }
+    MapDesc* d;
+    CMBL_TRY(dupload(mp, &d, v));
+    mp->d_desc_pol = d;
+    mp->d_pol_alm = alm;
+    return CMBL_OK;
+}
+
+int cmbl_alm2map_pol_device(cmbl_map* mp, const double* t, const double* e, const double* b, double* map_t,
+                            double* map_q, double* map_u, void* stream)
+{
+    CMBL_TRY(check_map(mp));
+    if (!t || !e || !b || !map_t || !map_q || !map_u) return fail(CMBL_ERR_ARG, "null pointer");
+    CMBL_TRY(pol_setup(mp));
+    hipStream_t s = (hipStream_t)stream;
+    MapDesc* dm = mp->d_desc_pol;
+    CMBL_TRY(launch_pre_pol(mp->desc, dm, t, e, b, mp->d_pol_alm, s));
+    for (int v = 0; v < kPolVectors; ++v) CMBL_TRY(launch_synth(mp->K1, mp->d_items1, mp->nitems1, dm + v, s));
+    CMBL_TRY(launch_pol_combine(mp->desc, dm, s));
+    CMBL_TRY(run_ring(kRingSyn, mp->d_ring_items, mp->ring_classes, dm + 0, map_t, nullptr, mp->d_tw, s));
+    CMBL_TRY(run_ring(kRingSyn, mp->d_ring_items, mp->ring_classes, dm + 1, map_q, nullptr, mp->d_tw, s));
+    CMBL_TRY(run_ring(kRingSyn, mp->d_ring_items, mp->ring_classes, dm + 5, map_u, nullptr, mp->d_tw, s));
+    return CMBL_OK;
+}
+
+int cmbl_alm2map_pol(cmbl_map* mp, const double* t, const double* e, const double* b, double* map_t, double* map_q,
+                     double* map_u)
+{
+    CMBL_TRY(check_map(mp));
+    if (!t || !e || !b || !map_t || !map_q || !map_u) return fail(CMBL_ERR_ARG, "null pointer");
+    if (!mp->d_pol_in) {
+        CMBL_TRY(dalloc(mp, &mp->d_pol_in, 3 * (size_t)mp->nparam));
+        CMBL_TRY(dalloc(mp, &mp->d_pol_out, 3 * (size_t)mp->npix));
+    }
+    double* in = mp->d_pol_in;
+    double* out = mp->d_pol_out;
+    const size_t n = mp->nparam, np = mp->npix;
+    CMBL_HIP(hipMemcpy(in, t, sizeof(double) * n, hipMemcpyHostToDevice));
+    CMBL_HIP(hipMemcpy(in + n, e, sizeof(double) * n, hipMemcpyHostToDevice));
+    CMBL_HIP(hipMemcpy(in + 2 * n, b, sizeof(double) * n, hipMemcpyHostToDevice));
+    CMBL_TRY(cmbl_alm2map_pol_device(mp, in, in + n, in + 2 * n, out, out + np, out + 2 * np, nullptr));
+    CMBL_HIP(hipMemcpy(map_t, out, sizeof(double) * np, hipMemcpyDeviceToHost));
+    CMBL_HIP(hipMemcpy(map_q, out + np, sizeof(double) * np, hipMemcpyDeviceToHost));
+    CMBL_HIP(hipMemcpy(map_u, out + 2 * np, sizeof(double) * np, hipMemcpyDeviceToHost));
     return CMBL_OK;
 }
 

@@ -177,6 +177,22 @@ int cmbl_whitenoise(int seed, double amplitude, long n, double* out);
  * layout), no beam. */
 int cmbl_map2alm_quadrature(cmbl_map* map, const double* m, double* alm);
 int cmbl_map2alm_quadrature_device(cmbl_map* map, const double* d_m, double* d_alm, void* stream);
+
+/* ---------------------------------------------------------------- spin-2 synthesis */
+/* HEALPix alm2map_pol(almT, almE, almB, mapT, mapQ, mapU) as called at
+ * source/likelihood.cpp:585 and source/c_matrix_generator.cpp:577-578,
+ * 645-646, 662-663: packed T, E, B a_lm (almToVector layout, no beam) ->
+ * RING maps
+ *   T = sum a^T Y,  Q = -sum (a^E W + i a^B X),  U = -sum (a^B W - i a^E X)
+ * with W = (2Y + -2Y)/2, X = (2Y - -2Y)/2 (spin-weighted harmonics of Goldberg
+ * et al. 1967 with the Condon-Shortley phase; a^E = -(a_2 + a_-2)/2,
+ * a^B = i (a_2 - a_-2)/2).  l < 2 of E and B do not contribute.  The first call
+ * allocates 9 coefficient vectors and 9 parity-sum buffers on the map's
+ * device (9 * 16 * nalm + 9 * 32 * 2 nside (lmax+1) bytes). */
+int cmbl_alm2map_pol(cmbl_map* map, const double* t, const double* e, const double* b, double* map_t,
+                     double* map_q, double* map_u);
+int cmbl_alm2map_pol_device(cmbl_map* map, const double* d_t, const double* d_e, const double* d_b,
+                            double* d_map_t, double* d_map_q, double* d_map_u, void* stream);
 /* ps(alm, l) = (|a_l0|^2 + 2 sum_m |a_lm|^2) / (2l+1) (source/master.cpp:62-73), l = 0..lmax. */
 int cmbl_pseudo_cl(const double* alm, int lmax, double* cl);
 /* Mode-coupling matrix of Master::calculateCouplingKernel (source/master.cpp:180-249):

@@ -45,6 +45,7 @@ def _load():
     lib.orc_fft.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int]
     lib.orc_dft.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
     lib.orc_synthesis.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_void_p, _dp]
+    lib.orc_alm2map_pol.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.orc_adjoint.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_void_p, _dp]
     lib.orc_neg2lnL.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_void_p]
     lib.orc_neg2lnL.restype = ctypes.c_double
@@ -112,6 +113,15 @@ def synthesis(nside, L, x, beam=None):
     b = None if beam is None else np.ascontiguousarray(beam, dtype=np.float64)
     lib().orc_synthesis(nside, L, x, _ptr(b), out)
     return out
+
+
+def alm2map_pol(nside, L, t, e, b):
+    """HEALPix alm2map_pol: packed T, E, B -> RING maps (T, Q, U), no beam."""
+    npix = 12 * nside * nside
+    mt, mq, mu = np.zeros(npix), np.zeros(npix), np.zeros(npix)
+    t, e, b = (np.ascontiguousarray(v, dtype=np.float64) for v in (t, e, b))
+    lib().orc_alm2map_pol(nside, L, t, e, b, mt, mq, mu)
+    return mt, mq, mu
 
 
 def adjoint(nside, L, m, beam=None):

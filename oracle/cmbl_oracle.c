@@ -27,6 +27,15 @@
  *       381-387 weighting, :541-549 sigma_l with m>0 doubled, :180-181
  *       C_0=C_1=0).  Pinned: exact quadratic => central finite differences,
  *       w = 0 (prior only) and noiseless-data (data term 0) closed cases.
+ *   orc_alm2map_pol  HEALPix alm2map_pol (called at source/likelihood.cpp:585,
+ *       source/c_matrix_generator.cpp:577-578).  Pinned: brute force against
+ *       spin-weighted harmonics from the explicit sum of Goldberg et al. 1967
+ *       (itself pinned by s = 0 -> Y_lm and by the eth-operator definition
+ *       2Y = sqrt((l-2)!/(l+2)!) eth eth Y), the closed form of the l = 2,
+ *       m = 0 E mode, E <-> B rotation.  Parity with HEALPix's own Q/U
+ *       sign convention rests on its documentation (Q + iU = sum a_2 2Y,
+ *       a^E = -(a_2 + a_-2)/2, a^B = i (a_2 - a_-2)/2): unpinned by any
+ *       reference file or run.
  *   orc_beam  Utils::beamFunction (source/utils.cpp:54-64).  Closed form.
  *   orc_mask_latitude / orc_mask_regions  source/test_like_high.cpp:123-130,
  *       Utils::maskRegions (source/utils.cpp:235-253).  Pinned: brute-force
@@ -348,6 +357,40 @@ static void phase(long m, const orc_ring* r, double* c, double* s)
 /* m-major complex coefficients B_l a_lm: index off(m) + (l - m) */
 static long moff(int L, int m) { return (long)m * (L + 1) - (long)m * (m - 1) / 2; }
 
+/* Real ring values from the per-ring coefficients F[ring][m], m = 0..L:
+ * f(phi_j) = F_0 + 2 Re sum_{m>=1} F_m e^{i m phi_j} (the real-field sum over
+ * +-m), by folding m onto the ring's nph bins and one DFT per ring. */
+static void rings_from_F(int nside, int L, const double* Fr, const double* Fi, double* map)
+{
+    const int nring = orc_nring(nside);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int r = 0; r < nring; ++r) {
+        orc_ring rg;
+        orc_ring_info(nside, r, &rg);
+        const long n = rg.nph;
+        double* gr = (double*)calloc(n, sizeof(double));
+        double* gi = (double*)calloc(n, sizeof(double));
+        for (long m = 0; m <= L; ++m) {
+            double c, s;
+            phase(m, &rg, &c, &s);
+            const double fr = Fr[(size_t)r * (L + 1) + m], fi = Fi[(size_t)r * (L + 1) + m];
+            const double pr = fr * c - fi * s, pi = fr * s + fi * c;
+            const long b = m % n;
+            gr[b] += pr;
+            gi[b] += pi;
+            if (m > 0) {
+                const long b2 = ((-m) % n + n) % n;
+                gr[b2] += pr;
+                gi[b2] -= pi;
+            }
+        }
+        orc_fft((int)n, gr, gi, +1);
+        for (long j = 0; j < n; ++j) map[rg.ofs + j] = gr[j];
+        free(gr);
+        free(gi);
+    }
+}
+
 void orc_synthesis(int nside, int L, const double* x, const double* beam, double* map)
 {
     const int nring = orc_nring(nside);
@@ -403,33 +446,97 @@ void orc_synthesis(int nside, int L, const double* x, const double* beam, double
         }
     }
 
+    rings_from_F(nside, L, Fr, Fi, map);
+    free(Fr); free(Fi); free(ar); free(ai);
+}
+
+/* HEALPix alm2map_pol (external; called at source/likelihood.cpp:585 and
+ * source/c_matrix_generator.cpp:577-578): T by orc_synthesis, and
+ *   Q_m = -sum_l (a^E F1_lm + i a^B F2_lm),  U_m = -sum_l (a^B F1_lm - i a^E F2_lm)
+ * per ring, F1 and F2 evaluated for each l from lambda_lm and lambda_{l-1,m}
+ * by the relations of the spin-2 harmonics to the scalar ones
+ * (W = (2Y + -2Y)/2 = F1 e^{i m phi}, X = (2Y - -2Y)/2 = F2 e^{i m phi}):
+ *   F1 = kappa_l [-((l - m^2)/s^2 + l(l-1)/2) lambda_l + (c/s^2) q_lm lambda_{l-1}]
+ *   F2 = kappa_l (m/s^2) [-(l-1) c lambda_l + q_lm lambda_{l-1}]
+ * kappa_l = 2/sqrt((l-1) l (l+1) (l+2)), q_lm = sqrt((2l+1)/(2l-1) (l^2 - m^2)).
+ * The south ring of a pair takes c -> -c and lambda_l -> (-1)^{l+m} lambda_l. */
+void orc_alm2map_pol(int nside, int L, const double* t, const double* e, const double* b, double* mt,
+                     double* mq, double* mu)
+{
+    orc_synthesis(nside, L, t, NULL, mt);
+    const int nring = orc_nring(nside);
+    const int npair = 2 * nside;
+    const long nalm = (long)(L + 1) * (L + 2) / 2;
+    double* er = (double*)calloc(nalm, sizeof(double));
+    double* ei = (double*)calloc(nalm, sizeof(double));
+    double* br = (double*)calloc(nalm, sizeof(double));
+    double* bi = (double*)calloc(nalm, sizeof(double));
+    for (int m = 0; m <= L; ++m)
+        for (int l = m; l <= L; ++l) {
+            const long k = moff(L, m) + (l - m);
+            const long i = (long)l * l + (m == 0 ? 0 : 2 * m - 1);
+            er[k] = e[i];
+            br[k] = b[i];
+            ei[k] = m == 0 ? 0.0 : e[i + 1];
+            bi[k] = m == 0 ? 0.0 : b[i + 1];
+        }
+    const size_t nf = (size_t)nring * (L + 1);
+    double* Qr = (double*)calloc(nf, sizeof(double));
+    double* Qi = (double*)calloc(nf, sizeof(double));
+    double* Ur = (double*)calloc(nf, sizeof(double));
+    double* Ui = (double*)calloc(nf, sizeof(double));
+
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int r = 0; r < nring; ++r) {
+    for (int p = 0; p < npair; ++p) {
         orc_ring rg;
-        orc_ring_info(nside, r, &rg);
-        const long n = rg.nph;
-        double* gr = (double*)calloc(n, sizeof(double));
-        double* gi = (double*)calloc(n, sizeof(double));
-        for (long m = 0; m <= L; ++m) {
-            double c, s;
-            phase(m, &rg, &c, &s);
-            const double fr = Fr[(size_t)r * (L + 1) + m], fi = Fi[(size_t)r * (L + 1) + m];
-            const double pr = fr * c - fi * s, pi = fr * s + fi * c;
-            const long b = m % n;
-            gr[b] += pr;
-            gi[b] += pi;
-            if (m > 0) {
-                const long b2 = ((-m) % n + n) % n;
-                gr[b2] += pr;
-                gi[b2] -= pi;
+        orc_ring_info(nside, p, &rg);
+        const int rn = p, rs = nring - 1 - p;
+        const double s2 = rg.sint * rg.sint;
+        for (int m = 0; m <= L; ++m) {
+            orc_lam st;
+            lam_init(&st, m, rg.z, rg.sint);
+            double acc[2][4] = {{0}}; /* [north/south][Q.re, Q.im, U.re, U.im] */
+            const long base = moff(L, m) - m;
+            const double fm = m;
+            for (int l = m; l <= L; ++l) {
+                if (l > m) lam_step(&st);
+                if (l < 2) continue;
+                const double lam = lam_value(&st);
+                const double lamp = (l > m && st.e == 0) ? st.v2 : 0.0;
+                if (lam == 0.0 && lamp == 0.0) continue;
+                const double fl = l;
+                const double kap = 2.0 / sqrt((fl - 1.0) * fl * (fl + 1.0) * (fl + 2.0));
+                const double q = sqrt((2.0 * fl + 1.0) / (2.0 * fl - 1.0) * (fl * fl - fm * fm));
+                const double ar = er[base + l], ai = ei[base + l], cr = br[base + l], ci = bi[base + l];
+                for (int h = 0; h < 2; ++h) {
+                    /* south ring: (-1)^{l+m} on lambda_l, the opposite sign on lambda_{l-1} */
+                    const double sl = (h == 1 && ((l + m) & 1)) ? -1.0 : 1.0;
+                    const double c = h == 0 ? rg.z : -rg.z;
+                    const double ll = sl * lam, lpp = (h == 0 ? 1.0 : -sl) * lamp;
+                    const double f1 = kap * (-((fl - fm * fm) / s2 + 0.5 * fl * (fl - 1.0)) * ll + c / s2 * q * lpp);
+                    const double f2 = kap * (fm / s2) * (-(fl - 1.0) * c * ll + q * lpp);
+                    acc[h][0] -= ar * f1 - ci * f2;
+                    acc[h][1] -= ai * f1 + cr * f2;
+                    acc[h][2] -= cr * f1 + ai * f2;
+                    acc[h][3] -= ci * f1 - ar * f2;
+                }
+            }
+            Qr[(size_t)rn * (L + 1) + m] = acc[0][0];
+            Qi[(size_t)rn * (L + 1) + m] = acc[0][1];
+            Ur[(size_t)rn * (L + 1) + m] = acc[0][2];
+            Ui[(size_t)rn * (L + 1) + m] = acc[0][3];
+            if (rs != rn) {
+                Qr[(size_t)rs * (L + 1) + m] = acc[1][0];
+                Qi[(size_t)rs * (L + 1) + m] = acc[1][1];
+                Ur[(size_t)rs * (L + 1) + m] = acc[1][2];
+                Ui[(size_t)rs * (L + 1) + m] = acc[1][3];
             }
         }
-        orc_fft((int)n, gr, gi, +1);
-        for (long j = 0; j < n; ++j) map[rg.ofs + j] = gr[j];
-        free(gr);
-        free(gi);
     }
-    free(Fr); free(Fi); free(ar); free(ai);
+    rings_from_F(nside, L, Qr, Qi, mq);
+    rings_from_F(nside, L, Ur, Ui, mu);
+    free(Qr); free(Qi); free(Ur); free(Ui);
+    free(er); free(ei); free(br); free(bi);
 }
 
 /* u_lm = sum_p f_p lambda_lm(theta_p) e^{-i m phi_p}, m-major complex */

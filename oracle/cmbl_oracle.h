@@ -56,6 +56,9 @@ void orc_dft(int n, const double* re, const double* im, double* ore, double* oim
 
 /* s = Y B a : packed real a_lm (n = (L+1)^2, almToVector layout) -> RING map. */
 void orc_synthesis(int nside, int L, const double* x_packed, const double* beam, double* map);
+/* HEALPix alm2map_pol: packed T, E, B -> RING maps T, Q, U (no beam). */
+void orc_alm2map_pol(int nside, int L, const double* t, const double* e, const double* b, double* mt,
+                     double* mq, double* mu);
 /* Exact transpose of orc_synthesis: out_packed = J^T map where J = d s / d x.
  * With u_lm = sum_p map_p lambda_lm(theta_p) e^{-i m phi_p}:
  *   out[l^2] = B_l Re u_l0, out[l^2+2m-1] = 2 B_l Re u_lm, out[l^2+2m] = 2 B_l Im u_lm. */

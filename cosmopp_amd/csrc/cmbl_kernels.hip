@@ -61,6 +61,13 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Global-address-space views of device memory: loads and stores through them
+// compile to global_* instructions (counted by vmcnt only) rather than flat_*
+// ones, whose lgkmcnt share makes every LDS wait also wait for memory, and
+// which the compiler must order after earlier LDS and global stores.
+typedef __attribute__((address_space(1))) double gdouble;
+typedef const __attribute__((address_space(1))) double gcdouble;
+
 // Block-wide sum for blockDim.x == nthreads (multiple of 64); result valid in thread 0.
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* sh)
@@ -591,8 +598,8 @@ __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items
     const int m = it.m0;
     const int l0 = it.lfirst;
     const long base = moff(L, m) - m;
-    const double* __restrict__ A = md.A + base;
-    const double2* __restrict__ a = md.alm + base;
+    gcdouble* __restrict__ A = (gcdouble*)(md.A + base);
+    gcdouble* __restrict__ a = (gcdouble*)(md.alm + base);  // (re, im) interleaved
 
     LegState<K> st;
     SynAcc<K> ac;
@@ -609,11 +616,11 @@ __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items
     }
     st.refresh();
     {
-        const double2 al = a[l0];
+        const double ar = a[2 * l0], ai = a[2 * l0 + 1];
         if ((l0 - m) & 1)
-            syn_gated<K, true>(st, ac, al.x, al.y);
+            syn_gated<K, true>(st, ac, ar, ai);
         else
-            syn_gated<K, false>(st, ac, al.x, al.y);
+            syn_gated<K, false>(st, ac, ar, ai);
     }
     // staged chunks of steps l0+1 .. L
     int cb = l0 + 1;
@@ -621,7 +628,7 @@ __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items
     double2 ta = make_double2(0.0, 0.0);
     if (cb + lane <= L) {
         tA = A[cb + lane];
-        ta = a[cb + lane];
+        ta = make_double2(a[2 * (cb + lane)], a[2 * (cb + lane) + 1]);
     }
     sA[0][lane] = tA;
     sa[0][lane] = ta;
@@ -632,7 +639,7 @@ __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items
         const int nb = cb + kChunk;
         if (nb + lane <= L) {
             tA = A[nb + lane];
-            ta = a[nb + lane];
+            ta = make_double2(a[2 * (nb + lane)], a[2 * (nb + lane) + 1]);
         }
         const int cend = min(L, nb - 1);
         syn_chunk<K>(st, ac, l, cend, cb, m, sA[buf], sa[buf]);
@@ -766,10 +773,6 @@ struct AdjIn {
     double per[K], pei[K], por[K], poi[K];
 };
 
-// A double in the global address space: stores through it compile to
-// global_store (counted by vmcnt only) rather than flat_store, whose lgkmcnt
-// share would make every LDS wait a full drain.
-typedef __attribute__((address_space(1))) double gdouble;
 
 // Lanes 0..31 get a summed over lanes (j, j + 32), lanes 32..63 get b summed
 // likewise (v_permlane32_swap exchanges lanes 32..63 of its first operand
@@ -1006,7 +1009,7 @@ __global__ __launch_bounds__(64) void k_adj(const WorkItem* __restrict__ items, 
     const int m = it.m0;
     const int l0 = it.lfirst;
     const long base = moff(L, m) - m;
-    const double* __restrict__ A = md.A + base;
+    gcdouble* __restrict__ A = (gcdouble*)(md.A + base);
 
     LegState<K> st;
     AdjIn<K> in;

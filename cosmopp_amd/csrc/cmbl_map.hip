@@ -273,6 +273,16 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     mp->nrb5 = mp->npair_pad / (64 * mp->K5);
     const int npad = mp->npair_pad;
 
+    // Bluestein rings longer than split_limit are split into two half-length
+    // transforms: above 4096 a ring's LDS (139 KB at 8192) would allow one
+    // workgroup per CU, while two halves of 4096 run two per CU (tuning
+    // override CMBL_SPLIT_M in {2048, 4096, 8192})
+    int split_limit = 4096;
+    if (const char* e = std::getenv("CMBL_SPLIT_M")) {
+        const int v = std::atoi(e);
+        if (v == 2048 || v == 4096 || v == 8192) split_limit = v;
+    }
+
     // ---- ring-pair tables
     std::vector<double> rx(npad, 0.0), rs(npad, 0.0), rs2(npad, 1.0);
     std::vector<int> nph(npad, 0), shift(npad, 0), Mfft(npad, 0);
@@ -295,7 +305,7 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
             // Bluestein length b (nph, or nph/2 for a split ring; nph = 4i is even)
             int b = g.nph, M = 1;
             while (M < 2 * b - 1) M <<= 1;
-            if (M > kMaxFFT) {
+            if (M > split_limit) {
                 b = g.nph / 2;
                 M = 1;
                 while (M < 2 * b - 1) M <<= 1;

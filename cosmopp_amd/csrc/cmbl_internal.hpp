@@ -269,11 +269,13 @@ int launch_synth(int K, const WorkItem* items, int nitems, const MapDesc* d_maps
 int launch_adj(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, hipStream_t s);
 int lm_tiles(int L);  // blocks of k_pre / k_post (prior partial sums)
 int launch_ldiv(double* y, const double* x, const double* dl, int L, hipStream_t s);
-// spin-2 synthesis: d_mds holds kPolVectors descriptors (one per coefficient vector)
-constexpr int kPolVectors = 9;
-int launch_pre_pol(const MapDesc& md, const MapDesc* d_mds, const double* t, const double* e, const double* b,
-                   double2* d_alm0, hipStream_t s);
-int launch_pol_combine(const MapDesc& md, const MapDesc* d_mds, hipStream_t s);
+// spin-2 synthesis: kPolRec complex coefficients per (l, m); parity sums of
+// T, Q and U in three buffers laid out as MapDesc::P
+constexpr int kPolRec = 5;
+int launch_pre_pol(const MapDesc& md, const MapDesc* d_md, const double* t, const double* e, const double* b,
+                   double2* d_rec, hipStream_t s);
+int launch_synth_pol(int K, const WorkItem* items, int nitems, const MapDesc* d_md, const double2* d_rec,
+                     double* d_pt, double* d_pq, double* d_pu, hipStream_t s);
 int launch_reorder(const double* in, double* out, long nside, int to_ring, hipStream_t s);
 int launch_coupling(const double* w, int L, double* s0, double* M, hipStream_t s);
 int launch_ring(int mode, const RingItem* items, int nitems, const MapDesc* d_maps, int lds_bytes,
@@ -319,10 +321,11 @@ struct cmbl_map {
     double* d_x = nullptr;  // staging for the host entry points
     double* d_g = nullptr;
     double* d_map = nullptr;
-    // spin-2 synthesis (allocated at its first call): nine coefficient
-    // vectors and their parity sums, one descriptor per vector
-    cmbl::MapDesc* d_desc_pol = nullptr;
-    double2* d_pol_alm = nullptr;  // [kPolVectors * nalm]
+    // spin-2 synthesis (allocated at its first call): coefficient records,
+    // parity sums of T, Q, U and one descriptor per sum for the ring stage
+    cmbl::MapDesc* d_desc_pol = nullptr;  // [3]
+    double2* d_pol_rec = nullptr;         // [kPolRec * nalm]
+    double* d_pol_P = nullptr;            // [3 * npair * (lmax + 1) * 4]
     double* d_pol_in = nullptr;   // [3 * nparam] host-call staging
     double* d_pol_out = nullptr;  // [3 * npix]
     // mutable views of the workspace / data owned by this map

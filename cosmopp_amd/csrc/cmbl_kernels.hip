@@ -218,27 +218,27 @@ __global__ __launch_bounds__(256) void k_ldiv(double* __restrict__ y, const doub
 // c = cos(theta), s2 = sin^2(theta) and lambda_lm = Y_lm(theta, 0):
 //   F1_lm = kappa_l [-((l - m^2)/s2 + l(l-1)/2) lambda_lm + c q_lm lambda_{l-1,m} / s2]
 //   F2_lm = kappa_l (m/s2) [-(l-1) c lambda_lm + q_lm lambda_{l-1,m}].
-// For one field the sums over l split into four scalar Legendre sums
-//   D1 = sum kappa (l - m^2) a lambda_l,  D2 = sum kappa l(l-1)/2 a lambda_l,
-//   D3 = sum kappa q a lambda_{l-1},       D4 = sum kappa (l-1) a lambda_l,
-// giving sum_l a F1 = -D1/s2 - D2 + c D3/s2 and sum_l a F2 = m (D3 - c D4)/s2.
-// D3 is a scalar sum after the shift l -> l+1 of its coefficients.  The
-// nine vectors (a^T, D1..D4 of E, D1..D4 of B) go through the scalar
-// synthesis kernels; k_pol_combine then forms Q_m and U_m per ring.
+// Per field the sums over l need four scalar Legendre sums of one recursion:
+//   Sa = sum kappa a lambda_l,  Sb = sum l kappa a lambda_l,  Sc = sum l^2 kappa a lambda_l,
+//   D3 = sum kappa q a lambda_{l-1}  (coefficients shifted by one l),
+// and with D1 = Sb - m^2 Sa, D2 = (Sc - Sb)/2, D4 = Sb - Sa:
+//   sum_l a F1 = -D1/s2 - D2 + c D3/s2,   sum_l a F2 = m (D3 - c D4)/s2.
+// k_synth_pol runs the recursion once per (m, ring block) for the T sum and
+// the eight field sums and forms Q_m, U_m per ring in its epilogue.
 
-// Vector v of k_pre_pol lives at alm0 + v * nalm (m-major, times N_lm).
+// Record of (l, m) (m-major, kPolRec complex): N a^T, N kappa_l a^E,
+// N kappa_{l+1} q_{l+1,m} a^E_{l+1,m}, and the same two for B.
 __global__ __launch_bounds__(256) void k_pre_pol(const MapDesc* __restrict__ md_, const double* __restrict__ t,
                                                  const double* __restrict__ e, const double* __restrict__ b,
-                                                 double2* __restrict__ alm0)
+                                                 double2* __restrict__ rec)
 {
     __shared__ double T[3][kLM + 1][2 * kLM + 1];
     const MapDesc& md = *md_;
     const int L = md.L;
-    const long nalm = md.nalm;
     int tl, tm;
     tile_of(blockIdx.x, tl, tm);
     const int l0 = tl * kLM, m0 = tm * kLM;
-    // rows l0 .. l0 + kLM: the extra row feeds the shifted D3 coefficients
+    // rows l0 .. l0 + kLM: the extra row feeds the shifted coefficients
     for (int i = threadIdx.x; i < (kLM + 1) * 2 * kLM; i += 256) {
         const int lr = i / (2 * kLM), c = i % (2 * kLM);
         const int l = l0 + lr, m = m0 + (c >> 1);
@@ -254,66 +254,20 @@ __global__ __launch_bounds__(256) void k_pre_pol(const MapDesc* __restrict__ md_
         if (l > L || m > l) continue;
         const long k = moff(L, m) + (l - m);
         const double nr = md.Nrm[k];
-        alm0[k] = make_double2(nr * T[0][lr][2 * mr], nr * T[0][lr][2 * mr + 1]);
         const double fl = l, m2 = (double)m * m;
-        const double kap = l >= 2 ? 2.0 / sqrt((fl - 1.0) * fl * (fl + 1.0) * (fl + 2.0)) : 0.0;
+        const double ck = l >= 2 ? nr * 2.0 / sqrt((fl - 1.0) * fl * (fl + 1.0) * (fl + 2.0)) : 0.0;
         const double l1 = fl + 1.0;
-        const double kq1 = (l + 1 <= L && l + 1 >= 2)
-                               ? 2.0 / sqrt((l1 - 1.0) * l1 * (l1 + 1.0) * (l1 + 2.0)) *
-                                     sqrt((2.0 * l1 + 1.0) / (2.0 * l1 - 1.0) * (l1 * l1 - m2))
-                               : 0.0;
-        const double c1 = kap * (fl - m2) * nr, c2 = kap * 0.5 * fl * (fl - 1.0) * nr;
-        const double c3 = kq1 * nr, c4 = kap * (fl - 1.0) * nr;
-#pragma unroll
-        for (int f = 1; f <= 2; ++f) {
-            const double re = T[f][lr][2 * mr], im = T[f][lr][2 * mr + 1];
-            const double re1 = T[f][lr + 1][2 * mr], im1 = T[f][lr + 1][2 * mr + 1];
-            double2* o = alm0 + (4 * f - 3) * nalm + k;
-            o[0] = make_double2(c1 * re, c1 * im);
-            o[nalm] = make_double2(c2 * re, c2 * im);
-            o[2 * nalm] = make_double2(c3 * re1, c3 * im1);
-            o[3 * nalm] = make_double2(c4 * re, c4 * im);
-        }
+        const double dk = (l + 1 <= L && l + 1 >= 2)
+                              ? nr * 2.0 / sqrt((l1 - 1.0) * l1 * (l1 + 1.0) * (l1 + 2.0)) *
+                                    sqrt((2.0 * l1 + 1.0) / (2.0 * l1 - 1.0) * (l1 * l1 - m2))
+                              : 0.0;
+        double2* o = rec + k * kPolRec;
+        o[0] = make_double2(nr * T[0][lr][2 * mr], nr * T[0][lr][2 * mr + 1]);
+        o[1] = make_double2(ck * T[1][lr][2 * mr], ck * T[1][lr][2 * mr + 1]);
+        o[2] = make_double2(dk * T[1][lr + 1][2 * mr], dk * T[1][lr + 1][2 * mr + 1]);
+        o[3] = make_double2(ck * T[2][lr][2 * mr], ck * T[2][lr][2 * mr + 1]);
+        o[4] = make_double2(dk * T[2][lr + 1][2 * mr], dk * T[2][lr + 1][2 * mr + 1]);
     }
-}
-
-// Q_m, U_m of both rings of pair blockIdx.y from the parity sums of vectors
-// 1..8 (on a ring: north E + O at c, south E - O at -c).  Q overwrites
-// vector 1's sums and U vector 5's, as ((north + south)/2, (north - south)/2),
-// the form the ring stage reads.  One thread per m.
-__global__ __launch_bounds__(256) void k_pol_combine(const MapDesc* __restrict__ mds)
-{
-    const MapDesc& md = mds[0];
-    const int L = md.L;
-    const int m = blockIdx.x * 256 + threadIdx.x;
-    const int p = blockIdx.y;
-    if (m > L) return;
-    const double c = md.ring_x[p], is2 = 1.0 / md.ring_s2[p], fm = m;
-    const long o = ((long)p * (L + 1) + m) * 4;
-    double2 Wn[2], Ws[2], Xn[2], Xs[2];
-#pragma unroll
-    for (int f = 0; f < 2; ++f) {
-        double2 dn[4], ds[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const double4 q = *reinterpret_cast<const double4*>(mds[1 + 4 * f + v].P + o);
-            dn[v] = make_double2(q.x + q.z, q.y + q.w);
-            ds[v] = make_double2(q.x - q.z, q.y - q.w);
-        }
-        Wn[f] = make_double2(-is2 * dn[0].x - dn[1].x + c * is2 * dn[2].x, -is2 * dn[0].y - dn[1].y + c * is2 * dn[2].y);
-        Ws[f] = make_double2(-is2 * ds[0].x - ds[1].x - c * is2 * ds[2].x, -is2 * ds[0].y - ds[1].y - c * is2 * ds[2].y);
-        Xn[f] = make_double2(fm * is2 * (dn[2].x - c * dn[3].x), fm * is2 * (dn[2].y - c * dn[3].y));
-        Xs[f] = make_double2(fm * is2 * (ds[2].x + c * ds[3].x), fm * is2 * (ds[2].y + c * ds[3].y));
-    }
-    // Q = -(W_E + i X_B), U = -(W_B - i X_E)
-    const double2 Qn = make_double2(-(Wn[0].x - Xn[1].y), -(Wn[0].y + Xn[1].x));
-    const double2 Qs = make_double2(-(Ws[0].x - Xs[1].y), -(Ws[0].y + Xs[1].x));
-    const double2 Un = make_double2(-(Wn[1].x + Xn[0].y), -(Wn[1].y - Xn[0].x));
-    const double2 Us = make_double2(-(Ws[1].x + Xs[0].y), -(Ws[1].y - Xs[0].x));
-    *reinterpret_cast<double4*>(mds[1].P + o) =
-        make_double4(0.5 * (Qn.x + Qs.x), 0.5 * (Qn.y + Qs.y), 0.5 * (Qn.x - Qs.x), 0.5 * (Qn.y - Qs.y));
-    *reinterpret_cast<double4*>(mds[5].P + o) =
-        make_double4(0.5 * (Un.x + Us.x), 0.5 * (Un.y + Us.y), 0.5 * (Un.x - Us.x), 0.5 * (Un.y - Us.y));
 }
 
 // ================================================================== MASTER coupling
@@ -760,6 +714,231 @@ __global__ __launch_bounds__(64) void k_synth_s(const WorkItem* __restrict__ ite
             dst[0] = make_double2(ac.er[k], ac.ei[k]);
             dst[1] = make_double2(ac.orr[k], ac.oi[k]);
         }
+    }
+}
+
+// -------------------------------------------------------------- fused spin-2 synthesis
+// Accumulators of one ring pair, per slot (0: steps at even offsets from the
+// item's first l, 1: odd offsets), 18 doubles: T (re, im), then per field f
+// (E, B) at 2 + 8 f: Sa, Sb, Sc, D3.
+template <int K>
+struct PolAcc {
+    double v[2][K][18];
+};
+
+template <int K, int SLOT>
+__device__ __forceinline__ void pol_accum(PolAcc<K>& ac, int k, double nu, double fl, double fl2,
+                                          const double2 (&c)[kPolRec])
+{
+    double* a = ac.v[SLOT][k];
+    a[0] = fma(nu, c[0].x, a[0]);
+    a[1] = fma(nu, c[0].y, a[1]);
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+        double* q = a + 2 + 8 * f;
+        const double ur = nu * c[1 + 2 * f].x, ui = nu * c[1 + 2 * f].y;
+        q[0] += ur;
+        q[1] += ui;
+        q[2] = fma(fl, ur, q[2]);
+        q[3] = fma(fl, ui, q[3]);
+        q[4] = fma(fl2, ur, q[4]);
+        q[5] = fma(fl2, ui, q[5]);
+        q[6] = fma(nu, c[2 + 2 * f].x, q[6]);
+        q[7] = fma(nu, c[2 + 2 * f].y, q[7]);
+    }
+}
+
+template <int K, int SLOT>
+__device__ __forceinline__ void pol_live(LegState<K>& st, PolAcc<K>& ac, double Al, int l,
+                                         const double2 (&c)[kPolRec])
+{
+    const double fl = l, fl2 = fl * fl;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double nu = fma(Al * st.x[k], st.mu1[k], -st.mu2[k]);
+        st.mu2[k] = st.mu1[k];
+        st.mu1[k] = nu;
+        pol_accum<K, SLOT>(ac, k, nu, fl, fl2, c);
+    }
+}
+
+// Gated step: negligible lanes contribute gate 0 (no branch, so the
+// accumulators stay in place across steps).
+template <int K, int SLOT>
+__device__ __forceinline__ void pol_gated(const LegState<K>& st, PolAcc<K>& ac, int l, const double2 (&c)[kPolRec])
+{
+    const double fl = l, fl2 = fl * fl;
+#pragma unroll
+    for (int k = 0; k < K; ++k) pol_accum<K, SLOT>(ac, k, st.gate[k] * st.mu1[k], fl, fl2, c);
+}
+
+template <int K>
+__device__ __forceinline__ void pol_load(const double2 (*cr)[kPolRec], int j, double2 (&c)[kPolRec])
+{
+#pragma unroll
+    for (int i = 0; i < kPolRec; ++i) c[i] = cr[j][i];
+}
+
+// Steps l .. cend of one staged chunk in pairs: the accumulators are indexed
+// by the step's position relative to the item's first l (slot 0 for even
+// offsets), so no parity decision sits between accumulator updates.  Chunks
+// start at odd offsets and hold an even number of steps except the last,
+// whose extra step reads the zero coefficients staged beyond lmax.  Scaled
+// pairs carry a rescale check each, then live pairs.
+template <int K>
+__device__ __forceinline__ void pol_chunk(LegState<K>& st, PolAcc<K>& ac, int& l, int cend, int cb,
+                                          const double* cA, const double2 (*cr)[kPolRec])
+{
+    double2 c[kPolRec];
+    while (!st.all && l <= cend) {
+        pol_load<K>(cr, l - cb, c);
+        st.advance(cA[l - cb]);
+        pol_gated<K, 1>(st, ac, l, c);
+        pol_load<K>(cr, l + 1 - cb, c);
+        st.advance(cA[l + 1 - cb]);
+        pol_gated<K, 0>(st, ac, l + 1, c);
+        l += 2;
+        if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
+    }
+    for (; l <= cend; l += 2) {
+        pol_load<K>(cr, l - cb, c);
+        pol_live<K, 1>(st, ac, cA[l - cb], l, c);
+        pol_load<K>(cr, l + 1 - cb, c);
+        pol_live<K, 0>(st, ac, cA[l + 1 - cb], l + 1, c);
+    }
+}
+
+// One (m, ring block) item of the synthesis work list: T, Q, U parity sums of
+// its ring pairs into pt, pq, pu ([pair][m] (E.re, E.im, O.re, O.im), the
+// form the ring stage reads; Q and U as ((north + south)/2, (north - south)/2)).
+template <int K>
+__global__ __launch_bounds__(64) void k_synth_pol(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps,
+                                                  const double2* __restrict__ rec, double* __restrict__ pt,
+                                                  double* __restrict__ pq, double* __restrict__ pu)
+{
+    __shared__ double sA[2][kChunk + kChunkPad];
+    __shared__ double2 sr[2][kChunk + kChunkPad][kPolRec];
+    const WorkItem it = items[blockIdx.x];
+    if (it.mcnt == 0) return;
+    const MapDesc& md = maps[it.map];
+    const int L = md.L;
+    const int npad = md.npair_pad;
+    const int lane = threadIdx.x;
+    const int m = it.m0;
+    const int l0 = it.lfirst;
+    const long base = moff(L, m) - m;
+    gcdouble* __restrict__ A = (gcdouble*)(md.A + base);
+    gcdouble* __restrict__ R = (gcdouble*)(rec + base * kPolRec);  // 2 kPolRec doubles per l
+
+    LegState<K> st;
+    PolAcc<K> ac;
+    int p[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        p[k] = it.rb * 64 * K + k * 64 + lane;
+        const long q = (long)m * npad + p[k];
+        st.x[k] = md.ring_x[p[k]];
+        st.mu1[k] = md.s1_cur[q];
+        st.mu2[k] = md.s1_prev[q];
+        st.sc[k] = md.s1_k[q];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) ac.v[0][k][i] = ac.v[1][k][i] = 0.0;
+    }
+    st.refresh();
+    {
+        double2 c[kPolRec];
+#pragma unroll
+        for (int i = 0; i < kPolRec; ++i)
+            c[i] = make_double2(R[(long)l0 * 2 * kPolRec + 2 * i], R[(long)l0 * 2 * kPolRec + 2 * i + 1]);
+        pol_gated<K, 0>(st, ac, l0, c);
+    }
+    // staged chunks of steps l0+1 .. L
+    int cb = l0 + 1;
+    double tA = 0.0;
+    double2 tc[kPolRec];
+#pragma unroll
+    for (int i = 0; i < kPolRec; ++i) tc[i] = make_double2(0.0, 0.0);
+    if (cb + lane <= L) {
+        tA = A[cb + lane];
+#pragma unroll
+        for (int i = 0; i < kPolRec; ++i)
+            tc[i] = make_double2(R[(long)(cb + lane) * 2 * kPolRec + 2 * i], R[(long)(cb + lane) * 2 * kPolRec + 2 * i + 1]);
+    }
+    sA[0][lane] = tA;
+#pragma unroll
+    for (int i = 0; i < kPolRec; ++i) sr[0][lane][i] = tc[i];
+    wave_sync();
+    int buf = 0;
+    int l = cb;
+    while (cb <= L) {
+        const int nb = cb + kChunk;
+        // entries beyond lmax stage zeros: the last chunk's extra step reads one
+        tA = 0.0;
+#pragma unroll
+        for (int i = 0; i < kPolRec; ++i) tc[i] = make_double2(0.0, 0.0);
+        if (nb + lane <= L) {
+            tA = A[nb + lane];
+#pragma unroll
+            for (int i = 0; i < kPolRec; ++i)
+                tc[i] = make_double2(R[(long)(nb + lane) * 2 * kPolRec + 2 * i],
+                                     R[(long)(nb + lane) * 2 * kPolRec + 2 * i + 1]);
+        }
+        pol_chunk<K>(st, ac, l, min(L, nb - 1), cb, sA[buf], sr[buf]);
+        if (nb <= L) {
+            sA[buf ^ 1][lane] = tA;
+#pragma unroll
+            for (int i = 0; i < kPolRec; ++i) sr[buf ^ 1][lane][i] = tc[i];
+        }
+        wave_sync();
+        buf ^= 1;
+        cb = nb;
+    }
+    // epilogue: per ring pair, north (E + O at c) and south (E - O at -c)
+    const double fm = m, fm2 = fm * fm;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (p[k] >= md.npair) continue;
+        const long o = ((long)p[k] * (L + 1) + m) * 4;
+        const bool swap = ((l0 - m) & 1) != 0;  // slot 0 holds the parity of l0 - m
+        double e[18], d[18];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) {
+            e[i] = swap ? ac.v[1][k][i] : ac.v[0][k][i];
+            d[i] = swap ? ac.v[0][k][i] : ac.v[1][k][i];
+        }
+        *reinterpret_cast<double4*>(pt + o) = make_double4(e[0], e[1], d[0], d[1]);
+        const double c = st.x[k], is2 = 1.0 / md.ring_s2[p[k]];
+        double W[2][2][2], X[2][2][2];  // [field][north/south][re/im]
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const double sg = h ? -1.0 : 1.0;  // south: E - O, c -> -c
+                const double ch = sg * c;
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    const int b = 2 + 8 * f + r;
+                    const double Sa = e[b] + sg * d[b], Sb = e[b + 2] + sg * d[b + 2];
+                    const double Sc = e[b + 4] + sg * d[b + 4], D3 = e[b + 6] + sg * d[b + 6];
+                    const double D1 = Sb - fm2 * Sa, D2 = 0.5 * (Sc - Sb), D4 = Sb - Sa;
+                    W[f][h][r] = -is2 * D1 - D2 + ch * is2 * D3;
+                    X[f][h][r] = fm * is2 * (D3 - ch * D4);
+                }
+            }
+        }
+        // Q = -(W_E + i X_B), U = -(W_B - i X_E)
+        double Q[2][2], U[2][2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            Q[h][0] = -(W[0][h][0] - X[1][h][1]);
+            Q[h][1] = -(W[0][h][1] + X[1][h][0]);
+            U[h][0] = -(W[1][h][0] + X[0][h][1]);
+            U[h][1] = -(W[1][h][1] - X[0][h][0]);
+        }
+        *reinterpret_cast<double4*>(pq + o) = make_double4(0.5 * (Q[0][0] + Q[1][0]), 0.5 * (Q[0][1] + Q[1][1]),
+                                                           0.5 * (Q[0][0] - Q[1][0]), 0.5 * (Q[0][1] - Q[1][1]));
+        *reinterpret_cast<double4*>(pu + o) = make_double4(0.5 * (U[0][0] + U[1][0]), 0.5 * (U[0][1] + U[1][1]),
+                                                           0.5 * (U[0][0] - U[1][0]), 0.5 * (U[0][1] - U[1][1]));
     }
 }
 
@@ -1895,19 +2074,34 @@ int launch_coupling(const double* w, int L, double* s0, double* M, hipStream_t s
     return CMBL_OK;
 }
 
-int launch_pre_pol(const MapDesc& md, const MapDesc* d_mds, const double* t, const double* e, const double* b,
-                   double2* d_alm0, hipStream_t s)
+int launch_pre_pol(const MapDesc& md, const MapDesc* d_md, const double* t, const double* e, const double* b,
+                   double2* d_rec, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pre_pol, dim3(lm_tiles(md.L)), dim3(256), 0, s, d_mds, t, e, b, d_alm0);
+    hipLaunchKernelGGL(k_pre_pol, dim3(lm_tiles(md.L)), dim3(256), 0, s, d_md, t, e, b, d_rec);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }
 
-int launch_pol_combine(const MapDesc& md, const MapDesc* d_mds, hipStream_t s)
+template <int K>
+static int synth_pol_k(const WorkItem* items, int n, const MapDesc* d_md, const double2* rec, double* pt, double* pq,
+                       double* pu, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pol_combine, dim3((md.L + 1 + 255) / 256, md.npair), dim3(256), 0, s, d_mds);
+    if (n <= 0) return CMBL_OK;
+    hipLaunchKernelGGL(k_synth_pol<K>, dim3(n), dim3(64), 0, s, items, d_md, rec, pt, pq, pu);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
+}
+
+int launch_synth_pol(int K, const WorkItem* items, int n, const MapDesc* d_md, const double2* d_rec, double* d_pt,
+                     double* d_pq, double* d_pu, hipStream_t s)
+{
+    switch (K) {
+    case 1: return synth_pol_k<1>(items, n, d_md, d_rec, d_pt, d_pq, d_pu, s);
+    case 2: return synth_pol_k<2>(items, n, d_md, d_rec, d_pt, d_pq, d_pu, s);
+    case 4: return synth_pol_k<4>(items, n, d_md, d_rec, d_pt, d_pq, d_pu, s);
+    case 8: return synth_pol_k<8>(items, n, d_md, d_rec, d_pt, d_pq, d_pu, s);
+    }
+    return fail(CMBL_ERR_ARG, "unsupported rings-per-lane K");
 }
 
 int launch_reorder(const double* in, double* out, long nside, int to_ring, hipStream_t s)

@@ -799,28 +799,22 @@ int cmbl_map2alm_quadrature(cmbl_map* mp, const double* map, double* alm)
 }
 
 // ---------------------------------------------------------------- spin-2 synthesis
-// Workspace of cmbl_alm2map_pol, allocated at its first call: kPolVectors
-// m-major coefficient vectors and parity-sum buffers (zeroed once: entries of
-// (m, ring block) items with no live l are never written), one descriptor
-// per vector that differs from the map's only in alm and P.
+// Workspace of cmbl_alm2map_pol, allocated at its first call: the
+// coefficient records and the T, Q, U parity-sum buffers (zeroed once: entries
+// of (m, ring block) items with no live l are never written), with one
+// descriptor per buffer that differs from the map's only in P.
 static int pol_setup(cmbl_map* mp)
 {
     if (mp->d_desc_pol) return CMBL_OK;
     const size_t pstride = (size_t)mp->npair * (mp->L + 1) * 4;
-    double2* alm;
-    double* P;
-    CMBL_TRY(dalloc(mp, &alm, (size_t)kPolVectors * mp->nalm));
-    CMBL_TRY(dalloc(mp, &P, (size_t)kPolVectors * pstride));
-    CMBL_HIP(hipMemset(P, 0, sizeof(double) * kPolVectors * pstride));
-    std::vector<MapDesc> v(kPolVectors, mp->desc);
-    for (int i = 0; i < kPolVectors; ++i) {
-        v[i].alm = alm + (size_t)i * mp->nalm;
-        v[i].P = P + (size_t)i * pstride;
-    }
+    CMBL_TRY(dalloc(mp, &mp->d_pol_rec, (size_t)kPolRec * mp->nalm));
+    CMBL_TRY(dalloc(mp, &mp->d_pol_P, 3 * pstride));
+    CMBL_HIP(hipMemset(mp->d_pol_P, 0, sizeof(double) * 3 * pstride));
+    std::vector<MapDesc> v(3, mp->desc);
+    for (int i = 0; i < 3; ++i) v[i].P = mp->d_pol_P + (size_t)i * pstride;
     MapDesc* d;
     CMBL_TRY(dupload(mp, &d, v));
     mp->d_desc_pol = d;
-    mp->d_pol_alm = alm;
     return CMBL_OK;
 }
 
@@ -832,12 +826,14 @@ int cmbl_alm2map_pol_device(cmbl_map* mp, const double* t, const double* e, cons
     CMBL_TRY(pol_setup(mp));
     hipStream_t s = (hipStream_t)stream;
     MapDesc* dm = mp->d_desc_pol;
-    CMBL_TRY(launch_pre_pol(mp->desc, dm, t, e, b, mp->d_pol_alm, s));
-    for (int v = 0; v < kPolVectors; ++v) CMBL_TRY(launch_synth(mp->K1, mp->d_items1, mp->nitems1, dm + v, s));
-    CMBL_TRY(launch_pol_combine(mp->desc, dm, s));
+    const size_t pstride = (size_t)mp->npair * (mp->L + 1) * 4;
+    double* P = mp->d_pol_P;
+    CMBL_TRY(launch_pre_pol(mp->desc, mp->d_desc, t, e, b, mp->d_pol_rec, s));
+    CMBL_TRY(launch_synth_pol(mp->K1, mp->d_items1, mp->nitems1, mp->d_desc, mp->d_pol_rec, P, P + pstride,
+                              P + 2 * pstride, s));
     CMBL_TRY(run_ring(kRingSyn, mp->d_ring_items, mp->ring_classes, dm + 0, map_t, nullptr, mp->d_tw, s));
     CMBL_TRY(run_ring(kRingSyn, mp->d_ring_items, mp->ring_classes, dm + 1, map_q, nullptr, mp->d_tw, s));
-    CMBL_TRY(run_ring(kRingSyn, mp->d_ring_items, mp->ring_classes, dm + 5, map_u, nullptr, mp->d_tw, s));
+    CMBL_TRY(run_ring(kRingSyn, mp->d_ring_items, mp->ring_classes, dm + 2, map_u, nullptr, mp->d_tw, s));
     return CMBL_OK;
 }
 

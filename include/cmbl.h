@@ -187,8 +187,8 @@ int cmbl_map2alm_quadrature_device(cmbl_map* map, const double* d_m, double* d_a
  * with W = (2Y + -2Y)/2, X = (2Y - -2Y)/2 (spin-weighted harmonics of Goldberg
  * et al. 1967 with the Condon-Shortley phase; a^E = -(a_2 + a_-2)/2,
  * a^B = i (a_2 - a_-2)/2).  l < 2 of E and B do not contribute.  The first call
- * allocates 9 coefficient vectors and 9 parity-sum buffers on the map's
- * device (9 * 16 * nalm + 9 * 32 * 2 nside (lmax+1) bytes). */
+ * allocates a coefficient table and three parity-sum buffers on the map's
+ * device (5 * 16 * nalm + 3 * 32 * 2 nside (lmax+1) bytes). */
 int cmbl_alm2map_pol(cmbl_map* map, const double* t, const double* e, const double* b, double* map_t,
                      double* map_q, double* map_u);
 int cmbl_alm2map_pol_device(cmbl_map* map, const double* d_t, const double* d_e, const double* d_b,

@@ -1104,21 +1104,68 @@ __device__ __forceinline__ void adj_step(LegState<K>& st, const AdjIn<K>& in, Ad
         adj_live<K, false>(st, in, rd, Al);
 }
 
-// Steps l..cend of one staged chunk while some lane is still scaled: gated
-// steps (rescale check every two) until every lane is live, then the rest of
-// the chunk one live step at a time.
+// One gated step returning its lane sums: negligible lanes contribute gate 0.
+template <int K>
+__device__ __forceinline__ void adj_step_gv(LegState<K>& st, const AdjIn<K>& in, bool o, double Al, double& sr,
+                                            double& si)
+{
+    st.advance(Al);
+    sr = 0.0;
+    si = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double f = st.gate[k] * st.mu1[k];
+        sr = fma(f, o ? in.por[k] : in.per[k], sr);
+        si = fma(f, o ? in.poi[k] : in.pei[k], si);
+    }
+}
+
+// Steps l..cend of one staged chunk while some lane is still scaled: single
+// gated steps up to a row boundary (rescale check after each), then rows of
+// kRedChunk = 8 gated steps reduced like the live rows with one rescale check
+// per row, then the rest of the chunk one step at a time (gated while some
+// lane is scaled).  Eight steps multiply |mu| by at most prod (A_l + 1)
+// < 2^45 (A_{m+1} = sqrt(2m + 2) is the largest factor), so from 2^900 no
+// value overflows between checks; a lane whose scale count reaches 0 inside
+// a row joins at the row's end, dropping terms below 2^-55 of O(1).
 template <int K>
 __device__ __forceinline__ void adj_chunk_scaled(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, int& l, int cend,
                                                  int cb, int m, const double* cA)
 {
-    while (!st.all && l <= cend) {
-        for (int rep = 0; rep < 2 && l <= cend; ++rep, ++l) {
-            st.advance(cA[l - cb]);
-            if ((l - m) & 1)
-                adj_gated<K, true>(st, in, rd);
-            else
-                adj_gated<K, false>(st, in, rd);
+    while (!st.all && rd.cnt != 0 && l <= cend) {
+        st.advance(cA[l - cb]);
+        if ((l - m) & 1)
+            adj_gated<K, true>(st, in, rd);
+        else
+            adj_gated<K, false>(st, in, rd);
+        ++l;
+        if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
+    }
+    if (!st.all && l + 7 <= cend) {
+        const bool odd = ((l - m) & 1) != 0;
+        for (; !st.all && l + 7 <= cend; l += 8) {
+            double v0[8], v1[8], z[4], t4[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                adj_step_gv<K>(st, in, odd ^ (q & 1), cA[l + q - cb], v0[2 * q], v0[2 * q + 1]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                adj_step_gv<K>(st, in, odd ^ (q & 1), cA[l + 4 + q - cb], v1[2 * q], v1[2 * q + 1]);
+            rd.fold_row(v0, v1, z);
+            rd.tile_store(z);
+            rd.tile_load(t4);
+            rd.finish_row(t4);
+            wave_sync();
+            if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
         }
+    }
+    while (!st.all && l <= cend) {
+        st.advance(cA[l - cb]);
+        if ((l - m) & 1)
+            adj_gated<K, true>(st, in, rd);
+        else
+            adj_gated<K, false>(st, in, rd);
+        ++l;
         if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
     }
     for (; l <= cend; ++l) adj_step<K>(st, in, rd, l, m, cA[l - cb]);
@@ -1174,8 +1221,11 @@ __device__ __forceinline__ void adj_chunk_live(LegState<K>& st, const AdjIn<K>& 
     for (; l <= cend; ++l) adj_step<K>(st, in, rd, l, m, cA[l - cb]);
 }
 
+// K = 4 (the adjoint's ring pairs per lane from nside 512 up) is held to
+// three waves per SIMD (168 VGPRs): the compiler's own choice of 202 leaves
+// two, and the spills the bound costs sit outside the step loops.
 template <int K>
-__global__ __launch_bounds__(64) void k_adj(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
+__global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
 {
     __shared__ double red[64 * kRedStride];
     __shared__ double sA[2][kChunk + kChunkPad];

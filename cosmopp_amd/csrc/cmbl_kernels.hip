@@ -505,13 +505,51 @@ __device__ __forceinline__ void syn_live_run(LegState<K>& st, SynAcc<K>& ac, int
     }
 }
 
+// Rows of eight gated steps (parity pattern ODD0, !ODD0, ...): negligible lanes
+// contribute gate 0, with no branch between the accumulator updates, and one
+// rescale check per row.  Eight steps multiply |mu| by at most
+// prod (A_l + 1) < 2^45 (A_{m+1} = sqrt(2m + 2) is the largest factor), so
+// from 2^900 nothing overflows between checks; a lane whose scale count
+// reaches 0 inside a row joins at the row's end, dropping terms below 2^-55
+// of O(1).
+template <int K, bool ODD0>
+__device__ __forceinline__ void syn_gated_rows(LegState<K>& st, SynAcc<K>& ac, int& l, int cend, int cb,
+                                               const double* cA, const double2* ca)
+{
+    for (; !st.all && l + 7 <= cend; l += 8) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const double2 al = ca[l + q - cb];
+            st.advance(cA[l + q - cb]);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const double f = st.gate[k] * st.mu1[k];
+                if (ODD0 ^ (q & 1)) {
+                    ac.orr[k] = fma(f, al.x, ac.orr[k]);
+                    ac.oi[k] = fma(f, al.y, ac.oi[k]);
+                } else {
+                    ac.er[k] = fma(f, al.x, ac.er[k]);
+                    ac.ei[k] = fma(f, al.y, ac.ei[k]);
+                }
+            }
+        }
+        if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
+    }
+}
+
 // Steps l = l .. cend of one staged chunk (coefficients cA[l - cb], ca[l - cb]).
-// Scaled path: rescale checks after every second step (|mu| grows by less
-// than 2^7 per step, so 2^900 * 2^14 stays finite).
+// Scaled path: gated rows while eight steps remain, then single gated steps
+// with a rescale check after every second step.
 template <int K>
 __device__ __forceinline__ void syn_chunk(LegState<K>& st, SynAcc<K>& ac, int& l, int cend, int cb, int m,
                                           const double* cA, const double2* ca)
 {
+    if (!st.all && l + 7 <= cend) {
+        if ((l - m) & 1)
+            syn_gated_rows<K, true>(st, ac, l, cend, cb, cA, ca);
+        else
+            syn_gated_rows<K, false>(st, ac, l, cend, cb, cA, ca);
+    }
     while (!st.all && l <= cend) {
         for (int rep = 0; rep < 2 && l <= cend; ++rep, ++l) {
             const double2 al = ca[l - cb];

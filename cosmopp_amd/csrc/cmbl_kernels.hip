@@ -653,108 +653,6 @@ __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items
     }
 }
 
-// Variant with SGPR operands: the coefficients stream through scalar loads
-// (constant address space) with an 8-step software prefetch, so no LDS ->
-// VGPR broadcast traffic is spent on wave-uniform values.
-template <typename T>
-using cptr = const __attribute__((address_space(4))) T*;
-
-template <int K>
-__global__ __launch_bounds__(64) void k_synth_s(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
-{
-    const WorkItem it = items[blockIdx.x];
-    if (it.mcnt == 0) return;
-    const MapDesc& md = maps[it.map];
-    const int L = md.L;
-    const int npad = md.npair_pad;
-    const int lane = threadIdx.x;
-    const int m = it.m0;
-    const int l0 = it.lfirst;
-    const long base = moff(L, m) - m;
-    cptr<double> A = (cptr<double>)(md.A + base);
-    cptr<double> a = (cptr<double>)(reinterpret_cast<const double*>(md.alm + base));
-
-    LegState<K> st;
-    SynAcc<K> ac;
-    int p[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        p[k] = it.rb * 64 * K + k * 64 + lane;
-        const long q = (long)m * npad + p[k];
-        st.x[k] = md.ring_x[p[k]];
-        st.mu1[k] = md.s1_cur[q];
-        st.mu2[k] = md.s1_prev[q];
-        st.sc[k] = md.s1_k[q];
-        ac.er[k] = ac.ei[k] = ac.orr[k] = ac.oi[k] = 0.0;
-    }
-    st.refresh();
-    if ((l0 - m) & 1)
-        syn_gated<K, true>(st, ac, a[2 * l0], a[2 * l0 + 1]);
-    else
-        syn_gated<K, false>(st, ac, a[2 * l0], a[2 * l0 + 1]);
-    int l = l0 + 1;
-    while (!st.all && l <= L) {
-        for (int rep = 0; rep < 2 && l <= L; ++rep, ++l) {
-            st.advance(A[l]);
-            if ((l - m) & 1)
-                syn_gated<K, true>(st, ac, a[2 * l], a[2 * l + 1]);
-            else
-                syn_gated<K, false>(st, ac, a[2 * l], a[2 * l + 1]);
-        }
-        if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
-    }
-    // live: groups of four steps, the next two groups' coefficients in flight
-    if (l + 11 <= L) {
-        double c0[4], r0[4], i0[4], c1[4], r1[4], i1[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            c0[q] = A[l + q];
-            r0[q] = a[2 * (l + q)];
-            i0[q] = a[2 * (l + q) + 1];
-            c1[q] = A[l + 4 + q];
-            r1[q] = a[2 * (l + 4 + q)];
-            i1[q] = a[2 * (l + 4 + q) + 1];
-        }
-        const bool odd = ((l - m) & 1) != 0;
-        for (; l + 11 <= L; l += 4) {
-            double c2[4], r2[4], i2[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                c2[q] = A[l + 8 + q];
-                r2[q] = a[2 * (l + 8 + q)];
-                i2[q] = a[2 * (l + 8 + q) + 1];
-            }
-            if (odd)
-                syn_live4<K, true>(st, ac, c0, r0, i0);
-            else
-                syn_live4<K, false>(st, ac, c0, r0, i0);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                c0[q] = c1[q];
-                r0[q] = r1[q];
-                i0[q] = i1[q];
-                c1[q] = c2[q];
-                r1[q] = r2[q];
-                i1[q] = i2[q];
-            }
-        }
-    }
-    for (; l <= L; ++l) {
-        if ((l - m) & 1)
-            syn_live<K, true>(st, ac, A[l], a[2 * l], a[2 * l + 1]);
-        else
-            syn_live<K, false>(st, ac, A[l], a[2 * l], a[2 * l + 1]);
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        if (p[k] < md.npair) {
-            double2* dst = reinterpret_cast<double2*>(md.P + ((long)p[k] * (L + 1) + m) * 4);
-            dst[0] = make_double2(ac.er[k], ac.ei[k]);
-            dst[1] = make_double2(ac.orr[k], ac.oi[k]);
-        }
-    }
-}
-
 // -------------------------------------------------------------- fused spin-2 synthesis
 // Accumulators of one ring pair, per slot (0: steps at even offsets from the
 // item's first l, 1: odd offsets), 18 doubles: T (re, im), then per field f
@@ -1997,24 +1895,11 @@ int launch_pre(const MapDesc& md, const MapDesc* d_md, const double* d_x, hipStr
     return CMBL_OK;
 }
 
-static int synth_variant()
-{
-    static int v = -1;
-    if (v < 0) {
-        const char* e = std::getenv("CMBL_SYNTH_SMEM");
-        v = (e && e[0] == '1') ? 1 : 0;
-    }
-    return v;
-}
-
 template <int K>
 static int synth_k(const WorkItem* items, int n, const MapDesc* d_maps, hipStream_t s)
 {
     if (n <= 0) return CMBL_OK;
-    if (synth_variant())
-        hipLaunchKernelGGL(k_synth_s<K>, dim3(n), dim3(64), 0, s, items, d_maps);
-    else
-        hipLaunchKernelGGL(k_synth<K>, dim3(n), dim3(64), 0, s, items, d_maps);
+    hipLaunchKernelGGL(k_synth<K>, dim3(n), dim3(64), 0, s, items, d_maps);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }

@@ -208,14 +208,17 @@ static std::vector<WorkItem> order_items(std::vector<ItemCost>& v)
 }
 
 // Ring items ordered by transform length, largest first, and cut into at
-// most four launches by LDS size (M > 4096, > 2048, > 512, the rest), so that
-// small transforms are not limited to one workgroup per CU by the largest.
+// most two launches by LDS size: M > 4096 (139 KB, one workgroup per CU) and
+// the rest.  Up to M = 4096 the ring kernels are held by their registers to
+// two workgroups per CU whatever their LDS, so one launch carries every ring
+// up to 4096 and the small ones fill its tail instead of running as launches
+// of a few hundred workgroups.
 template <typename MOf>
 static std::vector<RingClass> ring_classes(std::vector<RingItem>* items, MOf Mof)
 {
     std::stable_sort(items->begin(), items->end(),
                      [&](const RingItem& a, const RingItem& b) { return Mof(a) > Mof(b); });
-    auto cls = [](int M) { return M > 4096 ? 0 : M > 2048 ? 1 : M > 512 ? 2 : 3; };
+    auto cls = [](int M) { return M > 4096 ? 0 : 1; };
     std::vector<RingClass> out;
     for (int i = 0; i < (int)items->size(); ++i) {
         const int M = Mof((*items)[i]);

@@ -146,6 +146,40 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_c5_chains(args):
+    """The 16 chain maps of config 5 (nside 256 / lmax 512, FWHM 40 arcmin,
+    sigma 40 uK, 20 degree cut, independent seeds), evaluated as one batch."""
+    import torch
+
+    import cosmopp_amd as cm
+
+    cfg = dict(nside=256, lmax=512, fwhm_deg=40.0 / 60.0, sigma=40.0, cut_deg=20.0)
+    dev = torch.device("cuda:0")
+    maps, xs, gs = [], [], []
+    for i in range(16):
+        mp, inp = build_case(cfg, 2000 + 10 * i)
+        maps.append(mp)
+        xs.append(torch.from_numpy(inp["x"]).to(dev))
+        gs.append(torch.empty_like(xs[-1]))
+    f = torch.empty(len(maps), dtype=torch.float64, device=dev)
+    batch = cm.Batch(maps)
+    sp = torch.cuda.current_stream().cuda_stream
+    xp, gp = [x.data_ptr() for x in xs], [g.data_ptr() for g in gs]
+    for _ in range(args.warmup):
+        batch.value_and_grad_device(xp, f.data_ptr(), gp, sp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.value_and_grad_device(xp, f.data_ptr(), gp, sp)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    batch.close()
+    for mp in maps:
+        mp.close()
+    return {"maps": 16, "nside": 256, "lmax": 512, "ms_per_batch": round(ms, 4),
+            "map_evals_per_s": round(16 / ms * 1e3, 3)}
+
+
 def bench_pol(args, cfg):
     """Spin-2 synthesis (HEALPix alm2map_pol: T, E, B -> T, Q, U) on device
     buffers at the configuration's nside / lmax; one step = one call."""
@@ -391,6 +425,12 @@ def main():
         "neg2lnL": fval,
     }
     mp.close()
+    if args.config == 5:
+        log("bench: config 5 chain maps")
+        ch = bench_c5_chains(args)
+        out["chains"] = ch
+        # one config-5 step: the main map and the 16 chain maps once each
+        out["config5_steps_per_s"] = round(1e3 / (ms_per_step + ch["ms_per_batch"]), 3)
     print(json.dumps(out), flush=True)
 
 

@@ -283,6 +283,21 @@ int launch_ring(int mode, const RingItem* items, int nitems, const MapDesc* d_ma
 int launch_post(const MapDesc& md, const MapDesc* d_md, const double* d_x, double* d_g, int mode,
                 hipStream_t s, double c = 0.0);
 int launch_final(const MapDesc* d_md, double* d_f, hipStream_t s);
+
+// One launch of the a_lm preparation, the gradient assembly or the -2lnL
+// reduction over up to kBatchGroup maps of a batch (kernel argument, by
+// value): map i of the group owns blocks tile0[i] .. tile0[i+1]-1.
+constexpr int kBatchGroup = 64;
+struct BatchArgs {
+    const double* x[kBatchGroup];
+    double* g[kBatchGroup];
+    double* f;  // f[i]: -2lnL of map i of the group
+    int tile0[kBatchGroup + 1];
+    int n;
+};
+int launch_pre_batch(const MapDesc* d_mds, const BatchArgs& a, hipStream_t s);
+int launch_post_batch(const MapDesc* d_mds, const BatchArgs& a, int mode, hipStream_t s);
+int launch_final_batch(const MapDesc* d_mds, const BatchArgs& a, hipStream_t s);
 int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_bs, double2* d_out,
                            double2* d_chirp, int lds_bytes, const double2* d_tw, hipStream_t s);
 int launch_scan(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, int* d_lstart,

@@ -111,14 +111,13 @@ __device__ __forceinline__ long packed_index(int l, int m, int c)
 
 // alm = B_l N_lm a_lm (m-major) from the packed x, and the prior partial sum
 // of the tile: sum invcl_l (|a_l0|^2 + 2 sum_m>0 |a_lm|^2).
-__global__ __launch_bounds__(256) void k_pre(const MapDesc* __restrict__ md_, const double* __restrict__ x)
+__device__ __forceinline__ void pre_tile(const MapDesc& md, const double* __restrict__ x, int tile)
 {
     __shared__ double T[kLM][2 * kLM + 1];
     __shared__ double sh[4];
-    const MapDesc& md = *md_;
     const int L = md.L;
     int tl, tm;
-    tile_of(blockIdx.x, tl, tm);
+    tile_of(tile, tl, tm);
     const int l0 = tl * kLM, m0 = tm * kLM;
     for (int i = threadIdx.x; i < kLM * 2 * kLM; i += 256) {
         const int lr = i / (2 * kLM), c = i % (2 * kLM);
@@ -140,21 +139,39 @@ __global__ __launch_bounds__(256) void k_pre(const MapDesc* __restrict__ md_, co
         }
     }
     const double t = block_sum<256>(pr, sh);
-    if (threadIdx.x == 0) md.prior[blockIdx.x] = t;
+    if (threadIdx.x == 0) md.prior[tile] = t;
+}
+
+__global__ __launch_bounds__(256) void k_pre(const MapDesc* __restrict__ md_, const double* __restrict__ x)
+{
+    pre_tile(*md_, x, blockIdx.x);
+}
+
+// Map of block b in a batched launch (wave-uniform, at most kBatchGroup steps).
+__device__ __forceinline__ int batch_map(const BatchArgs& a, int b)
+{
+    int i = 0;
+    while (i + 1 < a.n && a.tile0[i + 1] <= b) ++i;
+    return i;
+}
+
+__global__ __launch_bounds__(256) void k_pre_b(const MapDesc* __restrict__ mds, const BatchArgs a)
+{
+    const int i = batch_map(a, blockIdx.x);
+    pre_tile(mds[i], a.x[i], blockIdx.x - a.tile0[i]);
 }
 
 // u = N_lm B_l sum_rb upart (fixed order) -> the packed output of `mode`
 // (PostMode): the gradient forms kPostGrad / kPostLike, or the Gibbs forms in
 // the HEALPix complex-a_lm convention (no factor 2 for m >= 1).  kPostAddAlm
 // scales by c instead of B_l and adds to g.
-__global__ __launch_bounds__(256) void k_post(const MapDesc* __restrict__ md_, const double* __restrict__ x,
-                                              double* __restrict__ g, int mode, double c)
+__device__ __forceinline__ void post_tile(const MapDesc& md, const double* __restrict__ x, double* __restrict__ g,
+                                          int mode, double c, int tile)
 {
     __shared__ double T[kLM][2 * kLM + 1];
-    const MapDesc& md = *md_;
     const int L = md.L;
     int tl, tm;
-    tile_of(blockIdx.x, tl, tm);
+    tile_of(tile, tl, tm);
     const int l0 = tl * kLM, m0 = tm * kLM;
     for (int i = threadIdx.x; i < kLM * kLM; i += 256) {
         const int mr = i / kLM, lr = i % kLM;
@@ -192,6 +209,18 @@ __global__ __launch_bounds__(256) void k_post(const MapDesc* __restrict__ md_, c
         default: g[pi] += u; break;  // kPostAddAlm
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_post(const MapDesc* __restrict__ md_, const double* __restrict__ x,
+                                              double* __restrict__ g, int mode, double c)
+{
+    post_tile(*md_, x, g, mode, c, blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_post_b(const MapDesc* __restrict__ mds, const BatchArgs a, int mode)
+{
+    const int i = batch_map(a, blockIdx.x);
+    post_tile(mds[i], a.x[i], a.g[i], mode, 0.0, blockIdx.x - a.tile0[i]);
 }
 
 // y_p = x_p / d_l over the packed layout (l = floor(sqrt(p))): the Gibbs
@@ -1833,15 +1862,25 @@ __global__ __launch_bounds__(kRingThreads) void k_bluestein_setup(const int* __r
 
 // ================================================================== k_final
 // One block: -2lnL = sum_pairs chi + sum_blocks prior, fixed order.
-__global__ __launch_bounds__(1024) void k_final(const MapDesc* __restrict__ md_, double* __restrict__ f)
+__device__ __forceinline__ void final_map(const MapDesc& md, double* __restrict__ f)
 {
     __shared__ double sh[16];
-    const MapDesc& md = *md_;
     double a = 0.0;
     for (int i = threadIdx.x; i < md.npair; i += 1024) a += md.chi[i];
     for (int i = threadIdx.x; i < md.nprior; i += 1024) a += md.prior[i];
     const double t = block_sum<1024>(a, sh);
     if (threadIdx.x == 0) *f = t;
+}
+
+__global__ __launch_bounds__(1024) void k_final(const MapDesc* __restrict__ md_, double* __restrict__ f)
+{
+    final_map(*md_, f);
+}
+
+// one block per map of the group
+__global__ __launch_bounds__(1024) void k_final_b(const MapDesc* __restrict__ mds, const BatchArgs a)
+{
+    final_map(mds[blockIdx.x], a.f + blockIdx.x);
 }
 
 // ================================================================== BLAS-1
@@ -2091,6 +2130,27 @@ int launch_ldiv(double* y, const double* x, const double* dl, int L, hipStream_t
     const long n = (long)(L + 1) * (L + 1);
     const int blocks = (int)std::min<long>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_ldiv, dim3(blocks), dim3(256), 0, s, y, x, dl, L);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_pre_batch(const MapDesc* d_mds, const BatchArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pre_b, dim3(a.tile0[a.n]), dim3(256), 0, s, d_mds, a);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_post_batch(const MapDesc* d_mds, const BatchArgs& a, int mode, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_post_b, dim3(a.tile0[a.n]), dim3(256), 0, s, d_mds, a, mode);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_final_batch(const MapDesc* d_mds, const BatchArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_final_b, dim3(a.n), dim3(1024), 0, s, d_mds, a);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }

@@ -1164,14 +1164,30 @@ int cmbl_batch_value_and_grad_device(cmbl_batch* b, const double* const* x, doub
     const int n = (int)b->maps.size();
     for (int i = 0; i < n; ++i) {
         if (!b->maps[i]->has_data) return fail(CMBL_ERR_STATE, "map without data in batch");
-        CMBL_TRY(launch_pre(b->maps[i]->desc, b->d_descs + i, x[i], s));
+        if (!x[i] || !g[i]) return fail(CMBL_ERR_ARG, "null pointer");
     }
+    // groups of up to kBatchGroup maps share one launch of k_pre, k_post, k_final
+    std::vector<BatchArgs> groups;
+    for (int g0 = 0; g0 < n; g0 += kBatchGroup) {
+        BatchArgs a{};
+        a.n = std::min(kBatchGroup, n - g0);
+        a.f = f + g0;
+        a.tile0[0] = 0;
+        for (int j = 0; j < a.n; ++j) {
+            a.x[j] = x[g0 + j];
+            a.g[j] = g[g0 + j];
+            a.tile0[j + 1] = a.tile0[j] + lm_tiles(b->maps[g0 + j]->L);
+        }
+        groups.push_back(a);
+    }
+    for (size_t q = 0; q < groups.size(); ++q)
+        CMBL_TRY(launch_pre_batch(b->d_descs + q * kBatchGroup, groups[q], s));
     for (int k = 0; k < 4; ++k) CMBL_TRY(launch_synth(1 << k, b->d_items1[k], b->n1[k], b->d_descs, s));
     CMBL_TRY(run_ring(kRingLike, b->d_ring_items, b->ring_classes, b->d_descs, nullptr, nullptr, b->d_tw, s));
     for (int k = 0; k < 4; ++k) CMBL_TRY(launch_adj(1 << k, b->d_items5[k], b->n5[k], b->d_descs, s));
-    for (int i = 0; i < n; ++i) {
-        CMBL_TRY(launch_post(b->maps[i]->desc, b->d_descs + i, x[i], g[i], kPostLike, s));
-        CMBL_TRY(launch_final(b->d_descs + i, f + i, s));
+    for (size_t q = 0; q < groups.size(); ++q) {
+        CMBL_TRY(launch_post_batch(b->d_descs + q * kBatchGroup, groups[q], kPostLike, s));
+        CMBL_TRY(launch_final_batch(b->d_descs + q * kBatchGroup, groups[q], s));
     }
     return CMBL_OK;
 }

@@ -41,6 +41,34 @@ def test_batch_uneven_matches_single_maps():
         mp.close()
 
 
+def test_batch_several_launch_groups():
+    """70 maps: the a_lm preparation, gradient assembly and reduction run in
+    launch groups of up to 64 maps; every map equals its own evaluation."""
+    import torch
+
+    print("case batch 70 maps", flush=True)
+    geoms = [(2, 4), (4, 8), (4, 12), (8, 16), (8, 24)]
+    cases = [Case(*geoms[i % 5], 10.0, 10.0 + (i % 7), 15.0 + (i % 5), seed=200 + i) for i in range(70)]
+    maps = [c.make_map() for c in cases]
+    dev = torch.device("cuda:0")
+    xs = [torch.from_numpy(c.x).to(dev) for c in cases]
+    gs = [torch.empty_like(x) for x in xs]
+    f = torch.empty(len(cases), dtype=torch.float64, device=dev)
+    b = cm.Batch(maps)
+    stream = torch.cuda.current_stream().cuda_stream
+    for rep in range(2):
+        b.value_and_grad_device([x.data_ptr() for x in xs], f.data_ptr(), [g.data_ptr() for g in gs], stream)
+        torch.cuda.synchronize()
+        fb = f.cpu().numpy()
+        for i, (c, mp) in enumerate(zip(cases, maps)):
+            fs, gsn = mp.value_and_grad(c.x)
+            assert abs(fb[i] - fs) <= 1e-13 * abs(fs)
+            assert rel(gs[i].cpu().numpy(), gsn) <= 1e-13
+    b.close()
+    for mp in maps:
+        mp.close()
+
+
 def test_torch_autograd_pair():
     import torch
 

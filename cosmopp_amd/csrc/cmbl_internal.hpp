@@ -146,7 +146,10 @@ constexpr double kScaleUp = 0x1p+1000;
 constexpr double kScaleLo = 0x1p-100;
 
 constexpr int kWave = 64;
-constexpr int kRingThreads = 512;
+constexpr int kRingThreads = 512;  // threads of a ring workgroup (largest)
+// Threads of a ring workgroup for transforms up to maxm: 256 up to 2048, so
+// that four such workgroups share a CU where two would leave most threads idle.
+constexpr int ring_threads(int maxm) { return maxm <= 2048 ? 256 : kRingThreads; }
 constexpr int kMaxFFT = 8192;          // largest LDS FFT (complex doubles)
 // LDS bytes of a sequence of n complex doubles in the ring stage's padded
 // layout (one spare slot per 16 values).

@@ -1429,13 +1429,14 @@ __device__ __forceinline__ constexpr int brev_c(int q)
 template <int R, int LR, int SIGN, int MAXM>
 __device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, const double2* __restrict__ tw)
 {
-    constexpr int B = (MAXM / R + kRingThreads - 1) / kRingThreads;  // butterflies per thread, at most
+    constexpr int NT = ring_threads(MAXM);
+    constexpr int B = (MAXM / R + NT - 1) / NT;  // butterflies per thread, at most
     const int tid = threadIdx.x;
     const int nb = M / R;
     double2 u[B][R];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-        const int i = tid + b * kRingThreads;
+        const int i = tid + b * NT;
         if (i < nb) {
 #pragma unroll
             for (int r = 0; r < R; ++r) u[b][r] = Z[zp(i + r * nb)];
@@ -1444,7 +1445,7 @@ __device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, const do
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-        const int i = tid + b * kRingThreads;
+        const int i = tid + b * NT;
         if (i < nb) {
             const int k = i & (p - 1);
             if (k) {
@@ -1628,6 +1629,7 @@ __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, boo
                            const double2* __restrict__ tw, double2* Z, double* sh)
 {
     const int L = md.L;
+    constexpr int NT = ring_threads(MAXM);
     const int tid = threadIdx.x;
     const int h = n >> 1;
     const double2* filt = md.bs_filt + md.ring_bsofs[p];
@@ -1638,7 +1640,7 @@ __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, boo
     if (MODE != kRingAna) {
         const double2* row = reinterpret_cast<const double2*>(md.P + (long)p * (L + 1) * 4);
         for (int r = 0; r < 2; ++r) {
-            for (int kk = tid; kk < h; kk += kRingThreads) {
+            for (int kk = tid; kk < h; kk += NT) {
                 const int k = 2 * kk + r;
                 double2 gN, gS;
                 fold_aliases(row, L, n, k, 0, 1, shifted, gN, gS);
@@ -1647,12 +1649,12 @@ __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, boo
             __syncthreads();
             bluestein<+1, MAXM>(Z, h, M, logM, chirp, filt, tw);
             if (r == 0) {
-                for (int j = tid; j < h; j += kRingThreads) SE[j] = Z[zp(j)];
+                for (int j = tid; j < h; j += NT) SE[j] = Z[zp(j)];
                 __syncthreads();
             }
         }
         double chi = 0.0;
-        for (int j = tid; j < h; j += kRingThreads) {
+        for (int j = tid; j < h; j += NT) {
             const double2 e = SE[j];
             const double2 o = cmul(Z[zp(j)], cis_pi_frac(2 * j, n));
             const double2 sv[2] = {cadd(e, o), csub(e, o)};
@@ -1681,7 +1683,7 @@ __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, boo
         }
         if (MODE == kRingSyn) return;
         if (MODE == kRingLike) {
-            const double t = block_sum<kRingThreads>(chi, sh);
+            const double t = block_sum<NT>(chi, sh);
             if (tid == 0) md.chi[p] = t;
         } else {
             __syncthreads();
@@ -1689,37 +1691,37 @@ __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, boo
     }
 
     for (int r = 0; r < 2; ++r) {
-        for (int j = tid; j < h; j += kRingThreads) {
+        for (int j = tid; j < h; j += NT) {
             const long t = 2 * j + r;
             Z[zp(j)] = (MODE == kRingAna) ? make_double2(map_in[oN + t], oS >= 0 ? map_in[oS + t] : 0.0) : SF[t];
         }
         __syncthreads();
         bluestein<-1, MAXM>(Z, h, M, logM, chirp, filt, tw);
         if (r == 0) {
-            for (int j = tid; j < h; j += kRingThreads) SE[j] = Z[zp(j)];
+            for (int j = tid; j < h; j += NT) SE[j] = Z[zp(j)];
             __syncthreads();
         }
     }
 
     double2* qrow = reinterpret_cast<double2*>(md.Q + (long)p * (L + 1) * 4);
-    for (int k = tid; k < n && k <= L; k += kRingThreads) {
+    for (int k = tid; k < n && k <= L; k += NT) {
         const int nk = (k == 0) ? 0 : n - k;
         unfold_bin(qrow, L, n, k, 0, 1, shifted, split_bin(SE, Z, k, n, h), split_bin(SE, Z, nk, n, h));
     }
 }
 
-// Classes up to M = 4096 fit two workgroups per CU in LDS: ask for the
-// registers that allow it (the second bound counts waves per SIMD: two
-// workgroups of kRingThreads over the CU's four SIMDs).
+// Ring kernels up to M = 4096 are held to four waves per SIMD (128 VGPRs):
+// two workgroups of 512 threads, or four of 256 up to M = 2048, per CU.
 template <int MODE, int MAXM>
-__global__ __launch_bounds__(kRingThreads, MAXM > 4096 ? 1 : 2 * kRingThreads / 256) void k_ring(const RingItem* __restrict__ items,
+__global__ __launch_bounds__(ring_threads(MAXM), MAXM > 4096 ? 1 : 4) void k_ring(const RingItem* __restrict__ items,
                                                        const MapDesc* __restrict__ maps,
                                                        double* __restrict__ map_out,
                                                        const double* __restrict__ map_in,
                                                        const double2* __restrict__ tw)
 {
+    constexpr int NT = ring_threads(MAXM);
     extern __shared__ double2 Z[];
-    __shared__ double sh[kRingThreads / 64];
+    __shared__ double sh[NT / 64];
     const RingItem it = items[blockIdx.x];
     const MapDesc& md = maps[it.map];
     const int L = md.L;
@@ -1739,15 +1741,15 @@ __global__ __launch_bounds__(kRingThreads, MAXM > 4096 ? 1 : 2 * kRingThreads / 
     const double2* filt = pow2 ? nullptr : md.bs_filt + md.ring_bsofs[p];
     const double2* chirp = pow2 ? nullptr : md.bs_chirp + md.ring_chofs[p];
     const int tid = threadIdx.x;
-    // for n < kRingThreads the aliases of each bin are split over R thread
+    // for n < NT the aliases of each bin are split over R thread
     // groups (reads stay coalesced over k), partial sums added in fixed order
-    const int R = n < kRingThreads ? kRingThreads / n : 1;
+    const int R = n < NT ? NT / n : 1;
 
     if (MODE != kRingAna) {
         const double2* row = reinterpret_cast<const double2*>(md.P + (long)p * (L + 1) * 4);
         // fold: Z_k = G_N,k + i G_S,k
         if (R == 1) {
-            for (int k = tid; k < n; k += kRingThreads) {
+            for (int k = tid; k < n; k += NT) {
                 double2 gN, gS;
                 fold_aliases(row, L, n, k, 0, 1, shifted, gN, gS);
                 fold_store(Z, k, k, n, shifted, oS < 0, gN, gS);
@@ -1776,7 +1778,7 @@ __global__ __launch_bounds__(kRingThreads, MAXM > 4096 ? 1 : 2 * kRingThreads / 
     }
 
     if (MODE == kRingSyn) {
-        for (int j = tid; j < n; j += kRingThreads) {
+        for (int j = tid; j < n; j += NT) {
             const double2 v = Z[zp(j)];
             map_out[oN + j] = v.x;
             if (oS >= 0) map_out[oS + j] = v.y;
@@ -1786,7 +1788,7 @@ __global__ __launch_bounds__(kRingThreads, MAXM > 4096 ? 1 : 2 * kRingThreads / 
 
     if (MODE == kRingLike) {
         double chi = 0.0;
-        for (int j = tid; j < n; j += kRingThreads) {
+        for (int j = tid; j < n; j += NT) {
             const double2 v = Z[zp(j)];
             const double rN = md.d[oN + j] - v.x;
             const double fN = md.w[oN + j] * rN;
@@ -1799,16 +1801,16 @@ __global__ __launch_bounds__(kRingThreads, MAXM > 4096 ? 1 : 2 * kRingThreads / 
             }
             Z[zp(j)] = make_double2(fN, fS);
         }
-        const double t = block_sum<kRingThreads>(chi, sh);
+        const double t = block_sum<NT>(chi, sh);
         if (tid == 0) md.chi[p] = t;
     } else if (MODE == kRingWeight) {
-        for (int j = tid; j < n; j += kRingThreads) {
+        for (int j = tid; j < n; j += NT) {
             const double2 v = Z[zp(j)];
             Z[zp(j)] = make_double2(md.w[oN + j] * v.x, oS >= 0 ? md.w[oS + j] * v.y : 0.0);
         }
         __syncthreads();
     } else {  // kRingAna: load the map to analyse
-        for (int j = tid; j < n; j += kRingThreads)
+        for (int j = tid; j < n; j += NT)
             Z[zp(j)] = make_double2(map_in[oN + j], oS >= 0 ? map_in[oS + j] : 0.0);
         __syncthreads();
     }
@@ -1816,7 +1818,7 @@ __global__ __launch_bounds__(kRingThreads, MAXM > 4096 ? 1 : 2 * kRingThreads / 
 
     // unfold, with the aliases of bin k split over R thread groups as in the fold
     double2* qrow = reinterpret_cast<double2*>(md.Q + (long)p * (L + 1) * 4);
-    for (int t = tid; t < R * n; t += kRingThreads) {
+    for (int t = tid; t < R * n; t += NT) {
         const int k = t % n, r = t / n;
         if (k > L) continue;
         const int nk = (k == 0) ? 0 : n - k;
@@ -2021,7 +2023,7 @@ static int ring_mm(const RingItem* items, int n, const MapDesc* d_maps, int lds,
                                      kMaxLds));
         attr_set = true;
     }
-    hipLaunchKernelGGL((k_ring<MODE, MAXM>), dim3(n), dim3(kRingThreads), lds, s, items, d_maps, out, in, tw);
+    hipLaunchKernelGGL((k_ring<MODE, MAXM>), dim3(n), dim3(ring_threads(MAXM)), lds, s, items, d_maps, out, in, tw);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }

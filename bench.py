@@ -270,6 +270,42 @@ def cpu_baseline(cfg, inputs, budget_s):
     }
 
 
+def cpu_table(args):
+    """The CPU oracle at configs 1, 2 and 4, on one thread and on every host
+    thread this process may use: bounded samples of full value + gradient
+    evaluations (the GPU only synthesises the input maps)."""
+    import os
+    import platform
+
+    import oracle
+
+    full = oracle.num_threads()
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    rows = []
+    for c in (1, 2, 4):
+        cfg = CONFIGS[c]
+        mp, inputs = build_case(cfg)
+        mp.close()
+        for threads in (full, 1):
+            oracle.set_threads(threads)
+            budget = args.cpu_budget if threads > 1 else 3 * args.cpu_budget
+            b = cpu_baseline(cfg, inputs, budget)
+            rows.append({"config": c, "nside": cfg["nside"], "lmax": cfg["lmax"], "threads": threads,
+                         "evals_per_s": b["value"], "sample": b["sample"]})
+            log("cpu table: config %d, %d threads: %.4g evals/s" % (c, threads, b["value"]))
+        oracle.set_threads(full)
+    print(json.dumps({"cpu_table": rows, "cpu_model": model, "nproc": os.cpu_count(),
+                      "threads_available": full}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,6 +316,8 @@ def main():
     ap.add_argument("--pol", action="store_true",
                     help="time the spin-2 synthesis (cmbl_alm2map_pol_device) at the config's nside/lmax")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-table", action="store_true",
+                    help="only time the CPU oracle at configs 1, 2, 4 on one and on all threads")
     ap.add_argument("--host-buffers", action="store_true",
                     help="time cmbl_value_and_grad with x and g in host memory (PCIe-inclusive rate)")
     ap.add_argument("--cpu-budget", type=float, default=40.0, help="seconds for the CPU baseline leg")
@@ -312,6 +350,9 @@ def main():
 
     import cosmopp_amd as cm
 
+    if args.cpu_table:
+        cpu_table(args)
+        return
     if args.config == 3:
         bench_batch(args)
         return

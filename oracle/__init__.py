@@ -53,6 +53,7 @@ def _load():
     lib.orc_mask_latitude.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
     lib.orc_mask_regions.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
     lib.orc_num_threads.restype = ctypes.c_int
+    lib.orc_set_threads.argtypes = [ctypes.c_int]
     return lib
 
 
@@ -161,6 +162,10 @@ def mask_regions(nside, theta, phi, angle, mask=None):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
 
 
 # ---------------------------------------------------------------- Gibbs CG system

@@ -78,6 +78,8 @@ void orc_mask_regions(int nside, int nreg, const double* theta, const double* ph
                       const double* angle, double* mask);
 
 int orc_num_threads(void);
+/* OpenMP threads of the calling thread's later parallel regions (timing). */
+void orc_set_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -315,6 +315,8 @@ def main():
                     help="workload: 4 (default, the headline), 1, 2, 5 (main map), 3 (64-map batch)")
     ap.add_argument("--pol", action="store_true",
                     help="time the spin-2 synthesis (cmbl_alm2map_pol_device) at the config's nside/lmax")
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the evaluation captured in a HIP graph and replayed (launch-bound sizes)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-table", action="store_true",
                     help="only time the CPU oracle at configs 1, 2, 4 on one and on all threads")
@@ -388,6 +390,25 @@ def main():
     for _ in range(args.warmup):
         mp.value_and_grad_device(x.data_ptr(), f.data_ptr(), g.data_ptr(), sp)
     torch.cuda.synchronize()
+
+    graph_rate = None
+    if args.graph:
+        # the same evaluation captured once in a HIP graph (on a side stream:
+        # capture needs a non-default stream) and replayed
+        side = torch.cuda.Stream()
+        side.wait_stream(stream)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=side):
+            mp.value_and_grad_device(x.data_ptr(), f.data_ptr(), g.data_ptr(), side.cuda_stream)
+        graph.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            graph.replay()
+        torch.cuda.synchronize()
+        graph_rate = args.steps / (time.perf_counter() - t0)
+        del graph
+        log("bench: graph replay %.1f evals/s" % graph_rate)
 
     mp.set_timing(True)
     log("bench: timing %d steps" % args.steps)
@@ -465,6 +486,8 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in stage_avg.items()},
         "neg2lnL": fval,
     }
+    if graph_rate is not None:
+        out["graph_replay_evals_per_s"] = round(graph_rate, 3)
     mp.close()
     if args.config == 5:
         log("bench: config 5 chain maps")

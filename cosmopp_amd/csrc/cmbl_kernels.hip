@@ -12,6 +12,7 @@
 //   k_adj     adjoint Legendre: u_lm = sum_rings lambda_lm Psi_m
 //   k_post    gradient assembly into the packed layout
 //   k_final   deterministic -2lnL reduction
+#include <atomic>
 #include <cstdlib>
 
 #include "cmbl_internal.hpp"
@@ -2013,16 +2014,25 @@ int launch_scan(int K, const WorkItem* items, int n, const MapDesc* d_maps, int*
     return fail(CMBL_ERR_ARG, "unsupported rings-per-lane K");
 }
 
+// Raise a kernel's dynamic-LDS limit to kMaxLds once per device (function
+// attributes belong to a device; one bit per device id in `done`).
+static int allow_dynamic_lds(const void* fn, std::atomic<unsigned long long>& done)
+{
+    int dev = 0;
+    CMBL_HIP(hipGetDevice(&dev));
+    const unsigned long long bit = 1ull << (dev & 63);
+    if (done.load(std::memory_order_acquire) & bit) return CMBL_OK;
+    CMBL_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
+    done.fetch_or(bit, std::memory_order_acq_rel);
+    return CMBL_OK;
+}
+
 template <int MODE, int MAXM>
 static int ring_mm(const RingItem* items, int n, const MapDesc* d_maps, int lds, double* out, const double* in,
                    const double2* tw, hipStream_t s)
 {
-    static bool attr_set = false;
-    if (!attr_set) {
-        CMBL_HIP(hipFuncSetAttribute((const void*)k_ring<MODE, MAXM>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     kMaxLds));
-        attr_set = true;
-    }
+    static std::atomic<unsigned long long> lds_set{0};
+    CMBL_TRY(allow_dynamic_lds((const void*)k_ring<MODE, MAXM>, lds_set));
     hipLaunchKernelGGL((k_ring<MODE, MAXM>), dim3(n), dim3(ring_threads(MAXM)), lds, s, items, d_maps, out, in, tw);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
@@ -2057,12 +2067,8 @@ int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_b
                            double2* d_chirp, int lds, const double2* tw, hipStream_t s)
 {
     if (npairs_bs <= 0) return CMBL_OK;
-    static bool attr_set = false;
-    if (!attr_set) {
-        CMBL_HIP(hipFuncSetAttribute((const void*)k_bluestein_setup, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     kMaxLds));
-        attr_set = true;
-    }
+    static std::atomic<unsigned long long> lds_set{0};
+    CMBL_TRY(allow_dynamic_lds((const void*)k_bluestein_setup, lds_set));
     hipLaunchKernelGGL(k_bluestein_setup, dim3(npairs_bs), dim3(kRingThreads), lds, s, d_pairs, d_md, d_out, d_chirp,
                        tw);
     CMBL_HIP(hipGetLastError());

@@ -59,15 +59,20 @@ def test_backward_matches_oracle(nside, lmax):
     mp.close()
 
 
-def test_split_rings_nside2048_match_oracle():
-    """nside 2048: the polar rings with nph = 4i > 4096 take the split transform
-    (two Bluestein halves of nph/2, one held in global scratch); the others the
-    unsplit paths.  Forward, backward and -2lnL + gradient against the oracle."""
-    nside, lmax = 2048, 96
-    print("case split rings nside=%d lmax=%d" % (nside, lmax), flush=True)
-    rng = np.random.default_rng(2048)
+@pytest.mark.parametrize("nside,lmax", [(2048, 96), (1024, 64), (256, 1024)])
+def test_large_geometries_match_oracle(nside, lmax):
+    """Polar rings whose Bluestein length would exceed 4096 (nph = 4i > 2048,
+    nside >= 1024) take the split transform (two Bluestein halves of nph/2, one
+    held in global scratch; at nside 2048 in a launch of their own), the others
+    the unsplit paths in one launch; nside 256 / lmax 1024 aliases every ring
+    in the 256-thread small-ring kernels.  Forward, backward and -2lnL +
+    gradient against the oracle."""
+    print("case large geometry nside=%d lmax=%d" % (nside, lmax), flush=True)
+    rng = np.random.default_rng(nside + lmax)
     n = (lmax + 1) ** 2
-    b = cm.beam(lmax, 20.0)
+    # B_lmax = 0.1: every l carries weight in the comparison
+    b = cm.beam(lmax, math.degrees(math.sqrt(8.0 * math.log(2.0)) * math.sqrt(2.0 * math.log(10.0) / (lmax * (lmax + 1.0)))))
+    assert 0.09 < b[lmax] < 0.11
     mp = cm.Map(nside, lmax)
     mp.set_beam(b)
     x = rng.standard_normal(n)

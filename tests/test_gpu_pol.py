@@ -45,11 +45,12 @@ def test_pol_matches_oracle(nside, L):
     mp.close()
 
 
-def test_pol_split_rings_nside2048():
-    """Polar rings with 2 nph - 1 > 8192 take the split ring transforms."""
-    nside, L = 2048, 48
+@pytest.mark.parametrize("nside,L", [(2048, 48), (1024, 64)])
+def test_pol_split_rings(nside, L):
+    """Polar rings whose Bluestein length would exceed 4096 take the split ring
+    transforms (at nside 2048 also in a launch of their own)."""
     print("case pol split nside=%d L=%d" % (nside, L), flush=True)
-    rng = np.random.default_rng(2048)
+    rng = np.random.default_rng(nside + L)
     t, e, b = spectrum_scaled(rng, L)
     mp = cm.Map(nside, L)
     gt, gq, gu = mp.alm2map_pol(t, e, b)

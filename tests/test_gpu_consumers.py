@@ -13,6 +13,7 @@ import json
 import math
 import os
 import subprocess
+import tempfile
 
 import numpy as np
 import pytest
@@ -32,9 +33,10 @@ NSIDE, LMAX = 8, 16
 
 def run(mode, iters):
     print("case consumers %s nside=%d lmax=%d iters=%d" % (mode, NSIDE, LMAX, iters), flush=True)
-    r = subprocess.run([BIN, mode, str(NSIDE), str(LMAX), str(iters)], stdout=subprocess.PIPE,
-                       stderr=subprocess.PIPE, text=True, timeout=240, cwd=os.path.join(ROOT, "gpurun_out")
-                       if os.path.isdir(os.path.join(ROOT, "gpurun_out")) else None)
+    # the HMC modes write their chain file into the working directory
+    with tempfile.TemporaryDirectory() as cwd:
+        r = subprocess.run([BIN, mode, str(NSIDE), str(LMAX), str(iters)], stdout=subprocess.PIPE,
+                           stderr=subprocess.PIPE, text=True, timeout=240, cwd=cwd)
     assert r.returncode == 0, r.stderr[-2000:]
     line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
     return json.loads(line)

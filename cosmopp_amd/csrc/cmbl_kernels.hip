@@ -913,10 +913,76 @@ __global__ __launch_bounds__(64) void k_synth_pol(const WorkItem* __restrict__ i
 // in LDS; every kRedChunk steps the wave reduces the 64 lanes and writes the
 // ring-block partial u_lm (re, im) with one coalesced store.
 
+// A lane's ring-stage inputs Psi_m (even and odd parts) of its K ring pairs.
+// Lanes whose scale count is > 0 hold zeros, so that the same four-operation
+// step serves the scaled and the live path: their scaled mu (|mu| < 2^945
+// between rescale checks) times 0 contributes exactly 0.  A lane that turns
+// live loads its inputs at the rescale check where its count reaches 0.
 template <int K>
 struct AdjIn {
     double per[K], pei[K], por[K], poi[K];
 };
+
+// Where the inputs of lane `lane`'s ring pair pbase + 64 k + lane live:
+// Q + p * row + 4 m (row = 4 (L + 1) doubles per ring pair); pairs past npair
+// (padding) have none.
+struct AdjSrc {
+    const double* Q;
+    long row;
+    int m, pbase, npair, lane;
+};
+
+template <int K>
+__device__ __forceinline__ void adj_load_in(AdjIn<K>& in, int k, const AdjSrc& s)
+{
+    const int p = s.pbase + k * 64 + s.lane;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    if (p < s.npair) {
+        gcdouble* src = (gcdouble*)(s.Q + (long)p * s.row + 4 * (long)s.m);
+        v0 = src[0];
+        v1 = src[1];
+        v2 = src[2];
+        v3 = src[3];
+    }
+    // one unconditional store per field: stores in both arms of a branch
+    // would be merged through a pointer and keep the inputs in memory
+    in.per[k] = v0;
+    in.pei[k] = v1;
+    in.por[k] = v2;
+    in.poi[k] = v3;
+}
+
+// Rescale check of the adjoint's scaled path: lanes above 2^900 drop by
+// 2^-1000; a lane whose count reaches 0 loads its inputs.
+template <int K>
+__device__ __forceinline__ void adj_rescale(LegState<K>& st, AdjIn<K>& in, const AdjSrc& s)
+{
+    int need = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) need |= (st.sc[k] > 0) & (fabs(st.mu1[k]) > kScaleHi);
+    if (!__any(need)) return;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (st.sc[k] > 0 && fabs(st.mu1[k]) > kScaleHi) {
+            st.mu1[k] *= kScaleDn;
+            st.mu2[k] *= kScaleDn;
+            st.sc[k] -= 1;
+            if (st.sc[k] == 0) adj_load_in<K>(in, k, s);
+        }
+    }
+    st.all = all_live<K>(st.sc);
+}
+
+// Psi (re, im) of parity o of ring pair k.  All four inputs are read and the
+// values selected: selecting array elements, or merging the even and odd
+// variants of a step, would keep the inputs in addressable memory.
+template <int K>
+__device__ __forceinline__ void adj_psi(const AdjIn<K>& in, int k, bool o, double& pr, double& pi)
+{
+    const double er = in.per[k], ei = in.pei[k], orr = in.por[k], oi = in.poi[k];
+    pr = o ? orr : er;
+    pi = o ? oi : ei;
+}
 
 
 // Lanes 0..31 get a summed over lanes (j, j + 32), lanes 32..63 get b summed
@@ -1023,23 +1089,25 @@ __device__ __forceinline__ void adj_live(LegState<K>& st, const AdjIn<K>& in, Ad
         const double nu = fma(Al * st.x[k], st.mu1[k], -st.mu2[k]);
         st.mu2[k] = st.mu1[k];
         st.mu1[k] = nu;
-        sr = fma(nu, ODD ? in.por[k] : in.per[k], sr);
-        si = fma(nu, ODD ? in.poi[k] : in.pei[k], si);
+        double pr, pi;
+        adj_psi<K>(in, k, ODD, pr, pi);
+        sr = fma(nu, pr, sr);
+        si = fma(nu, pi, si);
     }
     rd.push(sr, si);
 }
 
+// The item's first l (the recurrence state is already at it).
 template <int K, bool ODD>
-__device__ __forceinline__ void adj_gated(const LegState<K>& st, const AdjIn<K>& in, AdjRed& rd)
+__device__ __forceinline__ void adj_first(const LegState<K>& st, const AdjIn<K>& in, AdjRed& rd)
 {
     double sr = 0.0, si = 0.0;
-    if (st.live) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const double f = st.gate[k] * st.mu1[k];
-            sr = fma(f, ODD ? in.por[k] : in.per[k], sr);
-            si = fma(f, ODD ? in.poi[k] : in.pei[k], si);
-        }
+    for (int k = 0; k < K; ++k) {
+        double pr, pi;
+        adj_psi<K>(in, k, ODD, pr, pi);
+        sr = fma(st.mu1[k], pr, sr);
+        si = fma(st.mu1[k], pi, si);
     }
     rd.push(sr, si);
 }
@@ -1056,8 +1124,10 @@ __device__ __forceinline__ void adj_step_v(LegState<K>& st, const AdjIn<K>& in, 
         const double nu = fma(Al * st.x[k], st.mu1[k], -st.mu2[k]);
         st.mu2[k] = st.mu1[k];
         st.mu1[k] = nu;
-        sr = fma(nu, o ? in.por[k] : in.per[k], sr);
-        si = fma(nu, o ? in.poi[k] : in.pei[k], si);
+        double pr, pi;
+        adj_psi<K>(in, k, o, pr, pi);
+        sr = fma(nu, pr, sr);
+        si = fma(nu, pi, si);
     }
 }
 
@@ -1070,69 +1140,54 @@ __device__ __forceinline__ void adj_step(LegState<K>& st, const AdjIn<K>& in, Ad
         adj_live<K, false>(st, in, rd, Al);
 }
 
-// One gated step returning its lane sums: negligible lanes contribute gate 0.
-template <int K>
-__device__ __forceinline__ void adj_step_gv(LegState<K>& st, const AdjIn<K>& in, bool o, double Al, double& sr,
-                                            double& si)
+// Steps l..cend of one staged chunk while some lane is still scaled: single
+// steps up to a row boundary (rescale check after each), then rows of
+// kRedChunk = 8 steps reduced like the live rows with one rescale check per
+// row, then the rest of the chunk one step at a time (checked while some lane
+// is scaled).  Eight steps multiply |mu| by at most prod (A_l + 1) < 2^45
+// (A_{m+1} = sqrt(2m + 2) is the largest factor), so from 2^900 no value
+// overflows between checks; a lane whose scale count reaches 0 inside a row
+// joins at the row's end, dropping terms below 2^-55 of O(1).
+template <int K, bool ODD0>
+__device__ __forceinline__ void adj_scaled_rows(LegState<K>& st, AdjIn<K>& in, AdjRed& rd, int& l, int cend, int cb,
+                                                const double* cA, const AdjSrc& s)
 {
-    st.advance(Al);
-    sr = 0.0;
-    si = 0.0;
+    for (; !st.all && l + 7 <= cend; l += 8) {
+        double v0[8], v1[8], z[4], t4[4];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const double f = st.gate[k] * st.mu1[k];
-        sr = fma(f, o ? in.por[k] : in.per[k], sr);
-        si = fma(f, o ? in.poi[k] : in.pei[k], si);
+        for (int q = 0; q < 4; ++q)
+            adj_step_v<K>(st, in, ODD0 ^ (q & 1), cA[l + q - cb], v0[2 * q], v0[2 * q + 1]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            adj_step_v<K>(st, in, ODD0 ^ (q & 1), cA[l + 4 + q - cb], v1[2 * q], v1[2 * q + 1]);
+        rd.fold_row(v0, v1, z);
+        rd.tile_store(z);
+        rd.tile_load(t4);
+        rd.finish_row(t4);
+        wave_sync();
+        adj_rescale<K>(st, in, s);
     }
 }
 
-// Steps l..cend of one staged chunk while some lane is still scaled: single
-// gated steps up to a row boundary (rescale check after each), then rows of
-// kRedChunk = 8 gated steps reduced like the live rows with one rescale check
-// per row, then the rest of the chunk one step at a time (gated while some
-// lane is scaled).  Eight steps multiply |mu| by at most prod (A_l + 1)
-// < 2^45 (A_{m+1} = sqrt(2m + 2) is the largest factor), so from 2^900 no
-// value overflows between checks; a lane whose scale count reaches 0 inside
-// a row joins at the row's end, dropping terms below 2^-55 of O(1).
 template <int K>
-__device__ __forceinline__ void adj_chunk_scaled(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, int& l, int cend,
-                                                 int cb, int m, const double* cA)
+__device__ __forceinline__ void adj_chunk_scaled(LegState<K>& st, AdjIn<K>& in, AdjRed& rd, int& l, int cend,
+                                                 int cb, int m, const double* cA, const AdjSrc& s)
 {
     while (!st.all && rd.cnt != 0 && l <= cend) {
-        st.advance(cA[l - cb]);
-        if ((l - m) & 1)
-            adj_gated<K, true>(st, in, rd);
-        else
-            adj_gated<K, false>(st, in, rd);
+        adj_step<K>(st, in, rd, l, m, cA[l - cb]);
         ++l;
-        if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
+        adj_rescale<K>(st, in, s);
     }
     if (!st.all && l + 7 <= cend) {
-        const bool odd = ((l - m) & 1) != 0;
-        for (; !st.all && l + 7 <= cend; l += 8) {
-            double v0[8], v1[8], z[4], t4[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                adj_step_gv<K>(st, in, odd ^ (q & 1), cA[l + q - cb], v0[2 * q], v0[2 * q + 1]);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                adj_step_gv<K>(st, in, odd ^ (q & 1), cA[l + 4 + q - cb], v1[2 * q], v1[2 * q + 1]);
-            rd.fold_row(v0, v1, z);
-            rd.tile_store(z);
-            rd.tile_load(t4);
-            rd.finish_row(t4);
-            wave_sync();
-            if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
-        }
+        if ((l - m) & 1)
+            adj_scaled_rows<K, true>(st, in, rd, l, cend, cb, cA, s);
+        else
+            adj_scaled_rows<K, false>(st, in, rd, l, cend, cb, cA, s);
     }
     while (!st.all && l <= cend) {
-        st.advance(cA[l - cb]);
-        if ((l - m) & 1)
-            adj_gated<K, true>(st, in, rd);
-        else
-            adj_gated<K, false>(st, in, rd);
+        adj_step<K>(st, in, rd, l, m, cA[l - cb]);
         ++l;
-        if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
+        adj_rescale<K>(st, in, s);
     }
     for (; l <= cend; ++l) adj_step<K>(st, in, rd, l, m, cA[l - cb]);
 }
@@ -1188,8 +1243,8 @@ __device__ __forceinline__ void adj_chunk_live(LegState<K>& st, const AdjIn<K>& 
 }
 
 // K = 4 (the adjoint's ring pairs per lane from nside 512 up) is held to
-// three waves per SIMD (168 VGPRs): the compiler's own choice of 202 leaves
-// two, and the spills the bound costs sit outside the step loops.
+// three waves per SIMD (168 VGPRs): the compiler's own choice of 174 leaves
+// two, and the 4 VGPRs the bound spills are reloaded outside the step loops.
 template <int K>
 __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
 {
@@ -1208,27 +1263,33 @@ __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __re
 
     LegState<K> st;
     AdjIn<K> in;
-    int p[K];
+    AdjSrc src;
+    src.Q = md.Q;
+    src.row = 4 * (long)(L + 1);
+    src.m = m;
+    src.pbase = it.rb * 64 * K;
+    src.npair = md.npair;
+    src.lane = lane;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        p[k] = it.rb * 64 * K + k * 64 + lane;
-        const long q = (long)m * npad + p[k];
-        st.x[k] = md.ring_x[p[k]];
+        const int p = src.pbase + k * 64 + lane;
+        const long q = (long)m * npad + p;
+        st.x[k] = md.ring_x[p];
         st.mu1[k] = md.s5_cur[q];
         st.mu2[k] = md.s5_prev[q];
         st.sc[k] = md.s5_k[q];
-        if (p[k] < md.npair) {
-            const double2* src = reinterpret_cast<const double2*>(md.Q + ((long)p[k] * (L + 1) + m) * 4);
-            const double2 e = src[0], o = src[1];
-            in.per[k] = e.x;
-            in.pei[k] = e.y;
-            in.por[k] = o.x;
-            in.poi[k] = o.y;
-        } else {
-            in.per[k] = in.pei[k] = in.por[k] = in.poi[k] = 0.0;
-        }
+        adj_load_in<K>(in, k, src);
     }
-    st.refresh();
+    // lanes still scaled start from zero inputs
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const bool z = st.sc[k] != 0;
+        in.per[k] = z ? 0.0 : in.per[k];
+        in.pei[k] = z ? 0.0 : in.pei[k];
+        in.por[k] = z ? 0.0 : in.por[k];
+        in.poi[k] = z ? 0.0 : in.poi[k];
+    }
+    st.all = all_live<K>(st.sc);
     AdjRed rd;
     rd.red = red;
     rd.mine = red + lane * kRedStride;
@@ -1237,9 +1298,9 @@ __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __re
     rd.lbase = l0;
     rd.lane = lane;
     if ((l0 - m) & 1)
-        adj_gated<K, true>(st, in, rd);
+        adj_first<K, true>(st, in, rd);
     else
-        adj_gated<K, false>(st, in, rd);
+        adj_first<K, false>(st, in, rd);
 
     int cb = l0 + 1;
     double tA = (cb + lane <= L) ? A[cb + lane] : 0.0;
@@ -1251,7 +1312,7 @@ __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __re
     while (cb <= L && !st.all) {
         const int nb = cb + kChunk;
         if (nb + lane <= L) tA = A[nb + lane];
-        adj_chunk_scaled<K>(st, in, rd, l, min(L, nb - 1), cb, m, sA[buf]);
+        adj_chunk_scaled<K>(st, in, rd, l, min(L, nb - 1), cb, m, sA[buf], src);
         if (nb <= L) sA[buf ^ 1][lane] = tA;
         wave_sync();
         buf ^= 1;

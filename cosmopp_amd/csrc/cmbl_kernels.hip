@@ -459,21 +459,50 @@ __device__ __forceinline__ void syn_live(LegState<K>& st, SynAcc<K>& ac, double 
     }
 }
 
+// Scaled lanes (count > 0) accumulate like live ones: their scaled mu
+// (|mu| < 2^945 between rescale checks) makes meaningless sums, which are
+// cleared when the lane turns live (its count reaching 0 at a rescale check)
+// and never stored while it is scaled, so the result equals gating each term
+// by the lane's state at the last check, without the multiplies.
+
+// The item's first l (the recurrence state is already at it).
 template <int K, bool ODD>
-__device__ __forceinline__ void syn_gated(LegState<K>& st, SynAcc<K>& ac, double ar, double ai)
+__device__ __forceinline__ void syn_first(const LegState<K>& st, SynAcc<K>& ac, double ar, double ai)
 {
-    if (!st.live) return;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const double f = st.gate[k] * st.mu1[k];
         if (ODD) {
-            ac.orr[k] = fma(f, ar, ac.orr[k]);
-            ac.oi[k] = fma(f, ai, ac.oi[k]);
+            ac.orr[k] = fma(st.mu1[k], ar, ac.orr[k]);
+            ac.oi[k] = fma(st.mu1[k], ai, ac.oi[k]);
         } else {
-            ac.er[k] = fma(f, ar, ac.er[k]);
-            ac.ei[k] = fma(f, ai, ac.ei[k]);
+            ac.er[k] = fma(st.mu1[k], ar, ac.er[k]);
+            ac.ei[k] = fma(st.mu1[k], ai, ac.ei[k]);
         }
     }
+}
+
+// Rescale check of the synthesis' scaled path: lanes above 2^900 drop by
+// 2^-1000; a lane whose count reaches 0 clears its sums.
+template <int K>
+__device__ __forceinline__ void syn_rescale(LegState<K>& st, SynAcc<K>& ac)
+{
+    int need = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) need |= (st.sc[k] > 0) & (fabs(st.mu1[k]) > kScaleHi);
+    if (!__any(need)) return;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const bool r = st.sc[k] > 0 && fabs(st.mu1[k]) > kScaleHi;
+        st.mu1[k] = r ? st.mu1[k] * kScaleDn : st.mu1[k];
+        st.mu2[k] = r ? st.mu2[k] * kScaleDn : st.mu2[k];
+        st.sc[k] -= r ? 1 : 0;
+        const bool join = r && st.sc[k] == 0;
+        ac.er[k] = join ? 0.0 : ac.er[k];
+        ac.ei[k] = join ? 0.0 : ac.ei[k];
+        ac.orr[k] = join ? 0.0 : ac.orr[k];
+        ac.oi[k] = join ? 0.0 : ac.oi[k];
+    }
+    st.all = all_live<K>(st.sc);
 }
 
 // Four live steps with coefficients c (A), r/i (B_l N_lm a_lm), parity
@@ -535,61 +564,51 @@ __device__ __forceinline__ void syn_live_run(LegState<K>& st, SynAcc<K>& ac, int
     }
 }
 
-// Rows of eight gated steps (parity pattern ODD0, !ODD0, ...): negligible lanes
-// contribute gate 0, with no branch between the accumulator updates, and one
-// rescale check per row.  Eight steps multiply |mu| by at most
-// prod (A_l + 1) < 2^45 (A_{m+1} = sqrt(2m + 2) is the largest factor), so
-// from 2^900 nothing overflows between checks; a lane whose scale count
+// Rows of eight steps (parity pattern ODD0, !ODD0, ...) while some lane is
+// scaled, with one rescale check per row.  Eight steps multiply |mu| by at
+// most prod (A_l + 1) < 2^45 (A_{m+1} = sqrt(2m + 2) is the largest factor),
+// so from 2^900 nothing overflows between checks; a lane whose scale count
 // reaches 0 inside a row joins at the row's end, dropping terms below 2^-55
 // of O(1).
 template <int K, bool ODD0>
-__device__ __forceinline__ void syn_gated_rows(LegState<K>& st, SynAcc<K>& ac, int& l, int cend, int cb,
-                                               const double* cA, const double2* ca)
+__device__ __forceinline__ void syn_scaled_rows(LegState<K>& st, SynAcc<K>& ac, int& l, int cend, int cb,
+                                                const double* cA, const double2* ca)
 {
     for (; !st.all && l + 7 <= cend; l += 8) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const double2 al = ca[l + q - cb];
-            st.advance(cA[l + q - cb]);
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const double f = st.gate[k] * st.mu1[k];
-                if (ODD0 ^ (q & 1)) {
-                    ac.orr[k] = fma(f, al.x, ac.orr[k]);
-                    ac.oi[k] = fma(f, al.y, ac.oi[k]);
-                } else {
-                    ac.er[k] = fma(f, al.x, ac.er[k]);
-                    ac.ei[k] = fma(f, al.y, ac.ei[k]);
-                }
-            }
+            if (ODD0 ^ (q & 1))
+                syn_live<K, true>(st, ac, cA[l + q - cb], al.x, al.y);
+            else
+                syn_live<K, false>(st, ac, cA[l + q - cb], al.x, al.y);
         }
-        if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
+        syn_rescale<K>(st, ac);
     }
 }
 
 // Steps l = l .. cend of one staged chunk (coefficients cA[l - cb], ca[l - cb]).
-// Scaled path: gated rows while eight steps remain, then single gated steps
-// with a rescale check after every second step.
+// Scaled path: rows while eight steps remain, then single steps with a
+// rescale check after every second step.
 template <int K>
 __device__ __forceinline__ void syn_chunk(LegState<K>& st, SynAcc<K>& ac, int& l, int cend, int cb, int m,
                                           const double* cA, const double2* ca)
 {
     if (!st.all && l + 7 <= cend) {
         if ((l - m) & 1)
-            syn_gated_rows<K, true>(st, ac, l, cend, cb, cA, ca);
+            syn_scaled_rows<K, true>(st, ac, l, cend, cb, cA, ca);
         else
-            syn_gated_rows<K, false>(st, ac, l, cend, cb, cA, ca);
+            syn_scaled_rows<K, false>(st, ac, l, cend, cb, cA, ca);
     }
     while (!st.all && l <= cend) {
         for (int rep = 0; rep < 2 && l <= cend; ++rep, ++l) {
             const double2 al = ca[l - cb];
-            st.advance(cA[l - cb]);
             if ((l - m) & 1)
-                syn_gated<K, true>(st, ac, al.x, al.y);
+                syn_live<K, true>(st, ac, cA[l - cb], al.x, al.y);
             else
-                syn_gated<K, false>(st, ac, al.x, al.y);
+                syn_live<K, false>(st, ac, cA[l - cb], al.x, al.y);
         }
-        if (rescale<K>(st.mu1, st.mu2, st.sc)) st.refresh();
+        syn_rescale<K>(st, ac);
     }
     if (l > cend) return;
     // all lanes live: parity pattern fixed by (l - m)
@@ -636,13 +655,13 @@ __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items
         st.sc[k] = md.s1_k[q];
         ac.er[k] = ac.ei[k] = ac.orr[k] = ac.oi[k] = 0.0;
     }
-    st.refresh();
+    st.all = all_live<K>(st.sc);
     {
         const double ar = a[2 * l0], ai = a[2 * l0 + 1];
         if ((l0 - m) & 1)
-            syn_gated<K, true>(st, ac, ar, ai);
+            syn_first<K, true>(st, ac, ar, ai);
         else
-            syn_gated<K, false>(st, ac, ar, ai);
+            syn_first<K, false>(st, ac, ar, ai);
     }
     // staged chunks of steps l0+1 .. L
     int cb = l0 + 1;
@@ -673,12 +692,14 @@ __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items
         buf ^= 1;
         cb = nb;
     }
+    // lanes still scaled at L store zeros
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (p[k] < md.npair) {
+            const bool z = st.sc[k] != 0;
             double2* dst = reinterpret_cast<double2*>(md.P + ((long)p[k] * (L + 1) + m) * 4);
-            dst[0] = make_double2(ac.er[k], ac.ei[k]);
-            dst[1] = make_double2(ac.orr[k], ac.oi[k]);
+            dst[0] = make_double2(z ? 0.0 : ac.er[k], z ? 0.0 : ac.ei[k]);
+            dst[1] = make_double2(z ? 0.0 : ac.orr[k], z ? 0.0 : ac.oi[k]);
         }
     }
 }

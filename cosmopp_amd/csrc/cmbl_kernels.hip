@@ -1030,6 +1030,17 @@ __device__ __forceinline__ double swap16_add(double a, double b)
     return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
 }
 
+// s plus s of lane j ^ 8 (DPP row_ror:8 rotates each row of 16 lanes by 8).
+__device__ __forceinline__ double ror8_add(double s)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(s), 0x128, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(s), 0x128, 0xF, 0xF, false);
+    return s + __hiloint2double(hi, lo);
+}
+
+// the row tile (records 33 q + u, q < 8, u < 32) fits in the lane rows
+static_assert(2 * (33 * 7 + 32) <= 64 * kRedStride, "row tile exceeds the reduction buffer");
+
 struct AdjRed {
     double* red;   // [64 * kRedStride]
     double* mine;  // this lane's row
@@ -1057,46 +1068,47 @@ struct AdjRed {
         if (++cnt == kRedChunk) flush();
     }
     // Rows of kRedChunk = 8 steps held in registers, software-pipelined over
-    // the next row's steps: fold_row(v0, v1, z) folds a row to 16 columns with
-    // two swap-and-add stages (registers only); tile_store puts the column
-    // partials in the LDS tile, tile_load fetches a lane's four partials back,
-    // finish_row sums and stores the row.  v0 = values 0..7, v1 = values 8..15
-    // (value 2q + c is step q, c = 0 real / 1 imaginary).  The tile holds lane
-    // L's 4 values at 4 L + 4 (L >> 4) (the skew spreads a 16-lane read over
-    // distinct banks).
-    __device__ __forceinline__ void fold_row(const double (&v0)[8], const double (&v1)[8], double (&z)[4]) const
+    // the next row's steps.  fold_row(v0, v1, z) folds a row once over lane
+    // pairs (j, j + 32) with permlane32 swaps: lanes 0..31 then hold steps
+    // 0..3 and lanes 32..63 steps 4..7 (v0 = values 0..7, v1 = values 8..15;
+    // value 2q + c is step q, c = 0 real / 1 imaginary).  tile_store puts the
+    // four (re, im) records of lane L as records (q, L & 31), index 33 q +
+    // (L & 31); tile_load gives lane j the records (j & 7, 4 (j >> 3) + r),
+    // r < 4 (the skew of one record per q puts the eight lanes of a read on
+    // distinct banks); finish_row sums them and completes the sum over the
+    // eight lanes that share j & 7 (distances 32, 16, 8): the real part ends
+    // in lanes 0..31, the imaginary part in lanes 32..63.  A permlane swap
+    // costs about two FP64 issue slots, so the tile carries the second fold:
+    // 20 swaps per row instead of 28, for 4 KB of LDS traffic each way.
+    __device__ __forceinline__ void fold_row(const double (&v0)[8], const double (&v1)[8], double (&z)[8]) const
     {
-        double w[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) w[i] = swap32_add(v0[i], v1[i]);   // value i + 8 (lane >> 5)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) z[i] = swap16_add(w[i], w[i + 4]);  // value i + 4 (lane >> 4)
+        for (int i = 0; i < 8; ++i) z[i] = swap32_add(v0[i], v1[i]);
     }
-    __device__ __forceinline__ void tile_store(const double (&z)[4])
+    __device__ __forceinline__ void tile_store(const double (&z)[8])
     {
-        double* t = red + 4 * lane + 4 * (lane >> 4);
+        double2* t = reinterpret_cast<double2*>(red) + 33 * 4 * (lane >> 5) + (lane & 31);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) t[i] = z[i];
+        for (int q = 0; q < 4; ++q) t[33 * q] = make_double2(z[2 * q], z[2 * q + 1]);
         wave_sync();
     }
-    __device__ __forceinline__ void tile_load(double (&t4)[4]) const
+    __device__ __forceinline__ void tile_load(double2 (&t4)[4]) const
     {
-        const int o = lane & 15, part = lane >> 4, r = o >> 2, i = o & 3;
+        const double2* t = reinterpret_cast<const double2*>(red) + 33 * (lane & 7) + 4 * (lane >> 3);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int src = 16 * r + 4 * part + c;
-            t4[c] = red[4 * src + 4 * (src >> 4) + i];
-        }
+        for (int r = 0; r < 4; ++r) t4[r] = t[r];
     }
-    __device__ __forceinline__ void finish_row(const double (&t4)[4])
+    __device__ __forceinline__ void finish_row(const double2 (&t4)[4])
     {
-        double s = (t4[0] + t4[1]) + (t4[2] + t4[3]);
-        s = swap32_add(s, s);
+        double re = (t4[0].x + t4[1].x) + (t4[2].x + t4[3].x);
+        double im = (t4[0].y + t4[1].y) + (t4[2].y + t4[3].y);
+        double s = swap32_add(re, im);
         s = swap16_add(s, s);
-        // every lane holds the sum of column lane & 15: the four copies store
-        // the same value (no branch, so the store stays in the loop's block)
+        s = ror8_add(s);
+        // four lanes hold each value and store it (no branch, so the store
+        // stays in the loop's block)
         gdouble* g = (gdouble*)dst;
-        g[2 * lbase + (lane & 15)] = s;
+        g[2 * lbase + 2 * (lane & 7) + (lane >> 5)] = s;
         lbase += kRedChunk;
     }
 };
@@ -1174,7 +1186,8 @@ __device__ __forceinline__ void adj_scaled_rows(LegState<K>& st, AdjIn<K>& in, A
                                                 const double* cA, const AdjSrc& s)
 {
     for (; !st.all && l + 7 <= cend; l += 8) {
-        double v0[8], v1[8], z[4], t4[4];
+        double v0[8], v1[8], z[8];
+        double2 t4[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             adj_step_v<K>(st, in, ODD0 ^ (q & 1), cA[l + q - cb], v0[2 * q], v0[2 * q + 1]);
@@ -1225,7 +1238,8 @@ __device__ __forceinline__ void adj_chunk_live(LegState<K>& st, const AdjIn<K>& 
     for (; rd.cnt != 0 && l <= cend; ++l) adj_step<K>(st, in, rd, l, m, cA[l - cb]);
     if (l + 7 <= cend) {
         const bool odd = ((l - m) & 1) != 0;
-        double x[4], y[4], z[4], t4[4];
+        double x[4], y[4], z[8];
+        double2 t4[4];
         bool pend = false;  // z holds a folded row not yet stored
 #pragma unroll
         for (int q = 0; q < 4; ++q) x[q] = cA[l - cb + q];
@@ -1269,7 +1283,7 @@ __device__ __forceinline__ void adj_chunk_live(LegState<K>& st, const AdjIn<K>& 
 template <int K>
 __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
 {
-    __shared__ double red[64 * kRedStride];
+    __shared__ __align__(16) double red[64 * kRedStride];
     __shared__ double sA[2][kChunk + kChunkPad];
     const WorkItem it = items[blockIdx.x];
     if (it.mcnt == 0) return;

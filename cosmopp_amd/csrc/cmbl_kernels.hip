@@ -1067,29 +1067,29 @@ struct AdjRed {
         mine[2 * cnt + 1] = si;
         if (++cnt == kRedChunk) flush();
     }
-    // Rows of kRedChunk = 8 steps held in registers, software-pipelined over
-    // the next row's steps.  fold_row(v0, v1, z) folds a row once over lane
-    // pairs (j, j + 32) with permlane32 swaps: lanes 0..31 then hold steps
-    // 0..3 and lanes 32..63 steps 4..7 (v0 = values 0..7, v1 = values 8..15;
-    // value 2q + c is step q, c = 0 real / 1 imaginary).  tile_store puts the
-    // four (re, im) records of lane L as records (q, L & 31), index 33 q +
-    // (L & 31); tile_load gives lane j the records (j & 7, 4 (j >> 3) + r),
-    // r < 4 (the skew of one record per q puts the eight lanes of a read on
-    // distinct banks); finish_row sums them and completes the sum over the
-    // eight lanes that share j & 7 (distances 32, 16, 8): the real part ends
-    // in lanes 0..31, the imaginary part in lanes 32..63.  A permlane swap
-    // costs about two FP64 issue slots, so the tile carries the second fold:
-    // 20 swaps per row instead of 28, for 4 KB of LDS traffic each way.
-    __device__ __forceinline__ void fold_row(const double (&v0)[8], const double (&v1)[8], double (&z)[8]) const
+    // Rows of kRedChunk = 8 steps.  Steps 2p and 2p + 1 are folded as soon as
+    // both exist, once over lane pairs (j, j + 32) with permlane32 swaps
+    // (fold_pair): z[2p + c] then holds step 2p in lanes 0..31 and step 2p + 1
+    // in lanes 32..63 (c = 0 real, 1 imaginary).  tile_store puts lane L's four
+    // (re, im) records as records (q, L & 31), q = 2p + (L >> 5), at index
+    // 33 q + (L & 31); tile_load gives lane j the records (j & 7, 4 (j >> 3)
+    // + r), r < 4 (the skew of one record per q puts the eight lanes of a read
+    // on distinct banks); finish_row sums them and completes the sum over the
+    // eight lanes that share j & 7 (distances 32, 16, 8): the real part ends in
+    // lanes 0..31, the imaginary part in lanes 32..63.  A permlane swap costs
+    // about two FP64 issue slots, so the tile carries the second fold: 20 swaps
+    // per row instead of 28, for 4 KB of LDS traffic each way.
+    __device__ __forceinline__ static void fold_pair(double are, double aim, double bre, double bim, double& z0,
+                                                     double& z1)
     {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) z[i] = swap32_add(v0[i], v1[i]);
+        z0 = swap32_add(are, bre);
+        z1 = swap32_add(aim, bim);
     }
     __device__ __forceinline__ void tile_store(const double (&z)[8])
     {
-        double2* t = reinterpret_cast<double2*>(red) + 33 * 4 * (lane >> 5) + (lane & 31);
+        double2* t = reinterpret_cast<double2*>(red) + 33 * (lane >> 5) + (lane & 31);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) t[33 * q] = make_double2(z[2 * q], z[2 * q + 1]);
+        for (int p = 0; p < 4; ++p) t[66 * p] = make_double2(z[2 * p], z[2 * p + 1]);
         wave_sync();
     }
     __device__ __forceinline__ void tile_load(double2 (&t4)[4]) const
@@ -1186,15 +1186,15 @@ __device__ __forceinline__ void adj_scaled_rows(LegState<K>& st, AdjIn<K>& in, A
                                                 const double* cA, const AdjSrc& s)
 {
     for (; !st.all && l + 7 <= cend; l += 8) {
-        double v0[8], v1[8], z[8];
+        double z[8];
         double2 t4[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            adj_step_v<K>(st, in, ODD0 ^ (q & 1), cA[l + q - cb], v0[2 * q], v0[2 * q + 1]);
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            adj_step_v<K>(st, in, ODD0 ^ (q & 1), cA[l + 4 + q - cb], v1[2 * q], v1[2 * q + 1]);
-        rd.fold_row(v0, v1, z);
+        for (int p = 0; p < 4; ++p) {
+            double are, aim, bre, bim;
+            adj_step_v<K>(st, in, ODD0, cA[l + 2 * p - cb], are, aim);
+            adj_step_v<K>(st, in, !ODD0, cA[l + 2 * p + 1 - cb], bre, bim);
+            AdjRed::fold_pair(are, aim, bre, bim, z[2 * p], z[2 * p + 1]);
+        }
         rd.tile_store(z);
         rd.tile_load(t4);
         rd.finish_row(t4);
@@ -1227,10 +1227,12 @@ __device__ __forceinline__ void adj_chunk_scaled(LegState<K>& st, AdjIn<K>& in, 
 }
 
 // Steps l..cend of one staged chunk with every lane live: single steps up to
-// the next reduction row boundary, then rows of kRedChunk = 8 steps held in
-// registers and reduced by reduce_row, with two 4-coefficient register sets
-// alternating (the speculative load of the next set may read up to 4 entries
-// past the chunk, into the row padding).
+// the next reduction row boundary, then rows of kRedChunk = 8 steps with two
+// 4-coefficient register sets alternating (the speculative load of the next
+// set may read up to 4 entries past the chunk, into the row padding).  A
+// row's tile is stored at its end and read back after the next row's first
+// step, and the sum finishes halfway through that row, so the LDS round trip
+// overlaps FP64 work.
 template <int K>
 __device__ __forceinline__ void adj_chunk_live(LegState<K>& st, const AdjIn<K>& in, AdjRed& rd, int& l, int cend,
                                                int cb, int m, const double* cA)
@@ -1238,38 +1240,40 @@ __device__ __forceinline__ void adj_chunk_live(LegState<K>& st, const AdjIn<K>& 
     for (; rd.cnt != 0 && l <= cend; ++l) adj_step<K>(st, in, rd, l, m, cA[l - cb]);
     if (l + 7 <= cend) {
         const bool odd = ((l - m) & 1) != 0;
-        double x[4], y[4], z[8];
+        double x[4], y[4];
         double2 t4[4];
-        bool pend = false;  // z holds a folded row not yet stored
+        bool pend = false;  // the tile holds a row not yet summed
 #pragma unroll
         for (int q = 0; q < 4; ++q) x[q] = cA[l - cb + q];
         for (; l + 7 <= cend; l += 8) {
-            double v0[8], v1[8];
+            double z[8], are, aim, bre, bim;
             // each set's loads are issued after the first step that uses the
             // other set, so the wait before that step covers only LDS traffic
-            // issued half an iteration earlier; the previous row's tile round
-            // trip rides along with the Y loads
-            adj_step_v<K>(st, in, odd, x[0], v0[0], v0[1]);
+            // issued half an iteration earlier; the previous row's tile read
+            // rides along with the Y loads
+            adj_step_v<K>(st, in, odd, x[0], are, aim);
             __builtin_amdgcn_sched_barrier(0);
-            if (pend) {
-                rd.tile_store(z);
-                rd.tile_load(t4);
-            }
+            if (pend) rd.tile_load(t4);
 #pragma unroll
             for (int q = 0; q < 4; ++q) y[q] = cA[l - cb + 4 + q];
-#pragma unroll
-            for (int q = 1; q < 4; ++q) adj_step_v<K>(st, in, odd ^ (q & 1), x[q], v0[2 * q], v0[2 * q + 1]);
-            adj_step_v<K>(st, in, odd, y[0], v1[0], v1[1]);
+            adj_step_v<K>(st, in, !odd, x[1], bre, bim);
+            AdjRed::fold_pair(are, aim, bre, bim, z[0], z[1]);
+            adj_step_v<K>(st, in, odd, x[2], are, aim);
+            adj_step_v<K>(st, in, !odd, x[3], bre, bim);
+            AdjRed::fold_pair(are, aim, bre, bim, z[2], z[3]);
+            if (pend) rd.finish_row(t4);
+            adj_step_v<K>(st, in, odd, y[0], are, aim);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < 4; ++q) x[q] = cA[l - cb + 8 + q];
-#pragma unroll
-            for (int q = 1; q < 4; ++q) adj_step_v<K>(st, in, odd ^ (q & 1), y[q], v1[2 * q], v1[2 * q + 1]);
-            if (pend) rd.finish_row(t4);
-            rd.fold_row(v0, v1, z);
+            adj_step_v<K>(st, in, !odd, y[1], bre, bim);
+            AdjRed::fold_pair(are, aim, bre, bim, z[4], z[5]);
+            adj_step_v<K>(st, in, odd, y[2], are, aim);
+            adj_step_v<K>(st, in, !odd, y[3], bre, bim);
+            AdjRed::fold_pair(are, aim, bre, bim, z[6], z[7]);
+            rd.tile_store(z);
             pend = true;
         }
-        rd.tile_store(z);
         rd.tile_load(t4);
         rd.finish_row(t4);
         wave_sync();
@@ -1277,9 +1281,9 @@ __device__ __forceinline__ void adj_chunk_live(LegState<K>& st, const AdjIn<K>& 
     for (; l <= cend; ++l) adj_step<K>(st, in, rd, l, m, cA[l - cb]);
 }
 
-// K = 4 (the adjoint's ring pairs per lane from nside 512 up) is held to
-// three waves per SIMD (168 VGPRs): the compiler's own choice of 174 leaves
-// two, and the 4 VGPRs the bound spills are reloaded outside the step loops.
+// K = 4 (the adjoint's ring pairs per lane from nside 512 up) runs three
+// waves per SIMD: it needs 166 VGPRs (168 fit three waves); a four-wave bound
+// (128) spills around the step loops and measured slower.
 template <int K>
 __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
 {

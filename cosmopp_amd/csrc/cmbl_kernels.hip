@@ -1030,11 +1030,13 @@ __device__ __forceinline__ double swap16_add(double a, double b)
     return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
 }
 
-// s plus s of lane j ^ 8 (DPP row_ror:8 rotates each row of 16 lanes by 8).
-__device__ __forceinline__ double ror8_add(double s)
+// s plus s rotated by N lanes within each row of 16 (DPP row_ror:N); for
+// N = 8 that is s plus s of lane j ^ 8.
+template <int N>
+__device__ __forceinline__ double ror_add(double s)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(s), 0x128, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(s), 0x128, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(s), 0x120 + N, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(s), 0x120 + N, 0xF, 0xF, false);
     return s + __hiloint2double(hi, lo);
 }
 
@@ -1104,12 +1106,28 @@ struct AdjRed {
         double im = (t4[0].y + t4[1].y) + (t4[2].y + t4[3].y);
         double s = swap32_add(re, im);
         s = swap16_add(s, s);
-        s = ror8_add(s);
+        s = ror_add<8>(s);
         // four lanes hold each value and store it (no branch, so the store
         // stays in the loop's block)
         gdouble* g = (gdouble*)dst;
         g[2 * lbase + 2 * (lane & 7) + (lane >> 5)] = s;
         lbase += kRedChunk;
+    }
+    // One l on its own (the item's first): the whole-wave sum of (sr, si),
+    // the real part in lanes 0..31 and the imaginary part in lanes 32..63, all
+    // in registers, stored at lbase.  Keeping it out of the push buffer lets
+    // the staged chunks (64 steps from the next l) fall on row boundaries.
+    __device__ __forceinline__ void store_one(double sr, double si)
+    {
+        double s = swap32_add(sr, si);
+        s = swap16_add(s, s);
+        s = ror_add<8>(s);
+        s = ror_add<4>(s);
+        s = ror_add<2>(s);
+        s = ror_add<1>(s);
+        gdouble* g = (gdouble*)dst;
+        g[2 * lbase + (lane >> 5)] = s;
+        lbase += 1;
     }
 };
 
@@ -1142,7 +1160,7 @@ __device__ __forceinline__ void adj_first(const LegState<K>& st, const AdjIn<K>&
         sr = fma(st.mu1[k], pr, sr);
         si = fma(st.mu1[k], pi, si);
     }
-    rd.push(sr, si);
+    rd.store_one(sr, si);
 }
 
 // One live step returning its (real, imaginary) lane sums; o = parity of l - m.

@@ -930,9 +930,11 @@ __global__ __launch_bounds__(64) void k_synth_pol(const WorkItem* __restrict__ i
 }
 
 // -------------------------------------------------------------- adjoint
-// Per l-step each lane forms sum_k mu_l(p_k) Psi^{parity}(p_k) and parks it
-// in LDS; every kRedChunk steps the wave reduces the 64 lanes and writes the
-// ring-block partial u_lm (re, im) with one coalesced store.
+// Per l-step each lane forms sum_k mu_l(p_k) Psi^{parity}(p_k); every
+// kRedChunk steps the wave reduces the 64 lanes (register folds and an LDS
+// tile) and writes the ring-block partial u_lm (re, im) with one coalesced
+// store.  Steps outside whole rows (the item's last partial row) go through a
+// per-lane LDS buffer reduced when it holds kRedChunk steps or at the end.
 
 // A lane's ring-stage inputs Psi_m (even and odd parts) of its K ring pairs.
 // Lanes whose scale count is > 0 hold zeros, so that the same four-operation

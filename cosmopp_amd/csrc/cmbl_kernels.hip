@@ -180,8 +180,22 @@ __device__ __forceinline__ void post_tile(const MapDesc& md, const double* __res
         double ur = 0.0, ui = 0.0;
         if (l <= L && m <= l) {
             const long k = moff(L, m) + (l - m);
-            for (int rb = 0; rb < md.nrb5; ++rb) {
-                const double2 v = reinterpret_cast<const double2*>(md.upart + 2 * (long)rb * md.nalm)[k];
+            const double2* u = reinterpret_cast<const double2*>(md.upart) + k;
+            const long stride = md.nalm;  // double2 per ring block
+            int rb = 0;
+            // eight partials in flight per thread, added in ring-block order
+            for (; rb + 8 <= md.nrb5; rb += 8) {
+                double2 v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = u[(rb + j) * stride];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    ur += v[j].x;
+                    ui += v[j].y;
+                }
+            }
+            for (; rb < md.nrb5; ++rb) {
+                const double2 v = u[rb * stride];
                 ur += v.x;
                 ui += v.y;
             }

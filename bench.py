@@ -341,9 +341,9 @@ def main():
         return
 
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box in this setup has a single MI355X.  Several GPUs "
+        print("bench.py measures one MI355X only.  Several GPUs "
               "would take whole maps each (independent maps, LPT by Legendre cost) or m-slices of one map "
-              "with two all-to-all transposes around the ring stage; that layout is not run here.  "
+              "with two all-to-all transposes around the ring stage; that layout is not implemented.  "
               "Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 

@@ -159,6 +159,7 @@ constexpr int kRedChunk = 8;           // l-steps per cross-lane reduction in th
 constexpr int kRedStride = 17;         // doubles per lane row in the reduction buffer
 constexpr int kChunk = 64;             // l-steps per LDS-staged coefficient chunk
 constexpr int kChunkPad = 8;           // padding for speculative reads past a chunk
+constexpr int kAdjChunk = 2 * kChunk;  // l-steps per staged chunk in the adjoint
 
 struct MapDesc {
     int nside, L, npair, npair_pad, nring;

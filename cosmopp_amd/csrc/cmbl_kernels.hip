@@ -1322,7 +1322,7 @@ template <int K>
 __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
 {
     __shared__ __align__(16) double red[64 * kRedStride];
-    __shared__ double sA[2][kChunk + kChunkPad];
+    __shared__ double sA[2][kAdjChunk + kChunkPad];
     const WorkItem it = items[blockIdx.x];
     if (it.mcnt == 0) return;
     const MapDesc& md = maps[it.map];
@@ -1375,27 +1375,40 @@ __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __re
     else
         adj_first<K, false>(st, in, rd);
 
+    // staged chunks of kAdjChunk = 128 steps, two coefficients per lane (each
+    // chunk ends with one unpipelined row reduction, so longer chunks drain
+    // the pipeline less often)
     int cb = l0 + 1;
-    double tA = (cb + lane <= L) ? A[cb + lane] : 0.0;
-    sA[0][lane] = tA;
+    double tA0 = (cb + lane <= L) ? A[cb + lane] : 0.0;
+    double tA1 = (cb + 64 + lane <= L) ? A[cb + 64 + lane] : 0.0;
+    sA[0][lane] = tA0;
+    sA[0][64 + lane] = tA1;
     wave_sync();
     int buf = 0;
     int l = cb;
     // two loops so that the scaled-path state is dead in the second
     while (cb <= L && !st.all) {
-        const int nb = cb + kChunk;
-        if (nb + lane <= L) tA = A[nb + lane];
+        const int nb = cb + kAdjChunk;
+        if (nb + lane <= L) tA0 = A[nb + lane];
+        if (nb + 64 + lane <= L) tA1 = A[nb + 64 + lane];
         adj_chunk_scaled<K>(st, in, rd, l, min(L, nb - 1), cb, m, sA[buf], src);
-        if (nb <= L) sA[buf ^ 1][lane] = tA;
+        if (nb <= L) {
+            sA[buf ^ 1][lane] = tA0;
+            sA[buf ^ 1][64 + lane] = tA1;
+        }
         wave_sync();
         buf ^= 1;
         cb = nb;
     }
     while (cb <= L) {
-        const int nb = cb + kChunk;
-        if (nb + lane <= L) tA = A[nb + lane];
+        const int nb = cb + kAdjChunk;
+        if (nb + lane <= L) tA0 = A[nb + lane];
+        if (nb + 64 + lane <= L) tA1 = A[nb + 64 + lane];
         adj_chunk_live<K>(st, in, rd, l, min(L, nb - 1), cb, m, sA[buf]);
-        if (nb <= L) sA[buf ^ 1][lane] = tA;
+        if (nb <= L) {
+            sA[buf ^ 1][lane] = tA0;
+            sA[buf ^ 1][64 + lane] = tA1;
+        }
         wave_sync();
         buf ^= 1;
         cb = nb;

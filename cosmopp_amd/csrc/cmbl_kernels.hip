@@ -1210,8 +1210,8 @@ __device__ __forceinline__ void adj_step(LegState<K>& st, const AdjIn<K>& in, Ad
 // Steps l..cend of one staged chunk while some lane is still scaled: single
 // steps up to a row boundary (rescale check after each), then rows of
 // kRedChunk = 8 steps reduced like the live rows with one rescale check per
-// row, then the rest of the chunk one step at a time (checked while some lane
-// is scaled).  Eight steps multiply |mu| by at most prod (A_l + 1) < 2^45
+// row, then (at the item's end) the rest one step at a time.  It returns once
+// every lane is live; the caller finishes the chunk with live rows.  Eight steps multiply |mu| by at most prod (A_l + 1) < 2^45
 // (A_{m+1} = sqrt(2m + 2) is the largest factor), so from 2^900 no value
 // overflows between checks; a lane whose scale count reaches 0 inside a row
 // joins at the row's end, dropping terms below 2^-55 of O(1).
@@ -1257,7 +1257,6 @@ __device__ __forceinline__ void adj_chunk_scaled(LegState<K>& st, AdjIn<K>& in, 
         ++l;
         adj_rescale<K>(st, in, s);
     }
-    for (; l <= cend; ++l) adj_step<K>(st, in, rd, l, m, cA[l - cb]);
 }
 
 // Steps l..cend of one staged chunk with every lane live: single steps up to
@@ -1392,6 +1391,8 @@ __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __re
         if (nb + lane <= L) tA0 = A[nb + lane];
         if (nb + 64 + lane <= L) tA1 = A[nb + 64 + lane];
         adj_chunk_scaled<K>(st, in, rd, l, min(L, nb - 1), cb, m, sA[buf], src);
+        // every lane turned live inside the chunk: its rest runs as live rows
+        if (st.all && l <= min(L, nb - 1)) adj_chunk_live<K>(st, in, rd, l, min(L, nb - 1), cb, m, sA[buf]);
         if (nb <= L) {
             sA[buf ^ 1][lane] = tA0;
             sA[buf ^ 1][64 + lane] = tA1;

@@ -160,6 +160,7 @@ constexpr int kRedStride = 17;         // doubles per lane row in the reduction 
 constexpr int kChunk = 64;             // l-steps per LDS-staged coefficient chunk
 constexpr int kChunkPad = 8;           // padding for speculative reads past a chunk
 constexpr int kAdjChunk = 2 * kChunk;  // l-steps per staged chunk in the adjoint
+constexpr int kSynChunk = 2 * kChunk;  // l-steps per staged chunk in the synthesis
 
 struct MapDesc {
     int nside, L, npair, npair_pad, nring;

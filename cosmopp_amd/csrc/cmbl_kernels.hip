@@ -642,8 +642,9 @@ __device__ __forceinline__ void syn_chunk(LegState<K>& st, SynAcc<K>& ac, int& l
 template <int K>
 __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps)
 {
-    __shared__ double sA[2][kChunk + kChunkPad];
-    __shared__ double2 sa[2][kChunk + kChunkPad];
+    constexpr int CH = kSynChunk, NC = kSynChunk / 64;
+    __shared__ double sA[2][CH + kChunkPad];
+    __shared__ double2 sa[2][CH + kChunkPad];
     const WorkItem it = items[blockIdx.x];
     if (it.mcnt == 0) return;
     const MapDesc& md = maps[it.map];
@@ -677,30 +678,43 @@ __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items
         else
             syn_first<K, false>(st, ac, ar, ai);
     }
-    // staged chunks of steps l0+1 .. L
+    // staged chunks of CH steps from l0+1 to L, NC coefficients per lane
     int cb = l0 + 1;
-    double tA = 0.0;
-    double2 ta = make_double2(0.0, 0.0);
-    if (cb + lane <= L) {
-        tA = A[cb + lane];
-        ta = make_double2(a[2 * (cb + lane)], a[2 * (cb + lane) + 1]);
+    double tA[NC];
+    double2 ta[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int j = cb + 64 * c + lane;
+        tA[c] = 0.0;
+        ta[c] = make_double2(0.0, 0.0);
+        if (j <= L) {
+            tA[c] = A[j];
+            ta[c] = make_double2(a[2 * j], a[2 * j + 1]);
+        }
+        sA[0][64 * c + lane] = tA[c];
+        sa[0][64 * c + lane] = ta[c];
     }
-    sA[0][lane] = tA;
-    sa[0][lane] = ta;
     wave_sync();
     int buf = 0;
     int l = cb;
     while (cb <= L) {
-        const int nb = cb + kChunk;
-        if (nb + lane <= L) {
-            tA = A[nb + lane];
-            ta = make_double2(a[2 * (nb + lane)], a[2 * (nb + lane) + 1]);
+        const int nb = cb + CH;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int j = nb + 64 * c + lane;
+            if (j <= L) {
+                tA[c] = A[j];
+                ta[c] = make_double2(a[2 * j], a[2 * j + 1]);
+            }
         }
         const int cend = min(L, nb - 1);
         syn_chunk<K>(st, ac, l, cend, cb, m, sA[buf], sa[buf]);
         if (nb <= L) {
-            sA[buf ^ 1][lane] = tA;
-            sa[buf ^ 1][lane] = ta;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                sA[buf ^ 1][64 * c + lane] = tA[c];
+                sa[buf ^ 1][64 * c + lane] = ta[c];
+            }
         }
         wave_sync();
         buf ^= 1;

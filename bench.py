@@ -29,28 +29,9 @@ METRIC = "likelihood+gradient evals/sec at Nside=1024,lmax=2048 FP64; FP64 roofl
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
-CONFIGS = {
-    1: dict(nside=64, lmax=128, fwhm_deg=2.0, sigma=10.0, cut_deg=20.0),
-    2: dict(nside=512, lmax=1024, fwhm_deg=20.0 / 60.0, sigma=30.0, cut_deg=20.0),
-    4: dict(nside=1024, lmax=2048, fwhm_deg=10.0 / 60.0, sigma=20.0, cut_deg=20.0),
-    5: dict(nside=2048, lmax=4096, fwhm_deg=5.0 / 60.0, sigma=15.0, cut_deg=20.0),
-}
+from cases import CONFIG1, CONFIG2, CONFIG4, CONFIG5, CONFIG5_CHAIN, batch_configs, device_case  # noqa: E402
 
-
-def batch_configs(nmaps=64, seed=3):
-    """64 maps of uneven size: lmax ~ U{256, 512, 768, 1024, 1536}, nside =
-    2^round(log2(lmax/2)), FWHM = 5400/lmax arcmin, sigma ~ U[10, 50] uK,
-    cut ~ U[10, 30] deg, independent seeds."""
-    import numpy as np
-
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(nmaps):
-        lmax = int(rng.choice([256, 512, 768, 1024, 1536]))
-        nside = 1 << int(round(math.log2(lmax / 2)))
-        out.append(dict(nside=nside, lmax=lmax, fwhm_deg=5400.0 / lmax / 60.0, sigma=float(rng.uniform(10, 50)),
-                        cut_deg=float(rng.uniform(10, 30)), seed=1000 + 10 * i))
-    return out
+CONFIGS = {1: CONFIG1, 2: CONFIG2, 4: CONFIG4, 5: CONFIG5}
 
 
 def log(msg):
@@ -59,24 +40,7 @@ def log(msg):
 
 def build_case(cfg, seed=101):
     """Inputs of the workload; the data map is synthesised on the GPU."""
-    import numpy as np
-
-    import cosmopp_amd as cm
-
-    nside, lmax = cfg["nside"], cfg["lmax"]
-    cl = cm.cl_shape(lmax)
-    b = cm.beam(lmax, cfg["fwhm_deg"])
-    a_true = cm.simulate_alm(cl, lmax, seed)
-    mp = cm.Map(nside, lmax)
-    mp.set_beam(b)
-    mp.set_cl(cl)
-    s = mp.forward(a_true)
-    d = s + cm.simulate_white_noise(nside, cfg["sigma"], seed + 1)
-    mask = cm.mask_latitude(nside, math.radians(cfg["cut_deg"]))
-    w = cm.weights_from_mask(mask, cfg["sigma"])
-    mp.set_data(d, w)
-    x = a_true + 0.1 * cm.simulate_alm(cl, lmax, seed + 2)
-    return mp, dict(cl=cl, beam=b, d=d, w=w, x=x)
+    return device_case(cfg, seed)
 
 
 def bench_batch(args):
@@ -153,7 +117,7 @@ def bench_c5_chains(args):
 
     import cosmopp_amd as cm
 
-    cfg = dict(nside=256, lmax=512, fwhm_deg=40.0 / 60.0, sigma=40.0, cut_deg=20.0)
+    cfg = CONFIG5_CHAIN
     dev = torch.device("cuda:0")
     maps, xs, gs = [], [], []
     for i in range(16):
@@ -231,9 +195,15 @@ def bench_pol(args, cfg):
     print(json.dumps(out), flush=True)
 
 
+TRAFFIC_FILE = "profiles/pmc_traffic.json"
+
+
 def read_traffic(nside, lmax, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if any."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE passes, calibrated on a plain copy), if
+    that file holds this kernel and workload; the counters need a profiler run
+    of their own, so the bench line reads them rather than measuring them."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
     if not os.path.exists(path):
         return None
     try:
@@ -244,8 +214,9 @@ def read_traffic(nside, lmax, kernel):
         return None
 
 
-def cpu_baseline(cfg, inputs, budget_s):
-    """Time the oracle (CPU FP64 restatement) on the same workload."""
+def cpu_baseline(cfg, inputs, budget_s, result=None):
+    """Time the oracle (CPU FP64 restatement) on the same workload.  When
+    `result` is a dict, the oracle's last (-2lnL, gradient) go into it."""
     import oracle
 
     nside, lmax = cfg["nside"], cfg["lmax"]
@@ -254,8 +225,10 @@ def cpu_baseline(cfg, inputs, budget_s):
     t_start = time.time()
     while True:
         t0 = time.perf_counter()
-        oracle.neg2lnL(nside, lmax, inputs["x"], inputs["beam"], inputs["cl"], inputs["d"], inputs["w"])
+        fo, go = oracle.neg2lnL(nside, lmax, inputs["x"], inputs["beam"], inputs["cl"], inputs["d"], inputs["w"])
         times.append(time.perf_counter() - t0)
+        if result is not None:
+            result["f"], result["g"] = fo, go
         if time.time() - t_start + times[-1] > budget_s or len(times) >= 5:
             break
     times.sort()
@@ -436,27 +409,46 @@ def main():
     t_dom = stage_avg[dom] * 1e-3
     achieved = flop_per_launch / t_dom / 1e12
     kname = "k_adj" if dom == "adjoint" else "k_synth"
+    traffic = read_traffic(nside, lmax, kname)
     roofline = {
-        "bound": "mfma",
-        "compute": "FP64 VALU (MI355X FP64 vector peak equals its FP64 matrix peak, 78.6 TFLOP/s)",
+        "bound": "fp64_valu",
+        "compute": "FP64 VALU (no MFMA; MI355X FP64 vector peak equals its FP64 matrix peak, 78.6 TFLOP/s)",
         "kernel": "%s (Legendre %s)" % (kname, dom),
         "achieved": round(achieved, 3),
         "peak": FP64_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-        "traffic": read_traffic(nside, lmax, kname),
+        "traffic": traffic,
+        "traffic_source": TRAFFIC_FILE if traffic is not None else None,
         "flop_per_launch": flop_per_launch,
         "avg_launch_ms": round(stage_avg[dom], 4),
         "legendre_pair_frac": round(2 * flop_per_launch / ((stage_avg["adjoint"] + stage_avg["synthesis"]) * 1e-3)
                                     / 1e12 / FP64_PEAK_TFLOPS, 4),
     }
-    ring_bytes = 32.0 * (4 * nside - 1) * (lmax + 1) * 2 + 16.0 * 12 * nside * nside
+    # ring stage (HBM-bound): algorithmic bytes 32 Nring (L+1) (parity sums
+    # in, ring transforms out, 16 B per (ring, m) each way) + 16 Npix (d, w)
+    ring_bytes = 32.0 * (4 * nside - 1) * (lmax + 1) + 16.0 * 12 * nside * nside
+    roofline["ring_stage_bytes"] = ring_bytes
     roofline["ring_stage_hbm_frac"] = round(ring_bytes / (stage_avg["ring"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    ring_traffic = read_traffic(nside, lmax, "k_ring")
+    if ring_traffic is not None:
+        roofline["ring_traffic"] = ring_traffic
+        roofline["ring_traffic_ratio"] = round(ring_traffic / ring_bytes, 3)
 
     cpu = None
+    parity = None
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (oracle)")
-        cpu = cpu_baseline(cfg, inputs, args.cpu_budget)
+        ref = {}
+        cpu = cpu_baseline(cfg, inputs, args.cpu_budget, ref)
+        # the oracle's -2lnL and gradient at the same x against this run's
+        import numpy as np
+
+        gd = g.cpu().numpy()
+        parity = {"f_rel": abs(fval - ref["f"]) / abs(ref["f"]),
+                  "g_rel": float(np.linalg.norm(gd - ref["g"]) / np.linalg.norm(ref["g"])),
+                  "tolerance": 1e-10}
+        log("bench: parity vs oracle f_rel %.3e g_rel %.3e" % (parity["f_rel"], parity["g_rel"]))
 
     out = {
         "metric": METRIC,
@@ -483,6 +475,7 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity": parity,
         "stages_ms": {k: round(v, 4) for k, v in stage_avg.items()},
         "neg2lnL": fval,
     }

@@ -45,6 +45,8 @@ def _load():
     lib.orc_fft.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int]
     lib.orc_dft.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int]
     lib.orc_synthesis.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_void_p, _dp]
+    lib.orc_synthesis_rings.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_void_p, ctypes.c_int,
+                                        np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), _dp]
     lib.orc_alm2map_pol.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.orc_adjoint.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_void_p, _dp]
     lib.orc_neg2lnL.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_void_p]
@@ -113,6 +115,17 @@ def synthesis(nside, L, x, beam=None):
     x = np.ascontiguousarray(x, dtype=np.float64)
     b = None if beam is None else np.ascontiguousarray(beam, dtype=np.float64)
     lib().orc_synthesis(nside, L, x, _ptr(b), out)
+    return out
+
+
+def synthesis_rings(nside, L, x, rings, beam=None):
+    """Synthesis on the listed rings only: the pixels of each ring in turn."""
+    rings = np.ascontiguousarray(rings, dtype=np.int32)
+    total = sum(ring_info(nside, int(r))["nph"] for r in rings)
+    out = np.zeros(total)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    b = None if beam is None else np.ascontiguousarray(beam, dtype=np.float64)
+    lib().orc_synthesis_rings(nside, L, x, _ptr(b), len(rings), rings, out)
     return out
 
 

@@ -366,9 +366,37 @@ static void phase(long m, const orc_ring* r, double* c, double* s)
 /* m-major complex coefficients B_l a_lm: index off(m) + (l - m) */
 static long moff(int L, int m) { return (long)m * (L + 1) - (long)m * (m - 1) / 2; }
 
-/* Real ring values from the per-ring coefficients F[ring][m], m = 0..L:
+/* Real values of ring r from its coefficients F_m, m = 0..L:
  * f(phi_j) = F_0 + 2 Re sum_{m>=1} F_m e^{i m phi_j} (the real-field sum over
- * +-m), by folding m onto the ring's nph bins and one DFT per ring. */
+ * +-m), by folding m onto the ring's nph bins and one DFT. */
+static void ring_from_F(int nside, int L, int r, const double* Fr, const double* Fi, double* out)
+{
+    orc_ring rg;
+    orc_ring_info(nside, r, &rg);
+    const long n = rg.nph;
+    double* gr = (double*)calloc(n, sizeof(double));
+    double* gi = (double*)calloc(n, sizeof(double));
+    for (long m = 0; m <= L; ++m) {
+        double c, s;
+        phase(m, &rg, &c, &s);
+        const double fr = Fr[m], fi = Fi[m];
+        const double pr = fr * c - fi * s, pi = fr * s + fi * c;
+        const long b = m % n;
+        gr[b] += pr;
+        gi[b] += pi;
+        if (m > 0) {
+            const long b2 = ((-m) % n + n) % n;
+            gr[b2] += pr;
+            gi[b2] -= pi;
+        }
+    }
+    orc_fft((int)n, gr, gi, +1);
+    for (long j = 0; j < n; ++j) out[j] = gr[j];
+    free(gr);
+    free(gi);
+}
+
+/* Every ring of the map from F[ring][m]. */
 static void rings_from_F(int nside, int L, const double* Fr, const double* Fi, double* map)
 {
     const int nring = orc_nring(nside);
@@ -376,37 +404,13 @@ static void rings_from_F(int nside, int L, const double* Fr, const double* Fi, d
     for (int r = 0; r < nring; ++r) {
         orc_ring rg;
         orc_ring_info(nside, r, &rg);
-        const long n = rg.nph;
-        double* gr = (double*)calloc(n, sizeof(double));
-        double* gi = (double*)calloc(n, sizeof(double));
-        for (long m = 0; m <= L; ++m) {
-            double c, s;
-            phase(m, &rg, &c, &s);
-            const double fr = Fr[(size_t)r * (L + 1) + m], fi = Fi[(size_t)r * (L + 1) + m];
-            const double pr = fr * c - fi * s, pi = fr * s + fi * c;
-            const long b = m % n;
-            gr[b] += pr;
-            gi[b] += pi;
-            if (m > 0) {
-                const long b2 = ((-m) % n + n) % n;
-                gr[b2] += pr;
-                gi[b2] -= pi;
-            }
-        }
-        orc_fft((int)n, gr, gi, +1);
-        for (long j = 0; j < n; ++j) map[rg.ofs + j] = gr[j];
-        free(gr);
-        free(gi);
+        ring_from_F(nside, L, r, Fr + (size_t)r * (L + 1), Fi + (size_t)r * (L + 1), map + rg.ofs);
     }
 }
 
-void orc_synthesis(int nside, int L, const double* x, const double* beam, double* map)
+/* m-major complex B_l a_lm from the packed real vector. */
+static void pack_to_mmajor(int L, const double* x, const double* beam, double* ar, double* ai)
 {
-    const int nring = orc_nring(nside);
-    const int npair = 2 * nside;
-    const long nalm = (long)(L + 1) * (L + 2) / 2;
-    double* ar = (double*)malloc(sizeof(double) * nalm);
-    double* ai = (double*)malloc(sizeof(double) * nalm);
     for (int m = 0; m <= L; ++m)
         for (int l = m; l <= L; ++l) {
             const double b = beam ? beam[l] : 1.0;
@@ -419,44 +423,97 @@ void orc_synthesis(int nside, int L, const double* x, const double* beam, double
                 ai[k] = b * x[(long)l * l + 2 * m];
             }
         }
+}
+
+/* Ring coefficients F_m of ring pair p (north ring p, south ring nring-1-p):
+ * F_N = E + O, F_S = E - O from the even / odd (l - m) parity sums of
+ * lambda_lm(theta_p) B_l a_lm.  Fs may be NULL (the equator has no mirror). */
+static void pair_F(int nside, int L, int p, const double* ar, const double* ai, double* Fnr, double* Fni,
+                   double* Fsr, double* Fsi)
+{
+    orc_ring rg;
+    orc_ring_info(nside, p, &rg);
+    for (int m = 0; m <= L; ++m) {
+        orc_lam st;
+        lam_init(&st, m, rg.z, rg.sint);
+        double er = 0, ei = 0, orr = 0, oi = 0;
+        const long base = moff(L, m) - m;
+        for (int l = m; l <= L; ++l) {
+            if (l > m) lam_step(&st);
+            const double lam = lam_value(&st);
+            if (lam != 0.0) {
+                if (((l - m) & 1) == 0) {
+                    er += lam * ar[base + l];
+                    ei += lam * ai[base + l];
+                } else {
+                    orr += lam * ar[base + l];
+                    oi += lam * ai[base + l];
+                }
+            }
+        }
+        Fnr[m] = er + orr;
+        Fni[m] = ei + oi;
+        if (Fsr) {
+            Fsr[m] = er - orr;
+            Fsi[m] = ei - oi;
+        }
+    }
+}
+
+void orc_synthesis(int nside, int L, const double* x, const double* beam, double* map)
+{
+    const int nring = orc_nring(nside);
+    const int npair = 2 * nside;
+    const long nalm = (long)(L + 1) * (L + 2) / 2;
+    double* ar = (double*)malloc(sizeof(double) * nalm);
+    double* ai = (double*)malloc(sizeof(double) * nalm);
+    pack_to_mmajor(L, x, beam, ar, ai);
     /* F[ring][m] for both rings of each pair */
     double* Fr = (double*)malloc(sizeof(double) * (size_t)nring * (L + 1));
     double* Fi = (double*)malloc(sizeof(double) * (size_t)nring * (L + 1));
 
 #pragma omp parallel for schedule(dynamic, 1)
     for (int p = 0; p < npair; ++p) {
-        orc_ring rg;
-        orc_ring_info(nside, p, &rg);
         const int rn = p, rs = nring - 1 - p; /* rs == rn for the equator */
-        for (int m = 0; m <= L; ++m) {
-            orc_lam st;
-            lam_init(&st, m, rg.z, rg.sint);
-            double er = 0, ei = 0, orr = 0, oi = 0;
-            const long base = moff(L, m) - m;
-            for (int l = m; l <= L; ++l) {
-                if (l > m) lam_step(&st);
-                const double lam = lam_value(&st);
-                if (lam != 0.0) {
-                    if (((l - m) & 1) == 0) {
-                        er += lam * ar[base + l];
-                        ei += lam * ai[base + l];
-                    } else {
-                        orr += lam * ar[base + l];
-                        oi += lam * ai[base + l];
-                    }
-                }
-            }
-            Fr[(size_t)rn * (L + 1) + m] = er + orr;
-            Fi[(size_t)rn * (L + 1) + m] = ei + oi;
-            if (rs != rn) {
-                Fr[(size_t)rs * (L + 1) + m] = er - orr;
-                Fi[(size_t)rs * (L + 1) + m] = ei - oi;
-            }
-        }
+        const size_t on = (size_t)rn * (L + 1), os = (size_t)rs * (L + 1);
+        pair_F(nside, L, p, ar, ai, Fr + on, Fi + on, rs != rn ? Fr + os : NULL, rs != rn ? Fi + os : NULL);
     }
 
     rings_from_F(nside, L, Fr, Fi, map);
     free(Fr); free(Fi); free(ar); free(ai);
+}
+
+void orc_synthesis_rings(int nside, int L, const double* x, const double* beam, int nsel, const int* rings,
+                         double* out)
+{
+    const int nring = orc_nring(nside);
+    const long nalm = (long)(L + 1) * (L + 2) / 2;
+    double* ar = (double*)malloc(sizeof(double) * nalm);
+    double* ai = (double*)malloc(sizeof(double) * nalm);
+    pack_to_mmajor(L, x, beam, ar, ai);
+    long* ofs = (long*)malloc(sizeof(long) * (nsel + 1));
+    ofs[0] = 0;
+    for (int i = 0; i < nsel; ++i) {
+        orc_ring rg;
+        orc_ring_info(nside, rings[i], &rg);
+        ofs[i + 1] = ofs[i] + rg.nph;
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < nsel; ++i) {
+        const int r = rings[i];
+        const int north = r < 2 * nside;
+        const int p = north ? r : nring - 1 - r;
+        double* F = (double*)malloc(sizeof(double) * 4 * (L + 1));
+        double *Fnr = F, *Fni = F + (L + 1), *Fsr = F + 2 * (L + 1), *Fsi = F + 3 * (L + 1);
+        const int has_south = p != nring - 1 - p;
+        pair_F(nside, L, p, ar, ai, Fnr, Fni, has_south ? Fsr : NULL, has_south ? Fsi : NULL);
+        if (north)
+            ring_from_F(nside, L, r, Fnr, Fni, out + ofs[i]);
+        else
+            ring_from_F(nside, L, r, Fsr, Fsi, out + ofs[i]);
+        free(F);
+    }
+    free(ofs); free(ar); free(ai);
 }
 
 /* HEALPix alm2map_pol (external; called at source/likelihood.cpp:585 and

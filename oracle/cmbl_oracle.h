@@ -56,6 +56,10 @@ void orc_dft(int n, const double* re, const double* im, double* ore, double* oim
 
 /* s = Y B a : packed real a_lm (n = (L+1)^2, almToVector layout) -> RING map. */
 void orc_synthesis(int nside, int L, const double* x_packed, const double* beam, double* map);
+/* orc_synthesis restricted to the rings listed (0-based ring indices): the
+ * pixels of ring rings[i] follow those of rings[0..i-1] in out. */
+void orc_synthesis_rings(int nside, int L, const double* x_packed, const double* beam, int nsel,
+                         const int* rings, double* out);
 /* HEALPix alm2map_pol: packed T, E, B -> RING maps T, Q, U (no beam). */
 void orc_alm2map_pol(int nside, int L, const double* t, const double* e, const double* b, double* mt,
                      double* mq, double* mu);

@@ -88,7 +88,51 @@ class _MT19937:
         return y & 0xFFFFFFFF
 
 
-# The configurations of the benchmark definition
+# The workloads: nside 64 / lmax 128 (CPU-runnable), nside 512 / lmax 1024
+# with 200 point-source holes (finite-difference check), nside 1024 /
+# lmax 2048 (the headline workload), nside 2048 / lmax 4096 with 16 chain
+# maps of nside 256 / lmax 512, and the 64-map uneven batch below.
 CONFIG1 = dict(nside=64, lmax=128, fwhm_deg=2.0, sigma=10.0, cut_deg=20.0)
 CONFIG2 = dict(nside=512, lmax=1024, fwhm_deg=20.0 / 60.0, sigma=30.0, cut_deg=20.0, holes=200, hole_deg=0.5)
 CONFIG4 = dict(nside=1024, lmax=2048, fwhm_deg=10.0 / 60.0, sigma=20.0, cut_deg=20.0)
+CONFIG5 = dict(nside=2048, lmax=4096, fwhm_deg=5.0 / 60.0, sigma=15.0, cut_deg=20.0)
+CONFIG5_CHAIN = dict(nside=256, lmax=512, fwhm_deg=40.0 / 60.0, sigma=40.0, cut_deg=20.0)
+
+
+def batch_configs(nmaps=64, seed=3):
+    """64 maps of uneven size: lmax ~ U{256, 512, 768, 1024, 1536}, nside =
+    2^round(log2(lmax/2)), FWHM = 5400/lmax arcmin, sigma ~ U[10, 50] uK,
+    cut ~ U[10, 30] deg, independent seeds."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nmaps):
+        lmax = int(rng.choice([256, 512, 768, 1024, 1536]))
+        nside = 1 << int(round(math.log2(lmax / 2)))
+        out.append(dict(nside=nside, lmax=lmax, fwhm_deg=5400.0 / lmax / 60.0, sigma=float(rng.uniform(10, 50)),
+                        cut_deg=float(rng.uniform(10, 30)), seed=1000 + 10 * i))
+    return out
+
+
+def device_case(cfg, seed):
+    """Inputs of one workload map with the data synthesised by the library
+    (for sizes where the oracle's synthesis would dominate a test's time; the
+    oracle then evaluates the likelihood on the same data): returns the map
+    (beam, C_l and data set) and a dict with cl, beam, d, w, x."""
+    nside, lmax = cfg["nside"], cfg["lmax"]
+    cl = cm.cl_shape(lmax)
+    b = cm.beam(lmax, cfg["fwhm_deg"])
+    a_true = cm.simulate_alm(cl, lmax, seed)
+    mp = cm.Map(nside, lmax)
+    mp.set_beam(b)
+    mp.set_cl(cl)
+    d = mp.forward(a_true) + cm.simulate_white_noise(nside, cfg["sigma"], seed + 1)
+    mask = cm.mask_latitude(nside, math.radians(cfg["cut_deg"]))
+    holes = cfg.get("holes", 0)
+    if holes:
+        th = np.array(_uniform(seed + 3, math.pi / 50, math.pi - math.pi / 50, holes))
+        ph = np.array(_uniform(seed + 4, 0.0, 2 * math.pi, holes))
+        mask = cm.mask_regions(nside, th, ph, np.full(holes, math.radians(cfg.get("hole_deg", 0.5))), mask)
+    w = cm.weights_from_mask(mask, cfg["sigma"])
+    mp.set_data(d, w)
+    x = a_true + 0.1 * cm.simulate_alm(cl, lmax, seed + 2)
+    return mp, dict(cl=cl, beam=b, d=d, w=w, x=x)

@@ -198,6 +198,82 @@ def bench_pol(args, cfg):
 TRAFFIC_FILE = "profiles/pmc_traffic.json"
 
 
+def bench_loops(args):
+    """Device-resident optimisation and sampling at nside 1024 / lmax 2048:
+    200 L-BFGS iterations (m = 10, More-Thuente) from x = 0 and 500 HMC
+    leapfrog steps (50 trajectories of a fixed 10 steps of 0.05; masses from
+    the prior and noise curvature per l).  Overhead = loop time over (its
+    likelihood evaluations x the time of one evaluation) - 1."""
+    import numpy as np
+    import torch
+
+    import cosmopp_amd as cm
+
+    cfg = CONFIGS[4]
+    nside, lmax = cfg["nside"], cfg["lmax"]
+    log("bench: loops nside=%d lmax=%d" % (nside, lmax))
+    mp, inp = build_case(cfg)
+    dev = torch.device("cuda:0")
+    n = (lmax + 1) ** 2
+    x = torch.from_numpy(inp["x"]).to(dev)
+    g = torch.empty_like(x)
+    f = torch.empty(1, dtype=torch.float64, device=dev)
+    sp = torch.cuda.current_stream().cuda_stream
+    for _ in range(3):
+        mp.value_and_grad_device(x.data_ptr(), f.data_ptr(), g.data_ptr(), sp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        mp.value_and_grad_device(x.data_ptr(), f.data_ptr(), g.data_ptr(), sp)
+    torch.cuda.synchronize()
+    eval_s = (time.perf_counter() - t0) / 20
+    out = {"metric": "device-resident L-BFGS and HMC loops, nside=%d lmax=%d FP64" % (nside, lmax),
+           "eval_ms": round(eval_s * 1e3, 4), "n_gpus": 1, "dtype": "f64", "data": "synthetic"}
+    # L-BFGS: 200 iterations (epsilon tiny so that no convergence test stops it)
+    x0 = torch.zeros(n, dtype=torch.float64, device=dev)
+    res = torch.empty_like(x0)
+    lb = cm.LBFGS(mp, x0.data_ptr(), m=10)
+    lb.minimize(res.data_ptr(), epsilon=1e-300, gnorm_tol=0.0, max_iter=11)  # graph capture of every variant
+    lb.set_starting(x0.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fl, it, ev = lb.minimize(res.data_ptr(), epsilon=1e-300, gnorm_tol=0.0, max_iter=199)
+    t_lb = time.perf_counter() - t0
+    ev_total = ev  # trial evaluations of the line searches
+    out["lbfgs"] = {"iterations": it, "evaluations": ev_total, "seconds": round(t_lb, 4), "final_neg2lnL": fl,
+                    "ms_per_iteration": round(t_lb / it * 1e3, 4),
+                    "overhead_vs_evaluations": round(t_lb / (ev_total * eval_s) - 1, 4)}
+    lb.close()
+    log("bench: lbfgs %d iterations %d evaluations %.3f s" % (it, ev_total, t_lb))
+    # HMC: 50 trajectories x 10 fixed leapfrog steps of 0.05
+    l = np.floor(np.sqrt(np.arange(n))).astype(np.int64)
+    l -= (l * l > np.arange(n))
+    cl, b = inp["cl"], inp["beam"]
+    wbar = float(np.mean(inp["w"]))
+    curv = np.where(l >= 2, 2.0 / np.maximum(cl[l], 1e-300), 0.0) + 2.0 * b[l] ** 2 * wbar * 12 * nside * nside / (
+        4 * math.pi)
+    masses = torch.from_numpy(curv).to(dev)
+    start = torch.from_numpy(inp["x"]).to(dev)
+    hm = cm.HMC(mp, start.data_ptr(), masses.data_ptr(), 0.05, 10, 4242, fixed=True)
+    hm.run(1)  # graph capture
+    st0 = hm.state()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hm.run(50)
+    t_hmc = time.perf_counter() - t0
+    st = hm.state()
+    evs = st["evaluations"] - st0["evaluations"]
+    out["hmc"] = {"trajectories": 50, "leapfrog_steps": 500, "steps": "fixed: tau = 0.05, 10 per trajectory",
+                  "evaluations": evs, "accepted": st["accepted"] - st0["accepted"], "seconds": round(t_hmc, 4),
+                  "ms_per_leapfrog_step": round(t_hmc / 500 * 1e3, 4),
+                  "overhead_vs_evaluations": round(t_hmc / (evs * eval_s) - 1, 4),
+                  "momenta": "reference draws (whitenoise seeds) on host threads, 8 trajectories ahead"}
+    hm.close()
+    log("bench: hmc 500 steps %.3f s" % t_hmc)
+    mp.close()
+    print(json.dumps(out), flush=True)
+
+
 def read_traffic(nside, lmax, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE passes, calibrated on a plain copy), if
@@ -293,6 +369,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-table", action="store_true",
                     help="only time the CPU oracle at configs 1, 2, 4 on one and on all threads")
+    ap.add_argument("--loops", action="store_true",
+                    help="time the device-resident L-BFGS (200 iterations) and HMC (500 leapfrog steps) at config 4")
     ap.add_argument("--host-buffers", action="store_true",
                     help="time cmbl_value_and_grad with x and g in host memory (PCIe-inclusive rate)")
     ap.add_argument("--cpu-budget", type=float, default=40.0, help="seconds for the CPU baseline leg")
@@ -330,6 +408,9 @@ def main():
         return
     if args.config == 3:
         bench_batch(args)
+        return
+    if args.loops:
+        bench_loops(args)
         return
     cfg = CONFIGS[args.config]
     if args.pol:

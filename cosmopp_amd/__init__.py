@@ -56,6 +56,30 @@ _SIGS = {
     "cmbl_map_set_cl": ([_vp, _vp], ctypes.c_int),
     "cmbl_map_set_data": ([_vp, _vp, _vp], ctypes.c_int),
     "cmbl_map_set_data_device": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+    "cmbl_map_generation": ([_vp], ctypes.c_long),
+    "cmbl_map_check_finite": ([_vp, _vp], ctypes.c_int),
+    "cmbl_batch_check_finite": ([_vp, _vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "cmbl_vec_release_workspaces": ([], ctypes.c_int),
+    "cmbl_lbfgs_create": ([ctypes.POINTER(_vp), _vp, _vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "cmbl_lbfgs_destroy": ([_vp], None),
+    "cmbl_lbfgs_set_starting": ([_vp, _vp], ctypes.c_int),
+    "cmbl_lbfgs_minimize": ([_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _vp, _dp,
+                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _vp, _vp], ctypes.c_int),
+    "cmbl_lbfgs_gradient": ([_vp, _vp], ctypes.c_int),
+    "cmbl_lbfgs_state": ([_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+    "cmbl_hmc_create": ([ctypes.POINTER(_vp), _vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                         ctypes.c_int], ctypes.c_int),
+    "cmbl_hmc_destroy": ([_vp], None),
+    "cmbl_hmc_run": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "cmbl_hmc_state": ([_vp, _dp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),
+                        ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+    "cmbl_hmc_position": ([_vp, _vp], ctypes.c_int),
+    "cmbl_line_search_create": ([ctypes.POINTER(_vp), ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+    "cmbl_line_search_destroy": ([_vp], None),
+    "cmbl_line_search_begin": ([_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp], ctypes.c_int),
+    "cmbl_line_search_feed": ([_vp, ctypes.c_double, ctypes.c_double, _dp, ctypes.POINTER(ctypes.c_int)],
+                              ctypes.c_int),
     "cmbl_value_and_grad": ([_vp, _vp, _vp, _vp], ctypes.c_int),
     "cmbl_value_and_grad_device": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "cmbl_forward": ([_vp, _vp, _vp], ctypes.c_int),
@@ -327,6 +351,16 @@ class Map:
     def value_and_grad_device(self, x_ptr, f_ptr, g_ptr, stream=0):
         _check(lib().cmbl_value_and_grad_device(self._h, x_ptr, f_ptr, g_ptr, stream))
 
+    def check_finite(self, stream=0):
+        """Wait for stream; raise CmblError(CMBL_ERR_NONFINITE) if an evaluation
+        since the last check produced a non-finite -2lnL."""
+        _check(lib().cmbl_map_check_finite(self._h, stream))
+
+    @property
+    def generation(self):
+        """Number of set_* calls so far (a cached value is stale once it changes)."""
+        return lib().cmbl_map_generation(self._h)
+
     def forward(self, x):
         x = _arr(x, self.nparams)
         out = np.zeros(self.npix)
@@ -403,6 +437,12 @@ class Batch:
         gs = (_vp * n)(*g_ptrs)
         _check(lib().cmbl_batch_value_and_grad_device(self._h, ctypes.cast(xs, _vp), f_ptr,
                                                       ctypes.cast(gs, _vp), stream))
+
+    def check_finite(self, stream=0):
+        """Wait for stream; raise CmblError(CMBL_ERR_NONFINITE) naming the first
+        map whose evaluation produced a non-finite -2lnL."""
+        first = ctypes.c_int(-1)
+        _check(lib().cmbl_batch_check_finite(self._h, stream, ctypes.byref(first)))
 
 
 def pseudo_cl(alm, lmax):
@@ -522,14 +562,147 @@ class Gibbs:
         return x, it.value
 
 
+CMBL_LOOP_EAGER = 1
+CMBL_HMC_FIXED = 2
+
+_LBFGS_CB = ctypes.CFUNCTYPE(None, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp)
+_HMC_CB = ctypes.CFUNCTYPE(None, _vp, _vp, ctypes.c_double)
+
+
+class LBFGS:
+    """Device-resident LBFGS_General with the More-Thuente line search
+    (include/lbfgs_general.hpp:58-405) over a map's -2lnL (cmbl_lbfgs).
+
+    x0_ptr: device pointer to the starting point (nparams doubles).  The
+    constructor evaluates there (setStarting).  graphs=False launches the
+    same kernels without HIP graphs.
+    """
+
+    def __init__(self, mp, x0_ptr, m=10, graphs=True):
+        self.map = mp
+        h = _vp()
+        _check(lib().cmbl_lbfgs_create(ctypes.byref(h), mp.handle, x0_ptr, int(m), 0 if graphs else CMBL_LOOP_EAGER))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().cmbl_lbfgs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_starting(self, x0_ptr):
+        _check(lib().cmbl_lbfgs_set_starting(self._h, x0_ptr))
+
+    def minimize(self, res_ptr=None, epsilon=1e-3, gnorm_tol=1e-5, max_iter=1000000, callback=None):
+        """LBFGS_General::minimize: returns (-2lnL, iterations, evaluations).
+        callback(iter, f, gnorm, x_ptr, g_ptr, z_ptr) is called per iteration."""
+        f = ctypes.c_double()
+        it, ev = ctypes.c_int(), ctypes.c_int()
+        cb = None
+        if callback is not None:
+            cb = _LBFGS_CB(lambda user, i, fv, gn, x, g, z: callback(i, fv, gn, x, g, z))
+        _check(lib().cmbl_lbfgs_minimize(self._h, float(epsilon), float(gnorm_tol), int(max_iter), res_ptr,
+                                         ctypes.byref(f), ctypes.byref(it), ctypes.byref(ev),
+                                         ctypes.cast(cb, _vp) if cb else None, None))
+        return f.value, it.value, ev.value
+
+    def gradient(self, g_ptr):
+        _check(lib().cmbl_lbfgs_gradient(self._h, g_ptr))
+
+    def state(self):
+        f, gn = ctypes.c_double(), ctypes.c_double()
+        it, ev = ctypes.c_int(), ctypes.c_long()
+        _check(lib().cmbl_lbfgs_state(self._h, ctypes.byref(f), ctypes.byref(gn), ctypes.byref(it), ctypes.byref(ev)))
+        return dict(f=f.value, gnorm=gn.value, iter=it.value, evaluations=ev.value)
+
+
+class HMC:
+    """Device-resident HMCGeneral (include/hmc_general.hpp:64-246) over a
+    map's -2lnL (cmbl_hmc).  start_ptr, masses_ptr: device pointers.
+    fixed=True uses tau_max and n_max on every trajectory."""
+
+    def __init__(self, mp, start_ptr, masses_ptr, tau_max, n_max, seed, fixed=False, graphs=True):
+        self.map = mp
+        flags = (CMBL_HMC_FIXED if fixed else 0) | (0 if graphs else CMBL_LOOP_EAGER)
+        h = _vp()
+        _check(lib().cmbl_hmc_create(ctypes.byref(h), mp.handle, start_ptr, masses_ptr, float(tau_max), int(n_max),
+                                     int(seed), flags))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().cmbl_hmc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, max_iters, callback=None):
+        """HMCGeneral::run: callback(x_ptr, like) at the start and after every trajectory."""
+        cb = None
+        if callback is not None:
+            cb = _HMC_CB(lambda user, x, like: callback(x, like))
+        _check(lib().cmbl_hmc_run(self._h, int(max_iters), ctypes.cast(cb, _vp) if cb else None, None))
+
+    def state(self):
+        like = ctypes.c_double()
+        t, a, e = ctypes.c_long(), ctypes.c_long(), ctypes.c_long()
+        _check(lib().cmbl_hmc_state(self._h, ctypes.byref(like), ctypes.byref(t), ctypes.byref(a), ctypes.byref(e)))
+        return dict(like=like.value, trajectories=t.value, accepted=a.value, evaluations=e.value)
+
+    def position(self, out_ptr):
+        _check(lib().cmbl_hmc_position(self._h, out_ptr))
+
+
+class LineSearch:
+    """The More-Thuente line search as a state machine (cmbl_line_search):
+    begin(f0, dg0, stp0) -> first step; feed(f, dg) -> (info, step)."""
+
+    def __init__(self, ftol=0.1, gtol=0.5, xtol=1e-15, stpmin=1e-15, stpmax=1e15, maxfev=100):
+        h = _vp()
+        _check(lib().cmbl_line_search_create(ctypes.byref(h), ftol, gtol, xtol, stpmin, stpmax, maxfev))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().cmbl_line_search_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def begin(self, f0, dg0, stp0):
+        stp = ctypes.c_double()
+        _check(lib().cmbl_line_search_begin(self._h, f0, dg0, stp0, ctypes.byref(stp)))
+        return stp.value
+
+    def feed(self, f, dg):
+        stp = ctypes.c_double()
+        info = ctypes.c_int()
+        _check(lib().cmbl_line_search_feed(self._h, f, dg, ctypes.byref(stp), ctypes.byref(info)))
+        return info.value, stp.value
+
+
 class MapLikelihood:
     """-2lnL(a_lm) with gradient behind the reference's function interfaces.
 
     evaluate(x) -> -2lnL                      (RealFunctionMultiDim::evaluate)
     evaluate_grad(x, res) -> fills res        (RealFunctionMultiToMulti::evaluate; res is resized)
     calculate(params) / calculate_deriv(params, i)  (LikelihoodWithDerivs)
-    The last (x, value, gradient) is cached by contents, so value + derivative
-    at one point cost one device evaluation.
+    The last (x, value, gradient) is cached by contents and by the map's
+    generation (set_beam / set_cl / set_data invalidate it), so value +
+    derivative at one point cost one device evaluation.
     """
 
     def __init__(self, mp):
@@ -537,13 +710,16 @@ class MapLikelihood:
         self._x = None
         self._f = None
         self._g = None
+        self._gen = None
         self.evaluations = 0
 
     def _eval(self, x):
         x = _arr(x, self.map.nparams)
-        if self._x is None or not np.array_equal(self._x, x):
+        gen = self.map.generation
+        if self._x is None or self._gen != gen or not np.array_equal(self._x, x):
             self._f, self._g = self.map.value_and_grad(x)
             self._x = x.copy()
+            self._gen = gen
             self.evaluations += 1
         return self._f, self._g
 

@@ -208,6 +208,8 @@ struct MapDesc {
     double* chi;              // [npair] per-pair chi^2
     double* prior;            // [nprior_blocks]
     int nprior;
+    int* nonfinite;           // set to 1 by the -2lnL reduction when the value is NaN/Inf
+                              // (host-mapped memory, cleared by the host)
 };
 
 // One Legendre work item: degree m of one ring block of one map, starting at
@@ -249,6 +251,10 @@ enum RingMode { kRingLike = 0, kRingSyn = 1, kRingAna = 2, kRingWeight = 3 };
 //   kPostAddAlm  y += c u / B_l            (packed c Y^T v without the l-scaling)
 //   kPostQuad    y = c u / B_l             (quadrature analysis, c = 4 pi / Npix)
 enum PostMode { kPostGrad = 0, kPostLike = 1, kPostGibbs = 2, kPostAlm = 3, kPostAddAlm = 4, kPostQuad = 5 };
+
+// One likelihood + gradient evaluation of a map on stream s (device x, f, g);
+// marks = false records no stage-timing events (for HIP graph capture).
+int eval_device(::cmbl_map* mp, const double* x, double* f, double* g, hipStream_t s, bool marks = true);
 
 // Thread-local error string.
 void set_error(const std::string& msg);
@@ -310,7 +316,11 @@ int launch_scan(int K, const WorkItem* items, int nitems, const MapDesc* d_maps,
 int launch_capture(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, double* d_prev, double* d_cur,
                    int* d_k, hipStream_t s);
 
-// BLAS-1
+// BLAS-1.  The dot product runs kDotParts blocks of 256 threads (grid-stride
+// FMA accumulation, block sums) and one final block over the kDotParts
+// partial sums; the fused loop kernels replicate this order exactly.
+constexpr int kDotParts = 1024;
+enum { kVecCopy = 0, kVecSet, kVecAxpy, kVecMul, kVecDiv, kVecPow };
 int launch_vec(int op, double* y, const double* x, double c, long n, hipStream_t s);
 int launch_dot(const double* x, const double* y, long n, double* d_out, double* d_work, hipStream_t s);
 
@@ -324,6 +334,8 @@ struct cmbl_map {
     int device = 0;
     int lds_ring = 0;
     bool has_beam = false, has_cl = false, has_data = false;
+    long generation = 0;            // incremented by every cmbl_map_set_*
+    int* h_nonfinite = nullptr;     // host-mapped flag written by the -2lnL reduction
     std::vector<void*> allocs;
     cmbl::MapDesc desc{};
     cmbl::MapDesc* d_desc = nullptr;

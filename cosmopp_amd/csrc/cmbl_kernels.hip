@@ -15,6 +15,7 @@
 #include <atomic>
 #include <cstdlib>
 
+#include "cmbl_device.hpp"
 #include "cmbl_internal.hpp"
 
 namespace cmbl {
@@ -43,13 +44,6 @@ __device__ __forceinline__ double2 cis_pi_frac(long q, long n)
     return make_double2(c, s);
 }
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
 // Ordering point for LDS data exchanged inside one wavefront.  The Legendre
 // kernels run one wavefront per workgroup and the LDS executes the operations
 // of a wavefront in issue order, so a store is visible to every later load of
@@ -68,22 +62,6 @@ __device__ __forceinline__ void wave_sync()
 // which the compiler must order after earlier LDS and global stores.
 typedef __attribute__((address_space(1))) double gdouble;
 typedef const __attribute__((address_space(1))) double gcdouble;
-
-// Block-wide sum for blockDim.x == nthreads (multiple of 64); result valid in thread 0.
-template <int NT>
-__device__ __forceinline__ double block_sum(double v, double* sh)
-{
-    v = wave_sum(v);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) sh[w] = v;
-    __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < NT / 64; ++i) t += sh[i];
-    }
-    __syncthreads();
-    return t;
-}
 
 // ================================================================== k_pre / k_post
 // The packed parameter vector (almToVector: l-major, index l^2 + 2m - 1 + c)
@@ -2034,7 +2012,10 @@ __device__ __forceinline__ void final_map(const MapDesc& md, double* __restrict_
     for (int i = threadIdx.x; i < md.npair; i += 1024) a += md.chi[i];
     for (int i = threadIdx.x; i < md.nprior; i += 1024) a += md.prior[i];
     const double t = block_sum<1024>(a, sh);
-    if (threadIdx.x == 0) *f = t;
+    if (threadIdx.x == 0) {
+        *f = t;
+        if (!isfinite(t)) *md.nonfinite = 1;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_final(const MapDesc* __restrict__ md_, double* __restrict__ f)
@@ -2049,9 +2030,9 @@ __global__ __launch_bounds__(1024) void k_final_b(const MapDesc* __restrict__ md
 }
 
 // ================================================================== BLAS-1
-enum { kVecCopy = 0, kVecSet, kVecAxpy, kVecMul, kVecDiv, kVecPow };
 
-__global__ void k_vec(int op, double* __restrict__ y, const double* __restrict__ x, double c, long n)
+// y and x may be the same vector (in-place scaling y = c y, y += c y).
+__global__ void k_vec(int op, double* y, const double* x, double c, long n)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         switch (op) {
@@ -2344,7 +2325,7 @@ int launch_vec(int op, double* y, const double* x, double c, long n, hipStream_t
 
 int launch_dot(const double* x, const double* y, long n, double* d_out, double* d_work, hipStream_t s)
 {
-    const int np = 1024;
+    const int np = kDotParts;
     hipLaunchKernelGGL(k_dot_part, dim3(np), dim3(256), 0, s, x, y, n, d_work);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(1024), 0, s, d_work, np, d_out);
     CMBL_HIP(hipGetLastError());

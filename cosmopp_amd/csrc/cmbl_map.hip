@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -465,6 +466,11 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     CMBL_TRY(dalloc(mp, &mp->d_g, mp->nparam));
     CMBL_TRY(dalloc(mp, &mp->d_map, mp->npix));
     CMBL_TRY(dalloc(mp, &mp->d_desc, 1));
+    // non-finite flag of the -2lnL reduction, in host-mapped memory so that the
+    // host reads it without a copy
+    CMBL_HIP(hipHostMalloc((void**)&mp->h_nonfinite, sizeof(int), hipHostMallocMapped));
+    *mp->h_nonfinite = 0;
+    CMBL_HIP(hipHostGetDevicePointer((void**)&d.nonfinite, mp->h_nonfinite, 0));
     CMBL_HIP(hipMemcpy(mp->d_desc, &d, sizeof(d), hipMemcpyHostToDevice));
 
     // ---- Bluestein filters
@@ -538,6 +544,20 @@ static void map_free(cmbl_map* mp)
     mp->ev.clear();
     for (void* p : mp->allocs) (void)hipFree(p);
     mp->allocs.clear();
+    if (mp->h_nonfinite) (void)hipHostFree(mp->h_nonfinite);
+    mp->h_nonfinite = nullptr;
+}
+
+// Report (and clear) a non-finite -2lnL raised by an evaluation of this map
+// that has completed.  Does not wait for evaluations still in flight.
+static int take_nonfinite(cmbl_map* mp)
+{
+    if (mp->h_nonfinite && __atomic_load_n(mp->h_nonfinite, __ATOMIC_ACQUIRE) != 0) {
+        __atomic_store_n(mp->h_nonfinite, 0, __ATOMIC_RELEASE);
+        return fail(CMBL_ERR_NONFINITE, "an earlier evaluation of this map produced a non-finite -2lnL "
+                                        "(NaN/Inf in a_lm or data)");
+    }
+    return CMBL_OK;
 }
 
 static int check_map(const cmbl_map* mp)
@@ -568,10 +588,11 @@ static int drain_timing(cmbl_map* mp, int keep)
 struct StageMarks {
     cmbl_map* mp;
     hipStream_t s;
+    bool on;
     hipEvent_t* e = nullptr;
     int begin()
     {
-        if (!mp->timing) return CMBL_OK;
+        if (!on || !mp->timing) return CMBL_OK;
         CMBL_TRY(drain_timing(mp, cmbl_map::kEvSets - 1));
         e = &mp->ev[(size_t)mp->ev_next * (CMBL_NSTAGES + 1)];
         mp->ev_next = (mp->ev_next + 1) % cmbl_map::kEvSets;
@@ -586,10 +607,11 @@ struct StageMarks {
 };
 
 // One evaluation on stream s with device x; f and g are device pointers.
-static int eval_device(cmbl_map* mp, const double* x, double* f, double* g, hipStream_t s)
+// With marks, stage-timing events are recorded when the map's timing is on.
+int eval_device(cmbl_map* mp, const double* x, double* f, double* g, hipStream_t s, bool marks)
 {
     const MapDesc& d = mp->desc;
-    StageMarks t{mp, s};
+    StageMarks t{mp, s, marks};
     CMBL_TRY(t.begin());
     CMBL_TRY(launch_pre(d, mp->d_desc, x, s));
     CMBL_TRY(t.mark(1));
@@ -680,6 +702,7 @@ int cmbl_map_set_beam(cmbl_map* mp, const double* beam)
     if (!beam) return fail(CMBL_ERR_ARG, "beam is null");
     CMBL_HIP(hipMemcpy(mp->d_beam, beam, sizeof(double) * (mp->L + 1), hipMemcpyHostToDevice));
     mp->has_beam = true;
+    ++mp->generation;
     return CMBL_OK;
 }
 
@@ -691,6 +714,7 @@ int cmbl_map_set_cl(cmbl_map* mp, const double* cl)
     for (int l = 0; l <= mp->L; ++l) inv[l] = cl[l] > 0 ? 1.0 / cl[l] : 0.0;
     CMBL_HIP(hipMemcpy(mp->d_invcl, inv.data(), sizeof(double) * (mp->L + 1), hipMemcpyHostToDevice));
     mp->has_cl = true;
+    ++mp->generation;
     return CMBL_OK;
 }
 
@@ -701,6 +725,7 @@ int cmbl_map_set_data(cmbl_map* mp, const double* d, const double* w)
     CMBL_HIP(hipMemcpy(mp->d_data, d, sizeof(double) * mp->npix, hipMemcpyHostToDevice));
     CMBL_HIP(hipMemcpy(mp->d_w, w, sizeof(double) * mp->npix, hipMemcpyHostToDevice));
     mp->has_data = true;
+    ++mp->generation;
     return CMBL_OK;
 }
 
@@ -712,6 +737,7 @@ int cmbl_map_set_data_device(cmbl_map* mp, const double* d, const double* w, voi
     CMBL_HIP(hipMemcpyAsync(mp->d_data, d, sizeof(double) * mp->npix, hipMemcpyDeviceToDevice, s));
     CMBL_HIP(hipMemcpyAsync(mp->d_w, w, sizeof(double) * mp->npix, hipMemcpyDeviceToDevice, s));
     mp->has_data = true;
+    ++mp->generation;
     return CMBL_OK;
 }
 
@@ -720,20 +746,34 @@ int cmbl_value_and_grad_device(cmbl_map* mp, const double* x, double* f, double*
     CMBL_TRY(check_map(mp));
     if (!x || !f) return fail(CMBL_ERR_ARG, "x or f is null");
     if (!mp->has_data) return fail(CMBL_ERR_STATE, "cmbl_map_set_data has not been called");
+    CMBL_TRY(take_nonfinite(mp));
     hipStream_t s = (hipStream_t)stream;
     return eval_device(mp, x, f, g ? g : mp->d_g, s);
 }
+
+int cmbl_map_check_finite(cmbl_map* mp, void* stream)
+{
+    CMBL_TRY(check_map(mp));
+    CMBL_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return take_nonfinite(mp);
+}
+
+long cmbl_map_generation(const cmbl_map* mp) { return mp ? mp->generation : -1; }
 
 int cmbl_value_and_grad(cmbl_map* mp, const double* x, double* f, double* g)
 {
     CMBL_TRY(check_map(mp));
     if (!x || !f) return fail(CMBL_ERR_ARG, "x or f is null");
     if (!mp->has_data) return fail(CMBL_ERR_STATE, "cmbl_map_set_data has not been called");
+    CMBL_TRY(take_nonfinite(mp));
     CMBL_HIP(hipMemcpy(mp->d_x, x, sizeof(double) * mp->nparam, hipMemcpyHostToDevice));
     CMBL_TRY(eval_device(mp, mp->d_x, mp->d_f, mp->d_g, nullptr));
     CMBL_HIP(hipMemcpy(f, mp->d_f, sizeof(double), hipMemcpyDeviceToHost));
     if (g) CMBL_HIP(hipMemcpy(g, mp->d_g, sizeof(double) * mp->nparam, hipMemcpyDeviceToHost));
-    if (!std::isfinite(*f)) return fail(CMBL_ERR_NONFINITE, "-2lnL is not finite (NaN/Inf in a_lm or data)");
+    if (!std::isfinite(*f)) {
+        (void)take_nonfinite(mp);
+        return fail(CMBL_ERR_NONFINITE, "-2lnL is not finite (NaN/Inf in a_lm or data)");
+    }
     return CMBL_OK;
 }
 
@@ -1093,6 +1133,10 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
     for (int i = 0; i < nmaps; ++i) {
         cmbl_map* mp = maps[i];
         if (!mp) return cleanup(fail(CMBL_ERR_ARG, "null map in batch"));
+        for (int j = 0; j < i; ++j)
+            if (maps[j] == mp)
+                return cleanup(fail(CMBL_ERR_ARG, "map " + std::to_string(i) + " appears twice in the batch (the "
+                                                  "entries would share one workspace)"));
         b->maps.push_back(mp);
         descs.push_back(mp->desc);
         int k1 = 0, k5 = 0;
@@ -1148,6 +1192,19 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
     return CMBL_OK;
 }
 
+int cmbl_batch_check_finite(cmbl_batch* b, void* stream, int* first_map)
+{
+    if (!b) return fail(CMBL_ERR_ARG, "null batch");
+    if (first_map) *first_map = -1;
+    CMBL_HIP(hipStreamSynchronize((hipStream_t)stream));
+    int bad = -1;
+    for (int i = 0; i < (int)b->maps.size(); ++i)
+        if (take_nonfinite(b->maps[i]) != CMBL_OK && bad < 0) bad = i;
+    if (bad < 0) return CMBL_OK;
+    if (first_map) *first_map = bad;
+    return fail(CMBL_ERR_NONFINITE, "map " + std::to_string(bad) + " of the batch produced a non-finite -2lnL");
+}
+
 void cmbl_batch_destroy(cmbl_batch* b)
 {
     if (!b) return;
@@ -1165,6 +1222,10 @@ int cmbl_batch_value_and_grad_device(cmbl_batch* b, const double* const* x, doub
     for (int i = 0; i < n; ++i) {
         if (!b->maps[i]->has_data) return fail(CMBL_ERR_STATE, "map without data in batch");
         if (!x[i] || !g[i]) return fail(CMBL_ERR_ARG, "null pointer");
+    }
+    for (int i = 0; i < n; ++i) {
+        const int rc = take_nonfinite(b->maps[i]);
+        if (rc != CMBL_OK) return fail(rc, "map " + std::to_string(i) + " of the batch: " + g_err);
     }
     // groups of up to kBatchGroup maps share one launch of k_pre, k_post, k_final
     std::vector<BatchArgs> groups;
@@ -1367,19 +1428,54 @@ int cmbl_vec_mul(double* y, const double* x, long n, void* s) { return launch_ve
 int cmbl_vec_div(double* y, const double* x, long n, void* s) { return launch_vec(4, y, x, 0, n, (hipStream_t)s); }
 int cmbl_vec_pow(double* y, double p, long n, void* s) { return launch_vec(5, y, nullptr, p, n, (hipStream_t)s); }
 
-static double* dot_work()
+// Dot-product workspaces (kDotParts partial sums + 1 result), one per
+// (device, stream): calls queued on one stream run in order and may share
+// one, calls on different streams may not.
+struct DotWorkspace {
+    int device;
+    hipStream_t stream;
+    double* p;
+};
+static std::mutex g_dot_mu;
+static std::vector<DotWorkspace> g_dot_ws;
+
+static int dot_work(hipStream_t s, double** out)
 {
-    static thread_local double* w = nullptr;
-    if (!w) {
-        if (hipMalloc(&w, sizeof(double) * 1025) != hipSuccess) w = nullptr;
+    int dev = 0;
+    CMBL_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_dot_mu);
+    for (const DotWorkspace& w : g_dot_ws)
+        if (w.device == dev && w.stream == s) {
+            *out = w.p;
+            return CMBL_OK;
+        }
+    double* p = nullptr;
+    CMBL_HIP(hipMalloc(&p, sizeof(double) * (kDotParts + 1)));
+    g_dot_ws.push_back(DotWorkspace{dev, s, p});
+    *out = p;
+    return CMBL_OK;
+}
+
+int cmbl_vec_release_workspaces(void)
+{
+    std::lock_guard<std::mutex> lock(g_dot_mu);
+    int cur = 0;
+    CMBL_HIP(hipGetDevice(&cur));
+    int rc = CMBL_OK;
+    for (const DotWorkspace& w : g_dot_ws) {
+        if (hipSetDevice(w.device) != hipSuccess || hipDeviceSynchronize() != hipSuccess || hipFree(w.p) != hipSuccess)
+            rc = fail(CMBL_ERR_HIP, "freeing a dot-product workspace failed");
     }
-    return w;
+    g_dot_ws.clear();
+    (void)hipSetDevice(cur);
+    return rc;
 }
 
 int cmbl_vec_dot_device(const double* x, const double* y, long n, double* d_out, void* s)
 {
-    double* w = dot_work();
-    if (!w) return fail(CMBL_ERR_ALLOC, "dot workspace");
+    if (!x || !y || !d_out || n < 0) return fail(CMBL_ERR_ARG, "bad dot arguments");
+    double* w = nullptr;
+    CMBL_TRY(dot_work((hipStream_t)s, &w));
     return launch_dot(x, y, n, d_out, w, (hipStream_t)s);
 }
 
@@ -1414,10 +1510,11 @@ int cmbl_reorder(int nside, const double* in, double* out, int to_ring)
 
 int cmbl_vec_dot(const double* x, const double* y, long n, double* out, void* s)
 {
-    double* w = dot_work();
-    if (!w) return fail(CMBL_ERR_ALLOC, "dot workspace");
-    CMBL_TRY(launch_dot(x, y, n, w + 1024, w, (hipStream_t)s));
-    CMBL_HIP(hipMemcpyAsync(out, w + 1024, sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)s));
+    if (!x || !y || !out || n < 0) return fail(CMBL_ERR_ARG, "bad dot arguments");
+    double* w = nullptr;
+    CMBL_TRY(dot_work((hipStream_t)s, &w));
+    CMBL_TRY(launch_dot(x, y, n, w + kDotParts, w, (hipStream_t)s));
+    CMBL_HIP(hipMemcpyAsync(out, w + kDotParts, sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)s));
     CMBL_HIP(hipStreamSynchronize((hipStream_t)s));
     return CMBL_OK;
 }

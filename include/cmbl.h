@@ -66,6 +66,10 @@ int cmbl_map_set_cl(cmbl_map* map, const double* cl);
  * weighting of source/cmb_gibbs.cpp:381-387), host pointers, npix each. */
 int cmbl_map_set_data(cmbl_map* map, const double* d, const double* w);
 int cmbl_map_set_data_device(cmbl_map* map, const double* d_d, const double* d_w, void* stream);
+/* Count of cmbl_map_set_* calls so far: a cache of (x, -2lnL, gradient) is
+ * valid only while it is unchanged (the reference's function objects are
+ * rebuilt when their inputs change; here the map is updated in place). */
+long cmbl_map_generation(const cmbl_map* map);
 
 /* ---------------------------------------------------------------- likelihood */
 /* -2lnL and its gradient at the packed a_lm x.  This is the body of the
@@ -75,9 +79,17 @@ int cmbl_map_set_data_device(cmbl_map* map, const double* d_d, const double* d_w
  * and Math::LikelihoodWithDerivs::calculate / calculateDeriv
  * (include/likelihood_function.hpp:39,46).  g may be NULL (value only). */
 int cmbl_value_and_grad(cmbl_map* map, const double* x, double* f, double* g);
-/* Same, device pointers, asynchronous on stream; *d_f is written on device. */
+/* Same, device pointers, asynchronous on stream; *d_f is written on device.
+ * A non-finite -2lnL raises the map's non-finite flag on the device (the
+ * reference's CHECKS_ON check, include/macros.hpp:113-129, cannot run inside
+ * an asynchronous call): this entry point returns CMBL_ERR_NONFINITE when an
+ * earlier evaluation of the map that has completed raised it, and
+ * cmbl_map_check_finite waits for the stream and reports it.  Either clears it. */
 int cmbl_value_and_grad_device(cmbl_map* map, const double* d_x, double* d_f, double* d_g,
                                void* stream);
+/* Synchronise stream; CMBL_ERR_NONFINITE if an evaluation of the map since
+ * the last check produced a non-finite -2lnL (the flag is cleared). */
+int cmbl_map_check_finite(cmbl_map* map, void* stream);
 
 /* Forward half of the autograd pair: map = Y B a(x)  (HEALPix alm2map with
  * the beam applied, source/cmb_gibbs.cpp:370-378). */
@@ -107,9 +119,15 @@ int cmbl_map_stage_times(cmbl_map* map, double* ms, long* nevals);
  * stage over a cost-ordered work list.  The maps must stay alive. */
 int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps);
 void cmbl_batch_destroy(cmbl_batch* batch);
-/* d_x[i], d_g[i]: device pointers for map i; d_f: device array of nmaps. */
+/* d_x[i], d_g[i]: device pointers for map i; d_f: device array of nmaps.
+ * A map may appear once per batch.  Non-finite values raise each map's flag
+ * as in cmbl_value_and_grad_device; a flag raised by an earlier completed
+ * evaluation makes this call return CMBL_ERR_NONFINITE. */
 int cmbl_batch_value_and_grad_device(cmbl_batch* batch, const double* const* d_x, double* d_f,
                                      double* const* d_g, void* stream);
+/* Synchronise stream; CMBL_ERR_NONFINITE if any map's flag is raised
+ * (*first_map, may be NULL, receives the first such map, else -1).  Clears them. */
+int cmbl_batch_check_finite(cmbl_batch* batch, void* stream, int* first_map);
 
 /* ---------------------------------------------------------------- Gibbs CG */
 /* The constrained-realisation system of CMBGibbsSampler::generateSignal,
@@ -145,6 +163,78 @@ int cmbl_gibbs_solve(cmbl_gibbs* gibbs, const double* b, double* x, double preci
                      int* iterations);
 int cmbl_gibbs_solve_device(cmbl_gibbs* gibbs, const double* d_b, double* d_x, double precision, int max_iter,
                             int* iterations, void* stream);
+
+/* ---------------------------------------------------------------- device-resident loops */
+/* The reference's optimiser and sampler loops with every vector on the GPU
+ * and each phase's vector work in fused passes, captured in HIP graphs and
+ * replayed (pass CMBL_LOOP_EAGER to launch the same kernels without graphs).
+ * Their dot products run the order of cmbl_vec_dot and their updates the
+ * operations of the cmbl_vec_* entry points, so the iterates equal, bit for
+ * bit, those of the reference's LBFGS_General / HMCGeneral run on device
+ * vectors built from the cmbl_vec_* entry points (include/cmbl_likelihood.hpp).
+ * Each context owns a stream; calls block until their work is complete. */
+#define CMBL_LOOP_EAGER 1 /* no HIP graphs */
+#define CMBL_HMC_FIXED 2  /* step size tau_max and n_max steps on every trajectory */
+
+typedef struct cmbl_lbfgs cmbl_lbfgs;
+/* LBFGS_General(factory, f, starting, m) with the More-Thuente line search
+ * (include/lbfgs_general.hpp:144-200: setStarting evaluates -2lnL and its
+ * gradient at d_x0).  The map must have its data set. */
+int cmbl_lbfgs_create(cmbl_lbfgs** out, cmbl_map* map, const double* d_x0, int m, int flags);
+void cmbl_lbfgs_destroy(cmbl_lbfgs* lb);
+/* setStarting (include/lbfgs_general.hpp:167-200). */
+int cmbl_lbfgs_set_starting(cmbl_lbfgs* lb, const double* d_x0);
+/* Callback of LBFGS_General::minimize (iteration, -2lnL, gradient norm, and
+ * device pointers to x, g and the last direction z). */
+typedef void (*cmbl_lbfgs_callback)(void* user, int iter, double f, double gnorm, const double* d_x, const double* d_g,
+                                    const double* d_z);
+/* LBFGS_General::minimize(res, epsilon, gNormTol, maxIter, callback)
+ * (include/lbfgs_general.hpp:202-405): d_res (may be NULL) receives the
+ * minimum point, *f (may be NULL) the value, *iterations and *evaluations the
+ * counts of the call.  cb may be NULL. */
+int cmbl_lbfgs_minimize(cmbl_lbfgs* lb, double epsilon, double gnorm_tol, int max_iter, double* d_res, double* f,
+                        int* iterations, int* evaluations, cmbl_lbfgs_callback cb, void* user);
+/* LBFGS_General::getGradient. */
+int cmbl_lbfgs_gradient(cmbl_lbfgs* lb, double* d_g);
+/* Current value, gradient norm, total iterations and evaluations. */
+int cmbl_lbfgs_state(const cmbl_lbfgs* lb, double* f, double* gnorm, int* iter, long* evaluations);
+
+typedef struct cmbl_hmc cmbl_hmc;
+/* HMCGeneral(factory, f, starting, masses, tauMax, nMax, seed)
+ * (include/hmc_general.hpp:97-127).  seed != 0 (the reference's 0 means
+ * time(0)).  Without CMBL_HMC_FIXED each trajectory draws tau = U tau_max and
+ * n = ceil(U n_max) as the reference does (:158-164); with it, tau = tau_max
+ * and n = n_max and those two draws are not made.  Momenta are the
+ * reference's draws (BasicLBFGSFunc::whitenoise with seed + 1, + 2, ...,
+ * source/lbfgs.cpp:401-416), made on host threads up to eight trajectories
+ * ahead. */
+int cmbl_hmc_create(cmbl_hmc** out, cmbl_map* map, const double* d_start, const double* d_masses, double tau_max,
+                    int n_max, int seed, int flags);
+void cmbl_hmc_destroy(cmbl_hmc* hm);
+/* Callback of HMCGeneral::run: device pointer to the current point and its -2lnL. */
+typedef void (*cmbl_hmc_callback)(void* user, const double* d_x, double like);
+/* HMCGeneral::run(maxIters, callback) (include/hmc_general.hpp:135-231). */
+int cmbl_hmc_run(cmbl_hmc* hm, int max_iters, cmbl_hmc_callback cb, void* user);
+/* Current -2lnL and totals: trajectories, accepted, likelihood evaluations. */
+int cmbl_hmc_state(const cmbl_hmc* hm, double* like, long* trajectories, long* accepted, long* evaluations);
+int cmbl_hmc_position(cmbl_hmc* hm, double* d_out);
+
+/* The More-Thuente line search of the device-resident L-BFGS as a state
+ * machine (moreThuenteSearch, include/line_search.hpp:160-272, with its
+ * step rule moreThuenteStep, :12-157): begin() returns the first trial step;
+ * feed() takes the function value and slope along the search direction at
+ * the last trial and returns info = 0 with the next step in *stp, or the
+ * search's result code (1: sufficient decrease and curvature; 2-6: stopped
+ * by the interval width, step bounds, evaluation count or rounding) with the
+ * final step in *stp.  The reference passes ftol 0.1, gtol 0.5, xtol 1e-15,
+ * stpmin 1e-15, stpmax 1e15, maxfev 100 (include/lbfgs_general.hpp:257-262).
+ * Host only; no GPU. */
+typedef struct cmbl_line_search cmbl_line_search;
+int cmbl_line_search_create(cmbl_line_search** out, double ftol, double gtol, double xtol, double stpmin,
+                            double stpmax, int maxfev);
+void cmbl_line_search_destroy(cmbl_line_search* ls);
+int cmbl_line_search_begin(cmbl_line_search* ls, double f0, double dg0, double stp0, double* stp);
+int cmbl_line_search_feed(cmbl_line_search* ls, double f, double dg, double* stp, int* info);
 
 /* ---------------------------------------------------------------- inputs (host) */
 /* Simulate::simulateAlm (source/simulate.cpp:162-192): mt19937(seed) +
@@ -250,7 +340,9 @@ int cmbl_stream_synchronize(void* stream);
 /* BLAS-1 on device vectors of n doubles, for a device-resident LargeVector
  * (the operations of BasicLargeVector, source/lbfgs.cpp:323-399).  All are
  * asynchronous on stream except the reductions, which block and return the
- * value through a host pointer. */
+ * value through a host pointer.  y and x may be the same vector.  The dot
+ * products use a workspace per (device, stream), allocated at the first dot
+ * on that stream and kept until cmbl_vec_release_workspaces. */
 int cmbl_vec_copy(double* d_y, const double* d_x, double c, long n, void* stream);   /* y = c x */
 int cmbl_vec_set(double* d_y, double v, long n, void* stream);                       /* y = v */
 int cmbl_vec_axpy(double* d_y, const double* d_x, double c, long n, void* stream);   /* y += c x */
@@ -260,6 +352,8 @@ int cmbl_vec_pow(double* d_y, double p, long n, void* stream);                  
 int cmbl_vec_dot(const double* d_x, const double* d_y, long n, double* out, void* stream);
 /* Device-side dot product: *d_out = x . y, asynchronous. */
 int cmbl_vec_dot_device(const double* d_x, const double* d_y, long n, double* d_out, void* stream);
+/* Free every dot-product workspace (waits for each device first). */
+int cmbl_vec_release_workspaces(void);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,10 @@
 //   cmbl::DeviceLargeVectorFactory, cmbl::DeviceMapFunction
 //                            the Factory and Function concepts
 //                            (include/lbfgs_general.hpp:33-53, include/hmc_general.hpp:39-61).
+//   cmbl::DeviceLBFGS, cmbl::DeviceHMC
+//                            LBFGS_General (include/lbfgs_general.hpp:58-405) and
+//                            HMCGeneral (include/hmc_general.hpp:64-246) with the
+//                            whole loop on the GPU (cmbl_lbfgs / cmbl_hmc).
 //
 // Values are -2lnL and derivatives of -2lnL, as in the reference.  Failures
 // throw cmbl::Error (a std::exception, like StandardException).
@@ -82,7 +86,8 @@ private:
 };
 
 /// -2lnL(a_lm) and its gradient behind the reference's function interfaces.
-/// The last point, value and gradient are cached by contents, so that
+/// The last point, value and gradient are cached by contents and by the
+/// map's generation (any setBeam / setCl / setData invalidates them), so that
 /// value + derivative at one point (LBFGS, line searches) and the n
 /// per-component calls of HMC::calculateDerivs cost one device evaluation.
 class MapLikelihood : public Math::RealFunctionMultiDim,
@@ -124,12 +129,14 @@ private:
     void update(const double* x, long n) const
     {
         if (n != n_) throw Error(CMBL_ERR_ARG, "expected " + std::to_string(n_) + " parameters");
-        if (valid_ && std::memcmp(x, x_.data(), sizeof(double) * n) == 0) return;
+        const long gen = cmbl_map_generation(ctx_.get());
+        if (valid_ && gen == gen_ && std::memcmp(x, x_.data(), sizeof(double) * n) == 0) return;
         x_.assign(x, x + n);
         g_.resize(n);
         valid_ = false;
         raise_on_error(cmbl_value_and_grad(ctx_.get(), x_.data(), &f_, g_.data()));
         valid_ = true;
+        gen_ = gen;
         ++evals_;
     }
     MapContext& ctx_;
@@ -137,6 +144,7 @@ private:
     mutable std::vector<double> x_, g_;
     mutable double f_ = 0;
     mutable bool valid_ = false;
+    mutable long gen_ = -1;
     mutable long evals_ = 0;
 };
 
@@ -249,12 +257,17 @@ public:
 private:
     void eval()
     {
-        if (fresh_) return;
+        const long gen = cmbl_map_generation(ctx_.get());
+        if (fresh_ && gen == gen_) return;
         raise_on_error(cmbl_value_and_grad_device(ctx_.get(), x_.data(), d_f_, g_.data(), stream_));
         raise_on_error(cmbl_stream_synchronize(stream_));
         raise_on_error(cmbl_copy_to_host(&f_, d_f_, sizeof(double)));
-        if (!std::isfinite(f_)) throw Error(CMBL_ERR_NONFINITE, "-2lnL is not finite");
+        if (!std::isfinite(f_)) {
+            (void)cmbl_map_check_finite(ctx_.get(), stream_);  // clears the map's flag
+            throw Error(CMBL_ERR_NONFINITE, "-2lnL is not finite");
+        }
         fresh_ = true;
+        gen_ = gen;
         ++evals_;
     }
     MapContext& ctx_;
@@ -263,7 +276,108 @@ private:
     double* d_f_ = nullptr;
     double f_ = 0;
     bool fresh_ = false;
+    long gen_ = -1;
     long evals_ = 0;
+};
+
+/// LBFGS_General with the More-Thuente line search, run on the GPU: the
+/// vectors never leave device memory, the vector work of each phase is
+/// captured in HIP graphs, and the iterates equal those of the reference's
+/// LBFGS_General on DeviceLargeVector / DeviceMapFunction bit for bit.
+class DeviceLBFGS
+{
+public:
+    DeviceLBFGS(MapContext& ctx, const DeviceLargeVector& starting, int m = 10, bool graphs = true)
+    {
+        raise_on_error(cmbl_stream_synchronize(starting.stream()));
+        raise_on_error(cmbl_lbfgs_create(&lb_, ctx.get(), starting.data(), m, graphs ? 0 : CMBL_LOOP_EAGER));
+    }
+    ~DeviceLBFGS() { cmbl_lbfgs_destroy(lb_); }
+    DeviceLBFGS(const DeviceLBFGS&) = delete;
+    DeviceLBFGS& operator=(const DeviceLBFGS&) = delete;
+
+    void setStarting(const DeviceLargeVector& starting)
+    {
+        raise_on_error(cmbl_stream_synchronize(starting.stream()));
+        raise_on_error(cmbl_lbfgs_set_starting(lb_, starting.data()));
+    }
+    double minimize(DeviceLargeVector* res, double epsilon = 1e-3, double gNormTol = 1e-5, int maxIter = 1000000)
+    {
+        double f = 0;
+        raise_on_error(cmbl_lbfgs_minimize(lb_, epsilon, gNormTol, maxIter, res ? res->data() : nullptr, &f,
+                                           &iterations_, &evaluations_, nullptr, nullptr));
+        return f;
+    }
+    /// callback(iter, f, gradNorm, d_x, d_g, d_z) after every iteration.
+    template <class CallBack>
+    double minimize(DeviceLargeVector* res, double epsilon, double gNormTol, int maxIter, CallBack* callback)
+    {
+        double f = 0;
+        raise_on_error(cmbl_lbfgs_minimize(lb_, epsilon, gNormTol, maxIter, res ? res->data() : nullptr, &f,
+                                           &iterations_, &evaluations_, callback ? &call<CallBack> : nullptr,
+                                           callback));
+        return f;
+    }
+    void getGradient(DeviceLargeVector* g) { raise_on_error(cmbl_lbfgs_gradient(lb_, g->data())); }
+    int iterations() const { return iterations_; }
+    int evaluations() const { return evaluations_; }
+
+private:
+    template <class CallBack>
+    static void call(void* user, int iter, double f, double gn, const double* x, const double* g, const double* z)
+    {
+        (*static_cast<CallBack*>(user))(iter, f, gn, x, g, z);
+    }
+    cmbl_lbfgs* lb_ = nullptr;
+    int iterations_ = 0, evaluations_ = 0;
+};
+
+/// HMCGeneral run on the GPU: one HIP graph per trajectory, the reference's
+/// momentum draws made ahead on host threads.  With fixedSteps every
+/// trajectory takes step tauMax and nMax steps.
+class DeviceHMC
+{
+public:
+    DeviceHMC(MapContext& ctx, const DeviceLargeVector& starting, const DeviceLargeVector& masses, double tauMax,
+              int nMax, int seed, bool fixedSteps = false, bool graphs = true)
+    {
+        raise_on_error(cmbl_stream_synchronize(starting.stream()));
+        raise_on_error(cmbl_stream_synchronize(masses.stream()));
+        raise_on_error(cmbl_hmc_create(&hm_, ctx.get(), starting.data(), masses.data(), tauMax, nMax, seed,
+                                       (fixedSteps ? CMBL_HMC_FIXED : 0) | (graphs ? 0 : CMBL_LOOP_EAGER)));
+    }
+    ~DeviceHMC() { cmbl_hmc_destroy(hm_); }
+    DeviceHMC(const DeviceHMC&) = delete;
+    DeviceHMC& operator=(const DeviceHMC&) = delete;
+
+    /// callback(d_x, like) at the start and after every trajectory.
+    template <class CallBack>
+    void run(int maxIters, CallBack* callback)
+    {
+        raise_on_error(cmbl_hmc_run(hm_, maxIters, callback ? &call<CallBack> : nullptr, callback));
+    }
+    void run(int maxIters) { raise_on_error(cmbl_hmc_run(hm_, maxIters, nullptr, nullptr)); }
+    long accepted() const
+    {
+        long a = 0;
+        raise_on_error(cmbl_hmc_state(hm_, nullptr, nullptr, &a, nullptr));
+        return a;
+    }
+    long evaluations() const
+    {
+        long e = 0;
+        raise_on_error(cmbl_hmc_state(hm_, nullptr, nullptr, nullptr, &e));
+        return e;
+    }
+    void position(DeviceLargeVector* x) { raise_on_error(cmbl_hmc_position(hm_, x->data())); }
+
+private:
+    template <class CallBack>
+    static void call(void* user, const double* x, double like)
+    {
+        (*static_cast<CallBack*>(user))(x, like);
+    }
+    cmbl_hmc* hm_ = nullptr;
 };
 
 /// The Gibbs constrained-realisation system of CMBGibbsSampler::generateSignal

@@ -368,6 +368,12 @@ def ref():
         r.ref_uniform_draws.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_long, _dp]
         r.ref_lambda_table.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
         r.ref_w3j2_table.argtypes = [ctypes.c_int, _dp]
+        r.ref_phi.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_double)]
+        r.ref_mt_trace.argtypes = [ctypes.c_int] + [ctypes.c_double] * 6 + [ctypes.c_int, _dp, ctypes.c_int,
+                                                                            ctypes.POINTER(ctypes.c_int),
+                                                                            ctypes.POINTER(ctypes.c_double)]
+        r.ref_mt_trace.restype = ctypes.c_int
         r.ref_brute_synthesis.argtypes = [ctypes.c_int, _dp, ctypes.c_void_p, ctypes.c_long, _dp, _dp, _dp]
         _ref = r
     return _ref
@@ -393,6 +399,23 @@ def ref_brute_synthesis(L, x, beam, theta, phi):
     ref().ref_brute_synthesis(L, np.ascontiguousarray(x, dtype=np.float64), _ptr(b), len(theta),
                               theta, phi, out)
     return out
+
+
+def ref_phi(fn, a):
+    """More-Thuente test function fn (1..6) and its derivative at a."""
+    f, g = ctypes.c_double(), ctypes.c_double()
+    ref().ref_phi(fn, a, ctypes.byref(f), ctypes.byref(g))
+    return f.value, g.value
+
+
+def ref_mt_trace(fn, stp0, ftol=0.1, gtol=0.5, xtol=1e-15, stpmin=1e-15, stpmax=1e15, maxfev=100):
+    """The reference's moreThuenteSearch on test function fn from 0 along +1:
+    (info, trial steps, final step)."""
+    trials = np.zeros(maxfev + 1)
+    n, stp = ctypes.c_int(), ctypes.c_double()
+    info = ref().ref_mt_trace(fn, stp0, ftol, gtol, xtol, stpmin, stpmax, maxfev, trials, trials.size,
+                              ctypes.byref(n), ctypes.byref(stp))
+    return info, list(trials[:n.value]), stp.value
 
 
 def ref_w3j2_table(L):

@@ -14,6 +14,12 @@
 //   hmc_device     Math::HMCGeneral on device vectors
 //   cg             Math::ConjugateGradient on cmbl::GibbsSystem (the Gibbs
 //                  constrained-realisation system), against the device solve
+//   lbfgs_compare  Math::LBFGS_General on device vectors and the library's
+//                  device-resident cmbl::DeviceLBFGS from the same start:
+//                  per-iteration (f, gradient norm) and final x of both
+//   hmc_compare    Math::HMCGeneral on device vectors and cmbl::DeviceHMC
+//                  (same masses, tauMax, nMax, seed): the chain's -2lnL per
+//                  step and final point of both
 // Prints one JSON object on stdout.
 #include <chrono>
 #include <cmath>
@@ -80,6 +86,65 @@ void print_vec(const char* name, const std::vector<double>& v)
     std::printf("\"%s\": [", name);
     for (size_t i = 0; i < v.size(); ++i) std::printf("%s%.17g", i ? "," : "", v[i]);
     std::printf("]");
+}
+
+struct LbTrace {
+    std::vector<double> f, gn;
+    bool full = false;
+    long n = 0;
+    std::vector<double> xs, gs, zs;  // per iteration, when full
+    void keep(std::vector<double>& dst, const double* d)
+    {
+        std::vector<double> h(n);
+        cmbl::raise_on_error(cmbl_copy_to_host(h.data(), d, sizeof(double) * n));
+        dst.insert(dst.end(), h.begin(), h.end());
+    }
+    void operator()(int, double fv, double g, const cmbl::DeviceLargeVector& x, const cmbl::DeviceLargeVector& gr,
+                    const cmbl::DeviceLargeVector& z)
+    {
+        f.push_back(fv);
+        gn.push_back(g);
+        if (full) {
+            keep(xs, x.data());
+            keep(gs, gr.data());
+            keep(zs, z.data());
+        }
+    }
+    void operator()(int, double fv, double g, const double* x, const double* gr, const double* z)
+    {
+        f.push_back(fv);
+        gn.push_back(g);
+        if (full) {
+            keep(xs, x);
+            keep(gs, gr);
+            keep(zs, z);
+        }
+    }
+};
+
+struct ChainTrace {
+    long n = 0;
+    std::vector<double> like, last;
+    void operator()(const cmbl::DeviceLargeVector& x, double l)
+    {
+        like.push_back(l);
+        x.download(&last);
+    }
+    void operator()(const double* x, double l)
+    {
+        like.push_back(l);
+        last.resize(n);
+        cmbl::raise_on_error(cmbl_copy_to_host(last.data(), x, sizeof(double) * n));
+    }
+};
+
+std::vector<double> hmc_masses(const Inputs& in, int lmax)
+{
+    std::vector<double> m((size_t)(lmax + 1) * (lmax + 1));
+    // mass ~ inverse prior variance keeps the leapfrog stable across l
+    for (int l = 0; l <= lmax; ++l)
+        for (int k = 0; k < 2 * l + 1; ++k) m[(size_t)l * l + k] = l >= 2 ? 2.0 / in.cl[l] : 1.0;
+    return m;
 }
 
 struct ChainStats {
@@ -180,6 +245,84 @@ int main(int argc, char** argv)
             std::printf("{\"mode\": \"hmc_device\", \"samples\": %ld, \"evals\": %ld, \"mean_like\": %.17g, "
                         "\"seconds\": %.6f, ", cb.n, func.evaluations(), cb.like_sum / cb.n, t1 - t0);
             print_vec("mean", cb.mean);
+            std::printf("}\n");
+        } else if (mode == "lbfgs_compare" || mode == "lbfgs_compare_full") {
+            // the reference's LBFGS_General on device vectors ...
+            cmbl::DeviceLargeVectorFactory factory(n);
+            cmbl::DeviceMapFunction func(ctx);
+            cmbl::DeviceLargeVector start(n);
+            start.upload(in.x0);
+            Math::LBFGS_General<cmbl::DeviceLargeVector, cmbl::DeviceLargeVectorFactory, cmbl::DeviceMapFunction> ref(
+                &factory, &func, start, 10);
+            cmbl::DeviceLargeVector res(n), res2(n);
+            LbTrace tr, tr2;
+            tr.full = tr2.full = mode == "lbfgs_compare_full";
+            tr.n = tr2.n = n;
+            const double t0 = now_s();
+            const double f = ref.minimize(&res, 1e-14, 1e-8, iters, &tr);
+            const double t1 = now_s();
+            // ... and the library's device-resident loop from the same start
+            cmbl::DeviceLBFGS dev(ctx, start, 10);
+            const double f2 = dev.minimize(&res2, 1e-14, 1e-8, iters, &tr2);
+            const double t2 = now_s();
+            std::vector<double> h, h2;
+            res.download(&h);
+            res2.download(&h2);
+            std::printf("{\"mode\": \"lbfgs_compare\", \"f_ref\": %.17g, \"f_dev\": %.17g, \"evals_ref\": %ld, "
+                        "\"evals_dev\": %d, \"seconds_ref\": %.6f, \"seconds_dev\": %.6f, ",
+                        f, f2, func.evaluations(), dev.evaluations(), t1 - t0, t2 - t1);
+            print_vec("trace_f_ref", tr.f);
+            std::printf(", ");
+            print_vec("trace_f_dev", tr2.f);
+            std::printf(", ");
+            print_vec("trace_gn_ref", tr.gn);
+            std::printf(", ");
+            print_vec("trace_gn_dev", tr2.gn);
+            std::printf(", ");
+            print_vec("x_ref", h);
+            std::printf(", ");
+            print_vec("x_dev", h2);
+            if (tr.full) {
+                const char* names[6] = {"xs_ref", "xs_dev", "gs_ref", "gs_dev", "zs_ref", "zs_dev"};
+                const std::vector<double>* v[6] = {&tr.xs, &tr2.xs, &tr.gs, &tr2.gs, &tr.zs, &tr2.zs};
+                for (int k = 0; k < 6; ++k) {
+                    std::printf(", ");
+                    print_vec(names[k], *v[k]);
+                }
+            }
+            std::printf("}\n");
+        } else if (mode == "hmc_compare") {
+            cmbl::DeviceLargeVectorFactory factory(n);
+            cmbl::DeviceMapFunction func(ctx);
+            cmbl::DeviceLargeVector start(n), masses(n);
+            start.upload(in.x0);
+            masses.upload(hmc_masses(in, lmax));
+            ChainTrace tr, tr2;
+            tr.n = tr2.n = n;
+            double t0, t1, t2;
+            std::vector<double> h2;
+            {
+                Math::HMCGeneral<cmbl::DeviceLargeVector, cmbl::DeviceLargeVectorFactory, cmbl::DeviceMapFunction>
+                    ref(&factory, &func, start, masses, 0.5, 10, 778);
+                t0 = now_s();
+                ref.run(iters, &tr);
+                t1 = now_s();
+            }
+            cmbl::DeviceHMC dev(ctx, start, masses, 0.5, 10, 778);
+            dev.run(iters, &tr2);
+            t2 = now_s();
+            cmbl::DeviceLargeVector xd(n);
+            dev.position(&xd);
+            xd.download(&h2);
+            std::printf("{\"mode\": \"hmc_compare\", \"accepted_dev\": %ld, \"seconds_ref\": %.6f, "
+                        "\"seconds_dev\": %.6f, ", dev.accepted(), t1 - t0, t2 - t1);
+            print_vec("like_ref", tr.like);
+            std::printf(", ");
+            print_vec("like_dev", tr2.like);
+            std::printf(", ");
+            print_vec("x_ref", tr.last);
+            std::printf(", ");
+            print_vec("x_dev", h2);
             std::printf("}\n");
         } else if (mode == "cg") {
             // Gibbs system on the same data: mask 20 deg cut, sigma 10, white

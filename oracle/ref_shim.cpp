@@ -9,6 +9,7 @@
 //   Math::Legendre / AssociatedLegendre   include/legendre.hpp:26-103
 //   Math::GaussianGenerator               include/random.hpp:32-52
 //   Math::UniformRealGenerator            include/random.hpp:10-30
+//   Math::moreThuenteSearch               include/line_search.hpp:12-272
 #include <cmath>
 #include <vector>
 
@@ -16,6 +17,7 @@
 #include <legendre.hpp>
 #include <random.hpp>
 #include <wigner_3j.hpp>
+#include <line_search.hpp>
 
 extern "C" {
 
@@ -111,6 +113,107 @@ void ref_w3j2_table(int L, double* out)
     W3jCollect c{L, out};
     Math::Wigner3JZeroM<W3jCollect> w(c);
     w.calculate(L * 3 / 2 + 1);
+}
+
+// One-dimensional test functions of More and Thuente (ACM TOMS 20, 1994,
+// section 5) and their derivatives, for the line-search comparison.
+static void phi(int fn, double a, double* f, double* g)
+{
+    switch (fn) {
+    case 1: {
+        const double b = 2.0;
+        *f = -a / (a * a + b);
+        *g = (a * a - b) / ((a * a + b) * (a * a + b));
+        break;
+    }
+    case 2: {
+        const double b = 0.004, t = a + b;
+        *f = std::pow(t, 5) - 2.0 * std::pow(t, 4);
+        *g = 5.0 * std::pow(t, 4) - 8.0 * std::pow(t, 3);
+        break;
+    }
+    case 3: {
+        const double b = 0.01, l = 39.0, pi = 3.141592653589793;
+        double f0, g0;
+        if (a <= 1 - b) {
+            f0 = 1 - a;
+            g0 = -1;
+        } else if (a >= 1 + b) {
+            f0 = a - 1;
+            g0 = 1;
+        } else {
+            f0 = (a - 1) * (a - 1) / (2 * b) + b / 2;
+            g0 = (a - 1) / b;
+        }
+        *f = f0 + 2 * (1 - b) / (l * pi) * std::sin(l * pi * a / 2);
+        *g = g0 + (1 - b) * std::cos(l * pi * a / 2);
+        break;
+    }
+    default: {  // 4, 5, 6
+        const double b1 = fn == 5 ? 0.01 : 0.001, b2 = fn == 6 ? 0.01 : 0.001;
+        const double g1 = std::sqrt(1 + b1 * b1) - b1, g2 = std::sqrt(1 + b2 * b2) - b2;
+        const double r1 = std::sqrt((1 - a) * (1 - a) + b2 * b2), r2 = std::sqrt(a * a + b1 * b1);
+        *f = g1 * r1 + g2 * r2;
+        *g = g1 * (a - 1) / r1 + g2 * a / r2;
+        break;
+    }
+    }
+}
+
+void ref_phi(int fn, double a, double* f, double* g) { phi(fn, a, f, g); }
+
+struct V1 {
+    double v = 0;
+    void copy(const V1& o, double c = 1.) { v = c * o.v; }
+    double dotProduct(const V1& o) const { return v * o.v; }
+    void add(const V1& o, double c = 1.) { v += c * o.v; }
+};
+
+struct Phi1 {
+    int fn;
+    double x;
+    double* trials;
+    int n, cap;
+    Phi1(int f, double* t, int c) : fn(f), x(0.0), trials(t), n(0), cap(c) {}
+    void set(const V1& p)
+    {
+        x = p.v;
+        if (n < cap) trials[n] = x;
+        ++n;
+    }
+    double value()
+    {
+        double f, g;
+        phi(fn, x, &f, &g);
+        return f;
+    }
+    void derivative(V1* res)
+    {
+        double f, g;
+        phi(fn, x, &f, &g);
+        res->v = g;
+    }
+};
+
+// The reference's moreThuenteSearch from a = 0 along s = 1: trial steps into
+// trials (at most cap), their count into *ntrials, the final step into *stp;
+// returns info.
+int ref_mt_trace(int fn, double stp0, double ftol, double gtol, double xtol, double stpmin, double stpmax, int maxfev,
+                 double* trials, int cap, int* ntrials, double* stp)
+{
+    Phi1 func(fn, trials, cap);
+    V1 x0, g0, s, x, g;
+    s.v = 1.0;
+    double f0, d0;
+    phi(fn, 0.0, &f0, &d0);
+    g0.v = d0;
+    double f = f0, st = stp0;
+    int nfev = 0;
+    const int info = Math::moreThuenteSearch(&func, x0, f, g0, s, st, ftol, gtol, xtol, stpmin, stpmax, maxfev, &x, &g,
+                                             nfev);
+    *ntrials = func.n;
+    *stp = st;
+    return info;
 }
 
 }  // extern "C"

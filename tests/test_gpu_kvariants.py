@@ -1,7 +1,7 @@
 """GPU: every ring-pairs-per-lane variant of the Legendre kernels against the
 oracle.  The CMBL_K1 / CMBL_K5 overrides (read when a map is created) pick
-K = 1, 2, 4 or 8 ring pairs per lane for the synthesis and the adjoint; the
-default choice uses K = 4 for the adjoint only from nside 1024 up.  At lmax
+K = 1, 2, 4 or 8 ring pairs per lane for the synthesis and the adjoint,
+whatever the default choice for the map would be.  At lmax
 = 3 nside - 1 most (m, ring block) items start with scaled lanes that join
 at row ends, so the scaled path (zero adjoint inputs loaded when a lane
 turns live, synthesis sums cleared at that point) carries much of the work.

@@ -60,6 +60,7 @@ _SIGS = {
     "cmbl_map_check_finite": ([_vp, _vp], ctypes.c_int),
     "cmbl_batch_check_finite": ([_vp, _vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "cmbl_vec_release_workspaces": ([], ctypes.c_int),
+    "cmbl_vec_dot_device_static": ([_vp, _vp, ctypes.c_long, _vp, _vp], ctypes.c_int),
     "cmbl_lbfgs_create": ([ctypes.POINTER(_vp), _vp, _vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "cmbl_lbfgs_destroy": ([_vp], None),
     "cmbl_lbfgs_set_starting": ([_vp, _vp], ctypes.c_int),

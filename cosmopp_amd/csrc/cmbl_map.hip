@@ -1456,6 +1456,45 @@ static int dot_work(hipStream_t s, double** out)
     return CMBL_OK;
 }
 
+}  // extern "C"
+
+namespace cmbl {
+// Scratch of cmbl_vec_dot_device_static: the dot product's kDotParts partial
+// sums (8 KiB), one copy on every GPU the library runs on, held until the
+// process ends.  Host code never uses the array's name as an address: its
+// device address is looked up once per device with hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[kDotParts];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+static int static_scratch(double** out)
+{
+    int dev = 0;
+    CMBL_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(CMBL_ERR_ARG, "device index beyond the static scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        CMBL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(CMBL_ERR_HIP, "hipGetSymbolAddress returned no address for the dot scratch");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return CMBL_OK;
+}
+}  // namespace cmbl
+
+extern "C" {
+
+int cmbl_vec_dot_device_static(const double* x, const double* y, long n, double* d_out, void* s)
+{
+    if (!x || !y || !d_out || n < 0) return fail(CMBL_ERR_ARG, "bad dot arguments");
+    double* w = nullptr;
+    CMBL_TRY(static_scratch(&w));
+    return launch_dot(x, y, n, d_out, w, (hipStream_t)s);
+}
+
 int cmbl_vec_release_workspaces(void)
 {
     std::lock_guard<std::mutex> lock(g_dot_mu);

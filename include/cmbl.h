@@ -354,6 +354,16 @@ int cmbl_vec_dot(const double* d_x, const double* d_y, long n, double* out, void
 int cmbl_vec_dot_device(const double* d_x, const double* d_y, long n, double* d_out, void* stream);
 /* Free every dot-product workspace (waits for each device first). */
 int cmbl_vec_release_workspaces(void);
+/* cmbl_vec_dot_device without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Its scratch (1024 partial sums,
+ * 8 KiB) is a static array in device memory, one copy per GPU, whose
+ * address is looked up on the first call on each device; the scratch does
+ * not grow with n, so any n >= 0 is accepted.  Calls on one GPU share that
+ * scratch: calls that overlap (different streams, or several threads) race
+ * and must be ordered by the caller.  The result equals cmbl_vec_dot_device's
+ * bit for bit (same kernels, same summation order). */
+int cmbl_vec_dot_device_static(const double* d_x, const double* d_y, long n, double* d_out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -75,6 +75,7 @@ _SIGS = {
     "cmbl_hmc_state": ([_vp, _dp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),
                         ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
     "cmbl_hmc_position": ([_vp, _vp], ctypes.c_int),
+    "cmbl_hmc_last_trajectory": ([_vp, _dp, _dp, _dp], ctypes.c_int),
     "cmbl_line_search_create": ([ctypes.POINTER(_vp), ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_int], ctypes.c_int),
     "cmbl_line_search_destroy": ([_vp], None),
@@ -661,6 +662,12 @@ class HMC:
 
     def position(self, out_ptr):
         _check(lib().cmbl_hmc_position(self._h, out_ptr))
+
+    def last_trajectory(self):
+        """(-2lnL at the end point, kinetic term at start, kinetic term at end)."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().cmbl_hmc_last_trajectory(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
 
 
 class LineSearch:

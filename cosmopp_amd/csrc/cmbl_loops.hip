@@ -42,11 +42,9 @@ namespace cmbl {
 constexpr int kPassThreads = 256;
 constexpr int kMaxPassDots = 3;
 
-// Arrival counter and partial sums of a fused pass.  The counter returns to 0
-// after every pass (atomicInc wraps at the block count).
+// Block sums of a fused pass, kDotParts per dot product.
 struct PassSync {
-    double* part;     // [kMaxPassDots * kDotParts]
-    unsigned* count;  // [1]
+    double* part;  // [kMaxPassDots * kDotParts]
 };
 
 // The plain dot product's final stage (1024 threads, thread t holding
@@ -69,39 +67,51 @@ __device__ double final_replay(const double* part, double* sh16)
     return t;
 }
 
-// One pass over n elements: Op::element updates element i and accumulates
-// its dot-product terms (a[d] = fma(x, y, a[d])); the last block to finish
-// reduces the partial sums and calls Op::finish on the kDots results.
-// Launched with exactly kDotParts blocks.
+// One pass over n elements: Op::fetch loads element i's inputs, Op::apply
+// updates element i and accumulates its dot-product terms
+// (a[d] = fma(x, y, a[d])), and each block stores its kDots block sums.
+// Launched with exactly kDotParts blocks.  Each thread fetches four of its
+// elements before applying them in their grid-stride order, so more loads
+// are in flight while the accumulation order stays that of the plain dot
+// product.
 template <class Op>
 __global__ __launch_bounds__(kPassThreads) void k_pass(Op op, PassSync ps)
 {
     constexpr int ND = Op::kDots;
     __shared__ double sh[kPassThreads / 64];
-    __shared__ double sh16[kDotParts / 64];
-    __shared__ int last;
     op.load();
     double a[ND > 0 ? ND : 1];
 #pragma unroll
     for (int d = 0; d < (ND > 0 ? ND : 1); ++d) a[d] = 0.0;
-    for (long i = blockIdx.x * (long)kPassThreads + threadIdx.x; i < op.n; i += (long)gridDim.x * kPassThreads)
-        op.element(i, a);
-    if (ND == 0) return;
-    double t[ND > 0 ? ND : 1];
+    const long stride = (long)gridDim.x * kPassThreads;
+    long i = blockIdx.x * (long)kPassThreads + threadIdx.x;
+    for (; i + 3 * stride < op.n; i += 4 * stride) {
+        typename Op::In v[4];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) t[d] = block_sum<kPassThreads>(a[d], sh);
-    if (threadIdx.x == 0) {
-        for (int d = 0; d < ND; ++d) ps.part[d * kDotParts + blockIdx.x] = t[d];
-        // release the partial sums, then count the block in; the block that
-        // counts last acquires every block's partial sums
-        __threadfence();
-        const bool is_last = atomicInc(ps.count, gridDim.x - 1) == gridDim.x - 1;
-        if (is_last) __threadfence();
-        last = is_last;
+        for (int u = 0; u < 4; ++u) v[u] = op.fetch(i + u * stride);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) op.apply(i + u * stride, v[u], a);
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+    for (; i < op.n; i += stride) op.apply(i, op.fetch(i), a);
+    if (ND == 0) return;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const double t = block_sum<kPassThreads>(a[d], sh);
+        if (threadIdx.x == 0) ps.part[d * kDotParts + blockIdx.x] = t;
+    }
+}
+
+// The pass's final stage, one block: the kDots results from the block sums,
+// then Op::finish.  (Completing the sums inside the pass in its last block
+// would need a device-scope release by every block, i.e. an L2 write-back per
+// block on a chip whose XCDs have separate L2s; the kernel boundary gives
+// the same ordering once.)
+template <class Op>
+__global__ __launch_bounds__(kPassThreads) void k_pass_final(Op op, PassSync ps)
+{
+    constexpr int ND = Op::kDots;
+    __shared__ double sh16[kDotParts / 64];
+    op.load();
     double r[ND > 0 ? ND : 1];
     for (int d = 0; d < ND; ++d) r[d] = final_replay(ps.part + d * kDotParts, sh16);
     if (threadIdx.x == 0) op.finish(r);
@@ -111,6 +121,7 @@ template <class Op>
 static int run_pass(const Op& op, const PassSync& ps, hipStream_t s)
 {
     hipLaunchKernelGGL(k_pass<Op>, dim3(kDotParts), dim3(kPassThreads), 0, s, op, ps);
+    if (Op::kDots > 0) hipLaunchKernelGGL(k_pass_final<Op>, dim3(1), dim3(kPassThreads), 0, s, op, ps);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }
@@ -146,13 +157,16 @@ struct OpStart {
     static constexpr int kDots = 1;
     LbDev d;
     long n;
+    struct In {
+        double x, g;
+    };
     __device__ void load() {}
-    __device__ void element(long i, double* a)
+    __device__ In fetch(long i) const { return In{d.x[i], d.g[i]}; }
+    __device__ void apply(long i, const In& v, double* a)
     {
-        const double xv = d.x[i], gv = d.g[i];
-        d.x0[i] = 1.0 * xv;
-        d.g0[i] = 1.0 * gv;
-        a[0] = fma(gv, gv, a[0]);
+        d.x0[i] = 1.0 * v.x;
+        d.g0[i] = 1.0 * v.g;
+        a[0] = fma(v.g, v.g, a[0]);
     }
     __device__ void finish(const double* r)
     {
@@ -176,18 +190,21 @@ struct OpHistory {
         sn = d.S + (long)k * d.n;
         yn = d.Y + (long)k * d.n;
     }
-    __device__ void element(long i, double* a)
+    struct In {
+        double x, x0, g, g0;
+    };
+    __device__ In fetch(long i) const { return In{d.x[i], d.x0[i], d.g[i], d.g0[i]}; }
+    __device__ void apply(long i, const In& v, double* a)
     {
-        const double xv = d.x[i], gv = d.g[i];
-        const double s = xv - d.x0[i];
-        const double y = gv - d.g0[i];
+        const double s = v.x - v.x0;
+        const double y = v.g - v.g0;
         sn[i] = s;
         yn[i] = y;
-        d.x0[i] = 1.0 * xv;
-        d.g0[i] = 1.0 * gv;
+        d.x0[i] = 1.0 * v.x;
+        d.g0[i] = 1.0 * v.g;
         a[0] = fma(s, y, a[0]);
         a[1] = fma(y, y, a[1]);
-        a[2] = fma(s, gv, a[2]);
+        a[2] = fma(s, v.g, a[2]);
     }
     __device__ void finish(const double* r)
     {
@@ -226,17 +243,20 @@ struct OpLoop1 {
         c = -d.alpha[ki];
         h = *d.h0k;
     }
-    __device__ void element(long e, double* a)
+    struct In {
+        double y, q, s;
+    };
+    __device__ In fetch(long e) const { return In{yi[e], qin[e], kn >= 0 ? sn[e] : 0.0}; }
+    __device__ void apply(long e, const In& v, double* a)
     {
-        const double y = yi[e];
-        const double qv = fma(c, y, qin[e]);
+        const double qv = fma(c, v.y, v.q);
         if (kn >= 0) {
             d.q[e] = qv;
-            a[0] = fma(sn[e], qv, a[0]);
+            a[0] = fma(v.s, qv, a[0]);
         } else {
             const double zv = h * qv;
             d.z[e] = zv;
-            a[0] = fma(y, zv, a[0]);
+            a[0] = fma(v.y, zv, a[0]);
         }
     }
     __device__ void finish(const double* r)
@@ -269,15 +289,19 @@ struct OpLoop2 {
         yp = kp >= 0 ? d.Y + (long)kp * d.n : nullptr;
         c = d.coef[ki];
     }
-    __device__ void element(long e, double* a)
+    struct In {
+        double s, z, o;  // o: y_{i-1}, or g at i = 0
+    };
+    __device__ In fetch(long e) const { return In{si[e], d.z[e], kp >= 0 ? yp[e] : d.g[e]}; }
+    __device__ void apply(long e, const In& v, double* a)
     {
-        const double zv = fma(c, si[e], d.z[e]);
+        const double zv = fma(c, v.s, v.z);
         d.z[e] = zv;
         if (kp >= 0) {
-            a[0] = fma(yp[e], zv, a[0]);
+            a[0] = fma(v.o, zv, a[0]);
         } else {
             a[0] = fma(zv, zv, a[0]);
-            a[1] = fma(zv, d.g[e], a[1]);
+            a[1] = fma(zv, v.o, a[1]);
         }
     }
     __device__ void finish(const double* r)
@@ -299,13 +323,16 @@ struct OpDir0 {
     long n;
     double h;
     __device__ void load() { h = *d.h0k; }
-    __device__ void element(long e, double* a)
+    struct In {
+        double g;
+    };
+    __device__ In fetch(long e) const { return In{d.g[e]}; }
+    __device__ void apply(long e, const In& v, double* a)
     {
-        const double gv = d.g[e];
-        const double zv = h * gv;
+        const double zv = h * v.g;
         d.z[e] = zv;
         a[0] = fma(zv, zv, a[0]);
-        a[1] = fma(zv, gv, a[1]);
+        a[1] = fma(zv, v.g, a[1]);
     }
     __device__ void finish(const double* r)
     {
@@ -320,7 +347,11 @@ struct OpZeqG {
     LbDev d;
     long n;
     __device__ void load() {}
-    __device__ void element(long e, double*) { d.z[e] = 1.0 * d.g[e]; }
+    struct In {
+        double g;
+    };
+    __device__ In fetch(long e) const { return In{d.g[e]}; }
+    __device__ void apply(long e, const In& v, double*) { d.z[e] = 1.0 * v.g; }
     __device__ void finish(const double*) {}
 };
 
@@ -331,7 +362,11 @@ struct OpTrial {
     long n;
     double stp;
     __device__ void load() { stp = d.h->stp; }
-    __device__ void element(long e, double*) { d.x[e] = fma(stp, -d.z[e], d.x0[e]); }
+    struct In {
+        double z, x0;
+    };
+    __device__ In fetch(long e) const { return In{d.z[e], d.x0[e]}; }
+    __device__ void apply(long e, const In& v, double*) { d.x[e] = fma(stp, -v.z, v.x0); }
     __device__ void finish(const double*) {}
 };
 
@@ -341,11 +376,14 @@ struct OpTrialDots {
     LbDev d;
     long n;
     __device__ void load() {}
-    __device__ void element(long e, double* a)
+    struct In {
+        double g, z;
+    };
+    __device__ In fetch(long e) const { return In{d.g[e], d.z[e]}; }
+    __device__ void apply(long, const In& v, double* a)
     {
-        const double gv = d.g[e];
-        a[0] = fma(gv, d.z[e], a[0]);
-        a[1] = fma(gv, gv, a[1]);
+        a[0] = fma(v.g, v.z, a[0]);
+        a[1] = fma(v.g, v.g, a[1]);
     }
     __device__ void finish(const double* r)
     {
@@ -553,21 +591,22 @@ int cmbl_lbfgs_create(cmbl_lbfgs** out, cmbl_map* map, const double* d_x0, int m
         (rc = lb->mem.dalloc(&d.slot, m)) || (rc = lb->mem.dalloc(&d.rho, m)) ||
         (rc = lb->mem.dalloc(&d.alpha, m)) || (rc = lb->mem.dalloc(&d.coef, m)) ||
         (rc = lb->mem.dalloc(&d.h0k, 1)) || (rc = lb->mem.dalloc(&lb->d_f, 1)) ||
-        (rc = lb->mem.dalloc(&lb->ps.part, (size_t)kMaxPassDots * kDotParts)) ||
-        (rc = lb->mem.dalloc(&lb->ps.count, 1)))
+        (rc = lb->mem.dalloc(&lb->ps.part, (size_t)kMaxPassDots * kDotParts)))
         return bail(rc);
     if ((rc = lb->mem.halloc(&lb->h, &d.h, 1))) return bail(rc);
     d.f = lb->d_f;
     std::vector<int> slots(m);
     for (int i = 0; i < m; ++i) slots[i] = i;
     std::vector<double> zeros(m, 0.0);
+    // the caller's x0 may still be in flight on its streams; the copy goes on
+    // this context's stream (a device-to-device hipMemcpy does not wait for
+    // completion, and this stream does not order with the null stream)
     if (hipDeviceSynchronize() != hipSuccess ||
         hipMemcpy(d.slot, slots.data(), sizeof(int) * m, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d.rho, zeros.data(), sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d.alpha, zeros.data(), sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d.coef, zeros.data(), sizeof(double) * m, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(lb->ps.count, 0, sizeof(unsigned)) != hipSuccess ||
-        hipMemcpy(d.x, d_x0, sizeof(double) * n, hipMemcpyDeviceToDevice) != hipSuccess)
+        hipMemcpyAsync(d.x, d_x0, sizeof(double) * n, hipMemcpyDeviceToDevice, lb->stream) != hipSuccess)
         return bail(fail(CMBL_ERR_HIP, "L-BFGS set-up copies failed"));
     if ((rc = lb_start(lb))) return bail(rc);
     *out = lb;
@@ -715,7 +754,8 @@ struct HmcHost {
 
 struct HmcDev {
     long n;
-    double *x, *p, *d, *prev, *dprev, *mass, *msq, *noise;
+    double *x, *p, *d, *prev, *dprev, *mass, *msq;
+    double* noise[2];  // momentum draws of even / odd trajectories
     const double* f;
     HmcHost* h;
 };
@@ -726,14 +766,19 @@ struct OpHmcInit {
     static constexpr int kDots = 1;
     HmcDev d;
     long n;
+    const double* noise;
     __device__ void load() {}
-    __device__ void element(long e, double* a)
+    struct In {
+        double noise, msq, x, d, mass;
+    };
+    __device__ In fetch(long e) const { return In{noise[e], d.msq[e], d.x[e], d.d[e], d.mass[e]}; }
+    __device__ void apply(long e, const In& v, double* a)
     {
-        const double pv = d.noise[e] * d.msq[e];
+        const double pv = v.noise * v.msq;
         d.p[e] = pv;
-        d.prev[e] = 1.0 * d.x[e];
-        d.dprev[e] = 1.0 * d.d[e];
-        const double t = (1.0 * pv) / d.mass[e];
+        d.prev[e] = 1.0 * v.x;
+        d.dprev[e] = 1.0 * v.d;
+        const double t = (1.0 * pv) / v.mass;
         a[0] = fma(t, pv, a[0]);
     }
     __device__ void finish(const double* r) { d.h->oldP = r[0]; }
@@ -752,15 +797,18 @@ struct OpHmcKickDrift {
         c = d.h->c;
         tau = d.h->tau;
     }
-    __device__ void element(long e, double*)
+    struct In {
+        double d, p, mass, x;
+    };
+    __device__ In fetch(long e) const { return In{d.d[e], d.p[e], d.mass[e], d.x[e]}; }
+    __device__ void apply(long e, const In& v, double*)
     {
-        const double dv = d.d[e];
-        double pv = d.p[e];
-        if (twice) pv = fma(c, dv, pv);
-        pv = fma(c, dv, pv);
+        double pv = v.p;
+        if (twice) pv = fma(c, v.d, pv);
+        pv = fma(c, v.d, pv);
         d.p[e] = pv;
-        const double t = (1.0 * pv) / d.mass[e];
-        d.x[e] = fma(tau, t, d.x[e]);
+        const double t = (1.0 * pv) / v.mass;
+        d.x[e] = fma(tau, t, v.x);
     }
     __device__ void finish(const double*) {}
 };
@@ -772,11 +820,15 @@ struct OpHmcFinal {
     long n;
     double c;
     __device__ void load() { c = d.h->c; }
-    __device__ void element(long e, double* a)
+    struct In {
+        double d, p, mass;
+    };
+    __device__ In fetch(long e) const { return In{d.d[e], d.p[e], d.mass[e]}; }
+    __device__ void apply(long e, const In& v, double* a)
     {
-        const double pv = fma(c, d.d[e], d.p[e]);
+        const double pv = fma(c, v.d, v.p);
         d.p[e] = pv;
-        const double t = (1.0 * pv) / d.mass[e];
+        const double t = (1.0 * pv) / v.mass;
         a[0] = fma(t, pv, a[0]);
     }
     __device__ void finish(const double* r)
@@ -792,11 +844,11 @@ struct OpHmcStart {
     HmcDev d;
     long n;
     __device__ void load() {}
-    __device__ void element(long e, double* a)
-    {
-        const double dv = d.d[e];
-        a[0] = fma(dv, dv, a[0]);
-    }
+    struct In {
+        double d;
+    };
+    __device__ In fetch(long e) const { return In{d.d[e]}; }
+    __device__ void apply(long, const In& v, double* a) { a[0] = fma(v.d, v.d, a[0]); }
     __device__ void finish(const double*) { d.h->f = *d.f; }
 };
 
@@ -806,10 +858,14 @@ struct OpHmcReject {
     HmcDev d;
     long n;
     __device__ void load() {}
-    __device__ void element(long e, double*)
+    struct In {
+        double prev, dprev;
+    };
+    __device__ In fetch(long e) const { return In{d.prev[e], d.dprev[e]}; }
+    __device__ void apply(long e, const In& v, double*)
     {
-        d.x[e] = 1.0 * d.prev[e];
-        d.d[e] = 1.0 * d.dprev[e];
+        d.x[e] = 1.0 * v.prev;
+        d.d[e] = 1.0 * v.dprev;
     }
     __device__ void finish(const double*) {}
 };
@@ -836,15 +892,25 @@ struct cmbl_hmc {
     double like = 0;
     bool started = false;
     long trajectories = 0, accepted = 0, evaluations = 0, gradCalls = 0;
-    // momentum prefetch: pinned host buffers filled by worker threads
-    static constexpr int kSlots = 8;
+    // momentum draws: pinned host buffers filled by worker threads kSlots
+    // trajectories ahead, uploaded on a copy stream into the device buffer of
+    // the trajectory's parity while the previous trajectory runs
+    static constexpr int kSlots = 12;
     std::vector<double*> slot;
-    std::vector<hipEvent_t> slotEv;
+    std::vector<hipEvent_t> slotEv;  // upload from the slot complete
+    hipStream_t copy = nullptr;
+    hipEvent_t upDone[2] = {nullptr, nullptr}, used[2] = {nullptr, nullptr};
     ~cmbl_hmc()
     {
         cache.clear();
         if (stream) (void)hipStreamSynchronize(stream);
+        if (copy) (void)hipStreamSynchronize(copy);
         for (hipEvent_t e : slotEv) (void)hipEventDestroy(e);
+        for (int i = 0; i < 2; ++i) {
+            if (upDone[i]) (void)hipEventDestroy(upDone[i]);
+            if (used[i]) (void)hipEventDestroy(used[i]);
+        }
+        if (copy) (void)hipStreamDestroy(copy);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -854,10 +920,10 @@ namespace cmbl {
 static int hmc_eval(cmbl_hmc* hm, hipStream_t s) { return eval_device(hm->map, hm->d.x, hm->d_f, hm->d.d, s, false); }
 
 // One trajectory of nsteps leapfrog steps (momenta already in d.noise).
-static int hmc_trajectory(cmbl_hmc* hm, int nsteps, hipStream_t s)
+static int hmc_trajectory(cmbl_hmc* hm, int nsteps, int parity, hipStream_t s)
 {
     const HmcDev& d = hm->d;
-    CMBL_TRY(run_pass(OpHmcInit{d, d.n}, hm->ps, s));
+    CMBL_TRY(run_pass(OpHmcInit{d, d.n, d.noise[parity]}, hm->ps, s));
     for (int j = 0; j < nsteps; ++j) {
         OpHmcKickDrift kd{};
         kd.d = d;
@@ -896,17 +962,22 @@ int cmbl_hmc_create(cmbl_hmc** out, cmbl_map* map, const double* d_start, const 
     hm->graphs = (flags & CMBL_LOOP_EAGER) == 0;
     hm->ugen.seed(seed);
     hm->gaussSeed = seed + 1;
-    if (hipStreamCreateWithFlags(&hm->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&hm->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&hm->copy, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(CMBL_ERR_HIP, "hipStreamCreate failed"));
+    for (int i = 0; i < 2; ++i)
+        if (hipEventCreateWithFlags(&hm->upDone[i], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&hm->used[i], hipEventDisableTiming) != hipSuccess)
+            return bail(fail(CMBL_ERR_HIP, "hipEventCreate failed"));
     HmcDev& d = hm->d;
     d.n = hm->n;
     const size_t n = (size_t)hm->n;
     int rc;
     if ((rc = hm->mem.dalloc(&d.x, n)) || (rc = hm->mem.dalloc(&d.p, n)) || (rc = hm->mem.dalloc(&d.d, n)) ||
         (rc = hm->mem.dalloc(&d.prev, n)) || (rc = hm->mem.dalloc(&d.dprev, n)) ||
-        (rc = hm->mem.dalloc(&d.mass, n)) || (rc = hm->mem.dalloc(&d.msq, n)) || (rc = hm->mem.dalloc(&d.noise, n)) ||
-        (rc = hm->mem.dalloc(&hm->d_f, 1)) || (rc = hm->mem.dalloc(&hm->ps.part, (size_t)kMaxPassDots * kDotParts)) ||
-        (rc = hm->mem.dalloc(&hm->ps.count, 1)))
+        (rc = hm->mem.dalloc(&d.mass, n)) || (rc = hm->mem.dalloc(&d.msq, n)) ||
+        (rc = hm->mem.dalloc(&d.noise[0], n)) || (rc = hm->mem.dalloc(&d.noise[1], n)) ||
+        (rc = hm->mem.dalloc(&hm->d_f, 1)) || (rc = hm->mem.dalloc(&hm->ps.part, (size_t)kMaxPassDots * kDotParts)))
         return bail(rc);
     if ((rc = hm->mem.halloc(&hm->h, &d.h, 1))) return bail(rc);
     d.f = hm->d_f;
@@ -921,10 +992,13 @@ int cmbl_hmc_create(cmbl_hmc** out, cmbl_map* map, const double* d_start, const 
         hm->slotEv.push_back(e);
     }
     // x = start, M = masses, sqrt(M) as the reference forms it (pow(M, 0.5))
-    if (hipDeviceSynchronize() != hipSuccess || hipMemset(hm->ps.count, 0, sizeof(unsigned)) != hipSuccess ||
-        hipMemcpy(d.x, d_start, sizeof(double) * n, hipMemcpyDeviceToDevice) != hipSuccess ||
-        hipMemcpy(d.mass, d_masses, sizeof(double) * n, hipMemcpyDeviceToDevice) != hipSuccess ||
-        hipMemcpy(d.msq, d_masses, sizeof(double) * n, hipMemcpyDeviceToDevice) != hipSuccess)
+    // copies on this context's stream, ordered before the square root (a
+    // device-to-device hipMemcpy does not wait for completion, and this
+    // stream does not order with the null stream)
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpyAsync(d.x, d_start, sizeof(double) * n, hipMemcpyDeviceToDevice, hm->stream) != hipSuccess ||
+        hipMemcpyAsync(d.mass, d_masses, sizeof(double) * n, hipMemcpyDeviceToDevice, hm->stream) != hipSuccess ||
+        hipMemcpyAsync(d.msq, d_masses, sizeof(double) * n, hipMemcpyDeviceToDevice, hm->stream) != hipSuccess)
         return bail(fail(CMBL_ERR_HIP, "HMC set-up copies failed"));
     if ((rc = launch_vec(kVecPow, d.msq, nullptr, 0.5, hm->n, hm->stream))) return bail(rc);
     if (hipStreamSynchronize(hm->stream) != hipSuccess) return bail(fail(CMBL_ERR_HIP, "HMC set-up failed"));
@@ -938,10 +1012,11 @@ int cmbl_hmc_run(cmbl_hmc* hm, int max_iters, cmbl_hmc_callback cb, void* user)
 {
     if (!hm) return fail(CMBL_ERR_ARG, "null context");
     if (max_iters < 1) return fail(CMBL_ERR_ARG, "max_iters must be >= 1");
-    hipStream_t s = hm->stream;
+    hipStream_t s = hm->stream, cs = hm->copy;
     HmcHost* h = hm->h;
     const HmcDev& d = hm->d;
     const long n = hm->n;
+    const size_t bytes = sizeof(double) * (size_t)n;
     // f_->set(x); currentLike = f_->value() (include/hmc_general.hpp:139-144)
     CMBL_TRY(hmc_eval(hm, s));
     CMBL_TRY(run_pass(OpHmcStart{d, d.n}, hm->ps, s));
@@ -951,9 +1026,10 @@ int cmbl_hmc_run(cmbl_hmc* hm, int max_iters, cmbl_hmc_callback cb, void* user)
     if (!std::isfinite(hm->like)) return fail(CMBL_ERR_NONFINITE, "-2lnL at the starting point is not finite");
     if (cb) cb(user, d.x, hm->like);
 
-    // momenta of trajectory k: whitenoise(gaussSeed + k) drawn on worker
-    // threads up to kSlots trajectories ahead into pinned buffers; a slot is
-    // refilled once its previous upload has completed
+    // momenta of trajectory k: whitenoise(gaussSeed + k), drawn on worker
+    // threads up to kSlots trajectories ahead into pinned slots (a slot is
+    // refilled once its upload has completed), uploaded on the copy stream
+    // into noise[k % 2] once trajectory k - 2 has finished with it
     const int W = cmbl_hmc::kSlots;
     const int seed0 = hm->gaussSeed;
     std::vector<std::future<int>> fut(W);
@@ -962,10 +1038,21 @@ int cmbl_hmc_run(cmbl_hmc* hm, int max_iters, cmbl_hmc_callback cb, void* user)
         if (hipEventSynchronize(hm->slotEv[sl]) != hipSuccess) return CMBL_ERR_HIP;
         return cmbl_whitenoise(seed0 + k, 1.0, n, hm->slot[sl]);
     };
+    auto upload = [&](int k) -> int {
+        const int sl = k % W, par = k & 1;
+        const int rc = fut[sl].get();
+        if (rc != CMBL_OK) return fail(rc, "momentum draw failed");
+        CMBL_HIP(hipStreamWaitEvent(cs, hm->used[par], 0));
+        CMBL_HIP(hipMemcpyAsync(d.noise[par], hm->slot[sl], bytes, hipMemcpyHostToDevice, cs));
+        CMBL_HIP(hipEventRecord(hm->upDone[par], cs));
+        CMBL_HIP(hipEventRecord(hm->slotEv[sl], cs));
+        if (k + W < max_iters) fut[sl] = std::async(std::launch::async, produce, k + W);
+        return CMBL_OK;
+    };
     for (int k = 0; k < std::min(W, max_iters); ++k) fut[k] = std::async(std::launch::async, produce, k);
-    int rc = CMBL_OK;
-    for (int iter = 0; iter < max_iters; ++iter) {
-        double tau, c;
+    int rc = upload(0);
+    for (int iter = 0; iter < max_iters && rc == CMBL_OK; ++iter) {
+        double tau;
         int steps;
         if (hm->flags & CMBL_HMC_FIXED) {
             tau = hm->tauMax;
@@ -979,24 +1066,25 @@ int cmbl_hmc_run(cmbl_hmc* hm, int max_iters, cmbl_hmc_callback cb, void* user)
             rc = fail(CMBL_ERR_STATE, "step size or step count out of range");
             break;
         }
-        c = -tau / 2 / 2;
         ++hm->gradCalls;  // f_->derivative(d_): the gradient at x is already held
-        const int sl = iter % W;
-        if ((rc = fut[sl].get()) != CMBL_OK) {
-            rc = fail(rc, "momentum draw failed");
-            break;
-        }
         ++hm->gaussSeed;
+        const int par = iter & 1;
         h->tau = tau;
-        h->c = c;
-        if (hipMemcpyAsync(d.noise, hm->slot[sl], sizeof(double) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
-            hipEventRecord(hm->slotEv[sl], s) != hipSuccess) {
-            rc = fail(CMBL_ERR_HIP, "momentum upload failed");
+        h->c = -tau / 2 / 2;
+        if (hipStreamWaitEvent(s, hm->upDone[par], 0) != hipSuccess) {
+            rc = fail(CMBL_ERR_HIP, "waiting for the momentum upload failed");
             break;
         }
-        if ((rc = hm->cache.run(hm->graphs, steps, s, [&](hipStream_t st) { return hmc_trajectory(hm, steps, st); })))
+        const long key = (long)steps * 2 + par;
+        if ((rc = hm->cache.run(hm->graphs, key, s,
+                                [&](hipStream_t st) { return hmc_trajectory(hm, steps, par, st); })))
             break;
-        if (iter + W < max_iters) fut[sl] = std::async(std::launch::async, produce, iter + W);
+        if (hipEventRecord(hm->used[par], s) != hipSuccess) {
+            rc = fail(CMBL_ERR_HIP, "event record failed");
+            break;
+        }
+        // next trajectory's momenta go up while this one runs
+        if (iter + 1 < max_iters && (rc = upload(iter + 1)) != CMBL_OK) break;
         if (hipStreamSynchronize(s) != hipSuccess) {
             rc = fail(CMBL_ERR_HIP, "trajectory failed");
             break;
@@ -1022,9 +1110,10 @@ int cmbl_hmc_run(cmbl_hmc* hm, int max_iters, cmbl_hmc_callback cb, void* user)
         }
         if (cb) cb(user, d.x, hm->like);
     }
-    // let outstanding draws finish before the buffers can go
+    // outstanding draws and uploads finish before the buffers can go
     for (auto& f : fut)
         if (f.valid()) (void)f.get();
+    if (hipStreamSynchronize(cs) != hipSuccess && rc == CMBL_OK) rc = fail(CMBL_ERR_HIP, "momentum uploads failed");
     return rc;
 }
 
@@ -1035,6 +1124,15 @@ int cmbl_hmc_state(const cmbl_hmc* hm, double* like, long* trajectories, long* a
     if (trajectories) *trajectories = hm->trajectories;
     if (accepted) *accepted = hm->accepted;
     if (evaluations) *evaluations = hm->evaluations;
+    return CMBL_OK;
+}
+
+int cmbl_hmc_last_trajectory(const cmbl_hmc* hm, double* like_end, double* kinetic_start, double* kinetic_end)
+{
+    if (!hm) return fail(CMBL_ERR_ARG, "null context");
+    if (like_end) *like_end = hm->h->f;
+    if (kinetic_start) *kinetic_start = hm->h->oldP;
+    if (kinetic_end) *kinetic_end = hm->h->newP;
     return CMBL_OK;
 }
 

@@ -218,6 +218,10 @@ int cmbl_hmc_run(cmbl_hmc* hm, int max_iters, cmbl_hmc_callback cb, void* user);
 /* Current -2lnL and totals: trajectories, accepted, likelihood evaluations. */
 int cmbl_hmc_state(const cmbl_hmc* hm, double* like, long* trajectories, long* accepted, long* evaluations);
 int cmbl_hmc_position(cmbl_hmc* hm, double* d_out);
+/* The last trajectory's -2lnL at its end point and its kinetic terms
+ * p . M^-1 p at start and end (the Metropolis test compares
+ * like_end + kinetic_end with the current -2lnL + kinetic_start). */
+int cmbl_hmc_last_trajectory(const cmbl_hmc* hm, double* like_end, double* kinetic_start, double* kinetic_end);
 
 /* The More-Thuente line search of the device-resident L-BFGS as a state
  * machine (moreThuenteSearch, include/line_search.hpp:160-272, with its

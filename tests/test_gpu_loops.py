@@ -147,3 +147,32 @@ def test_hmc_fixed_steps_and_graph_replay():
     assert chains[0][0] == chains[1][0] and np.array_equal(chains[0][1], chains[1][1])
     assert len(chains[0][0]) == 13 and all(math.isfinite(v) for v in chains[0][0])
     mp.close()
+
+
+def test_loop_set_up_copies_are_ordered():
+    """The set-up copies of the start point (and of the masses) complete before
+    the context's first kernels, at a size where an unordered device-to-device
+    copy is still in flight: the L-BFGS starting value equals a direct
+    evaluation at x0, and each HMC trajectory's initial kinetic term
+    p . M^-1 p = sum noise^2 lies within 5 sigma of n (noise ~ N(0, 1))."""
+    import torch
+
+    print("case loop set-up ordering nside=256 lmax=512", flush=True)
+    c = Case(256, 512, 40.0 / 60.0, 40.0, 20.0, seed=31)
+    mp = c.make_map()
+    n = c.x.size
+    f_direct, _ = mp.value_and_grad(c.x)
+    x0 = torch.from_numpy(c.x).to("cuda:0")
+    lb = cm.LBFGS(mp, x0.data_ptr(), m=10)
+    assert lb.state()["f"] == f_direct
+    lb.close()
+    l = np.floor(np.sqrt(np.arange(n))).astype(int)
+    masses = torch.from_numpy(np.where(l >= 2, 2.0 / np.maximum(c.cl[l], 1e-300), 1.0) * 1e3).to("cuda:0")
+    h = cm.HMC(mp, x0.data_ptr(), masses.data_ptr(), 1e-3, 2, 57, fixed=True)
+    k0 = []
+    h.run(3, callback=lambda x, like: k0.append(h.last_trajectory()[1]))
+    print("kinetic terms at trajectory starts: %s (n = %d)" % (k0[1:], n), flush=True)
+    for v in k0[1:]:
+        assert abs(v - n) < 5 * math.sqrt(2 * n)
+    h.close()
+    mp.close()

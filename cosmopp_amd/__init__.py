@@ -48,6 +48,12 @@ _SIGS = {
     "cmbl_version": ([], ctypes.c_char_p),
     "cmbl_map_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "cmbl_map_destroy": ([_vp], None),
+    "cmbl_map_create_tuned": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "cmbl_map_get_tuning": ([_vp, _vp], ctypes.c_int),
+    "cmbl_tune": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _dp], ctypes.c_int),
+    "cmbl_tuning_set": ([ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "cmbl_tuning_load": ([ctypes.c_char_p], ctypes.c_int),
+    "cmbl_tuning_save": ([ctypes.c_char_p], ctypes.c_int),
     "cmbl_map_nside": ([_vp], ctypes.c_int),
     "cmbl_map_lmax": ([_vp], ctypes.c_int),
     "cmbl_map_npix": ([_vp], ctypes.c_long),
@@ -301,12 +307,43 @@ def reorder(m, nside, to_ring):
     return out
 
 
-class Map:
-    """One HEALPix map with its likelihood tables on the current GPU (cmbl_map)."""
+class Tuning(ctypes.Structure):
+    """Launch shapes of a map (cmbl_tuning): ring pairs per lane of the
+    Legendre synthesis and adjoint, Bluestein split length."""
+    _fields_ = [("k_synth", ctypes.c_int), ("k_adj", ctypes.c_int), ("split_m", ctypes.c_int)]
 
-    def __init__(self, nside, lmax):
+    def as_tuple(self):
+        return (self.k_synth, self.k_adj, self.split_m)
+
+
+def tune(nside, lmax, reps=10):
+    """Autotuner (cmbl_tune): returns (Tuning, stage times in ms:
+    synthesis at k = 1, 2, 4, 8; adjoint at k = 1, 2, 4, 8; ring at split 2048, 4096, 8192)."""
+    t = Tuning()
+    times = (ctypes.c_double * 11)()
+    _check(lib().cmbl_tune(int(nside), int(lmax), int(reps), ctypes.byref(t), times))
+    return t, list(times)
+
+
+def tuning_load(path):
+    _check(lib().cmbl_tuning_load(os.fsencode(path)))
+
+
+def tuning_save(path):
+    _check(lib().cmbl_tuning_save(os.fsencode(path)))
+
+
+class Map:
+    """One HEALPix map with its likelihood tables on the current GPU (cmbl_map).
+    tuning: (k_synth, k_adj, split_m) to override the tuning table."""
+
+    def __init__(self, nside, lmax, tuning=None):
         h = _vp()
-        _check(lib().cmbl_map_create(ctypes.byref(h), nside, lmax))
+        if tuning is None:
+            _check(lib().cmbl_map_create(ctypes.byref(h), nside, lmax))
+        else:
+            t = Tuning(*tuning)
+            _check(lib().cmbl_map_create_tuned(ctypes.byref(h), nside, lmax, ctypes.byref(t)))
         self._h = h
         self.nside = nside
         self.lmax = lmax
@@ -316,6 +353,12 @@ class Map:
     @property
     def handle(self):
         return self._h
+
+    @property
+    def tuning(self):
+        t = Tuning()
+        _check(lib().cmbl_map_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_tuple()
 
     def close(self):
         if self._h:

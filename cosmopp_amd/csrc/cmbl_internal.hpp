@@ -330,6 +330,7 @@ int launch_dot(const double* x, const double* y, long n, double* d_out, double* 
 struct cmbl_map {
     int nside = 0, L = 0, npair = 0, npair_pad = 0, nring = 0;
     int K1 = 1, K5 = 1, nrb1 = 0, nrb5 = 0;
+    cmbl_tuning tuning{1, 1, 4096};
     long npix = 0, nalm = 0, nparam = 0;
     int device = 0;
     int lds_ring = 0;

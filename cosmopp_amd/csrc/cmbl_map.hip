@@ -5,7 +5,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <dlfcn.h>
+#include <fstream>
 #include <mutex>
+#include <sstream>
 #include <random>
 #include <string>
 #include <vector>
@@ -246,7 +249,94 @@ static int choose_k(int npair, int kmax)
     return k;
 }
 
-static int map_create(cmbl_map* mp, int nside, int lmax)
+// ------------------------------------------------------------------ launch-shape tuning
+// Per geometry: ring pairs per lane of the Legendre synthesis (k_synth) and
+// adjoint (k_adj), and the length above which Bluestein rings are split in
+// two half transforms.  Chosen by cmbl_tune (timed candidates) and kept in a
+// table keyed by (device architecture, nside, lmax); cmbl_map_create takes a
+// geometry's entry, else the rule of choose_k.  The table file next to the
+// library, tuning_<arch>.txt, is read at the first map creation (or the file
+// named by CMBL_TUNING_FILE).
+struct TuneEntry {
+    std::string arch;
+    int nside, lmax;
+    cmbl_tuning t;
+};
+static std::mutex g_tune_mu;
+static std::vector<TuneEntry> g_tune;
+static bool g_tune_loaded = false;
+
+static bool tuning_valid(const cmbl_tuning& t)
+{
+    auto k_ok = [](int k) { return k == 1 || k == 2 || k == 4 || k == 8; };
+    return k_ok(t.k_synth) && k_ok(t.k_adj) && (t.split_m == 2048 || t.split_m == 4096 || t.split_m == 8192);
+}
+
+static std::string device_arch()
+{
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return "unknown";
+    std::string a = prop.gcnArchName;
+    return a.substr(0, a.find(':'));
+}
+
+static int tuning_read(const std::string& path, bool must_exist)
+{
+    std::ifstream in(path);
+    if (!in) return must_exist ? fail(CMBL_ERR_ARG, "cannot open tuning file " + path) : CMBL_OK;
+    std::string line;
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream ls(line);
+        TuneEntry e;
+        if (!(ls >> e.arch >> e.nside >> e.lmax >> e.t.k_synth >> e.t.k_adj >> e.t.split_m) || !tuning_valid(e.t))
+            return fail(CMBL_ERR_ARG, "bad line in tuning file " + path + ": " + line);
+        bool replaced = false;
+        for (TuneEntry& o : g_tune)
+            if (o.arch == e.arch && o.nside == e.nside && o.lmax == e.lmax) {
+                o.t = e.t;
+                replaced = true;
+            }
+        if (!replaced) g_tune.push_back(e);
+    }
+    return CMBL_OK;
+}
+
+static void tuning_load_default(const std::string& arch)
+{
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    if (const char* f = std::getenv("CMBL_TUNING_FILE")) {
+        (void)tuning_read(f, false);
+        return;
+    }
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&tuning_load_default), &info) && info.dli_fname) {
+        std::string dir = info.dli_fname;
+        const size_t slash = dir.rfind('/');
+        dir = slash == std::string::npos ? "." : dir.substr(0, slash);
+        (void)tuning_read(dir + "/tuning_" + arch + ".txt", false);
+    }
+}
+
+// The tuning a new map of this geometry gets: the table's entry, else the rule.
+static cmbl_tuning default_tuning(int nside, int lmax)
+{
+    const std::string arch = device_arch();
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        tuning_load_default(arch);
+        for (const TuneEntry& e : g_tune)
+            if (e.arch == arch && e.nside == nside && e.lmax == lmax) return e.t;
+    }
+    // synthesis: no cross-lane reduction, so small ring blocks (less mixed
+    // live/negligible work per wave); adjoint: four rings per lane amortise
+    // the per-l lane reduction
+    return cmbl_tuning{choose_k(2 * nside, 2), choose_k(2 * nside, 4), 4096};
+}
+
+static int map_create(cmbl_map* mp, int nside, int lmax, const cmbl_tuning& tune)
 {
     const int N = nside;
     const int L = lmax;
@@ -257,20 +347,10 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     mp->npair = 2 * N;
     mp->nalm = (long)(L + 1) * (L + 2) / 2;
     mp->nparam = (long)(L + 1) * (L + 1);
+    mp->tuning = tune;
     CMBL_HIP(hipGetDevice(&mp->device));
-    // synthesis: no cross-lane reduction, so small ring blocks (less mixed
-    // live/negligible work per wave); adjoint: four rings per lane amortise
-    // the per-l lane reduction
-    mp->K1 = choose_k(mp->npair, 2);
-    mp->K5 = choose_k(mp->npair, 4);
-    // tuning override: CMBL_K1 / CMBL_K5 in {1, 2, 4, 8}
-    for (int which = 0; which < 2; ++which) {
-        const char* e = std::getenv(which ? "CMBL_K5" : "CMBL_K1");
-        if (e) {
-            const int v = std::atoi(e);
-            if (v == 1 || v == 2 || v == 4 || v == 8) (which ? mp->K5 : mp->K1) = v;
-        }
-    }
+    mp->K1 = tune.k_synth;
+    mp->K5 = tune.k_adj;
     const int kmax = std::max(mp->K1, mp->K5);
     mp->npair_pad = ((mp->npair + 64 * kmax - 1) / (64 * kmax)) * (64 * kmax);
     mp->nrb1 = mp->npair_pad / (64 * mp->K1);
@@ -278,14 +358,9 @@ static int map_create(cmbl_map* mp, int nside, int lmax)
     const int npad = mp->npair_pad;
 
     // Bluestein rings longer than split_limit are split into two half-length
-    // transforms: above 4096 a ring's LDS (139 KB at 8192) would allow one
-    // workgroup per CU, while two halves of 4096 run two per CU (tuning
-    // override CMBL_SPLIT_M in {2048, 4096, 8192})
-    int split_limit = 4096;
-    if (const char* e = std::getenv("CMBL_SPLIT_M")) {
-        const int v = std::atoi(e);
-        if (v == 2048 || v == 4096 || v == 8192) split_limit = v;
-    }
+    // transforms: above 4096 a ring's LDS (139 KB at 8192) allows one
+    // workgroup per CU, while two halves of 4096 run two per CU
+    const int split_limit = tune.split_m;
 
     // ---- ring-pair tables
     std::vector<double> rx(npad, 0.0), rs(npad, 0.0), rs2(npad, 1.0);
@@ -638,7 +713,7 @@ extern "C" {
 const char* cmbl_last_error(void) { return g_err.c_str(); }
 const char* cmbl_version(void) { return "cosmopp_amd 0.1 (gfx950)"; }
 
-int cmbl_map_create(cmbl_map** out, int nside, int lmax)
+static int map_create_checked(cmbl_map** out, int nside, int lmax, const cmbl_tuning* given)
 {
     if (!out) return fail(CMBL_ERR_ARG, "out is null");
     *out = nullptr;
@@ -646,8 +721,23 @@ int cmbl_map_create(cmbl_map** out, int nside, int lmax)
     if (lmax < 2 || lmax > 4 * nside) return fail(CMBL_ERR_ARG, "lmax must be in [2, 4*nside]");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CMBL_ERR_HIP, "no HIP device available");
+    cmbl_tuning t;
+    if (given) {
+        if (!tuning_valid(*given))
+            return fail(CMBL_ERR_ARG, "tuning: ring pairs per lane must be 1, 2, 4 or 8 and the split length "
+                                      "2048, 4096 or 8192");
+        t = *given;
+    } else {
+        t = default_tuning(nside, lmax);
+        // test overrides: CMBL_K1 / CMBL_K5 in {1, 2, 4, 8}, CMBL_SPLIT_M in {2048, 4096, 8192}
+        cmbl_tuning o = t;
+        if (const char* e = std::getenv("CMBL_K1")) o.k_synth = std::atoi(e);
+        if (const char* e = std::getenv("CMBL_K5")) o.k_adj = std::atoi(e);
+        if (const char* e = std::getenv("CMBL_SPLIT_M")) o.split_m = std::atoi(e);
+        if (tuning_valid(o)) t = o;
+    }
     cmbl_map* mp = new cmbl_map();
-    const int rc = map_create(mp, nside, lmax);
+    const int rc = map_create(mp, nside, lmax, t);
     if (rc != CMBL_OK) {
         const std::string msg = g_err;
         map_free(mp);
@@ -656,6 +746,143 @@ int cmbl_map_create(cmbl_map** out, int nside, int lmax)
         return rc;
     }
     *out = mp;
+    return CMBL_OK;
+}
+
+int cmbl_map_create(cmbl_map** out, int nside, int lmax) { return map_create_checked(out, nside, lmax, nullptr); }
+
+int cmbl_map_create_tuned(cmbl_map** out, int nside, int lmax, const cmbl_tuning* tuning)
+{
+    if (!tuning) return fail(CMBL_ERR_ARG, "tuning is null");
+    return map_create_checked(out, nside, lmax, tuning);
+}
+
+int cmbl_map_get_tuning(const cmbl_map* mp, cmbl_tuning* t)
+{
+    if (!mp || !t) return fail(CMBL_ERR_ARG, "null pointer");
+    *t = mp->tuning;
+    return CMBL_OK;
+}
+
+int cmbl_tuning_load(const char* path)
+{
+    if (!path) return fail(CMBL_ERR_ARG, "path is null");
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    return tuning_read(path, true);
+}
+
+int cmbl_tuning_save(const char* path)
+{
+    if (!path) return fail(CMBL_ERR_ARG, "path is null");
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    std::ofstream out(path);
+    if (!out) return fail(CMBL_ERR_ARG, std::string("cannot write ") + path);
+    out << "# arch nside lmax k_synth k_adj split_m  (cmbl_tune: fastest of timed candidates)\n";
+    for (const TuneEntry& e : g_tune)
+        out << e.arch << " " << e.nside << " " << e.lmax << " " << e.t.k_synth << " " << e.t.k_adj << " "
+            << e.t.split_m << "\n";
+    return out ? CMBL_OK : fail(CMBL_ERR_ARG, std::string("write failed: ") + path);
+}
+
+// Average ms of the synthesis, ring and adjoint stages of one evaluation of
+// a map created with tuning t (the stages' work does not depend on the data
+// or the a_lm values, so zero data, unit weights and x = 1 are used).
+static int time_candidate(int nside, int lmax, const cmbl_tuning& t, int reps, double* synth, double* ring,
+                          double* adj)
+{
+    cmbl_map* mp = nullptr;
+    CMBL_TRY(map_create_checked(&mp, nside, lmax, &t));
+    int rc = CMBL_OK;
+    double* x = nullptr;
+    double ms[CMBL_NSTAGES] = {0, 0, 0, 0, 0, 0};
+    long nev = 0;
+    if (hipMalloc(&x, sizeof(double) * mp->nparam) != hipSuccess) rc = fail(CMBL_ERR_ALLOC, "tuning x");
+    if (rc == CMBL_OK) rc = launch_vec(kVecSet, x, nullptr, 1.0, mp->nparam, nullptr);
+    if (rc == CMBL_OK) rc = launch_vec(kVecSet, mp->d_w, nullptr, 1.0, mp->npix, nullptr);
+    mp->has_data = true;
+    for (int i = 0; i < 2 && rc == CMBL_OK; ++i) rc = eval_device(mp, x, mp->d_f, mp->d_g, nullptr);
+    if (rc == CMBL_OK) rc = cmbl_map_set_timing(mp, 1);
+    for (int i = 0; i < reps && rc == CMBL_OK; ++i) rc = eval_device(mp, x, mp->d_f, mp->d_g, nullptr);
+    if (rc == CMBL_OK) rc = cmbl_map_stage_times(mp, ms, &nev);
+    if (rc == CMBL_OK && nev > 0) {
+        *synth = ms[1] / nev;
+        *ring = ms[2] / nev;
+        *adj = ms[3] / nev;
+    }
+    const std::string msg = g_err;
+    if (x) (void)hipFree(x);
+    cmbl_map_destroy(mp);
+    g_err = msg;
+    return rc;
+}
+
+// The first candidate, in preference order, within 2% of the fastest: timing
+// noise of a percent does not flip the choice between runs.
+static int pick(const std::vector<int>& order, const double* t)
+{
+    double best = 1e300;
+    for (int i : order)
+        if (t[i] >= 0) best = std::min(best, t[i]);
+    for (int i : order)
+        if (t[i] >= 0 && t[i] <= 1.02 * best) return i;
+    return order[0];
+}
+
+int cmbl_tune(int nside, int lmax, int reps, cmbl_tuning* best, double* times)
+{
+    if (!best) return fail(CMBL_ERR_ARG, "best is null");
+    if (reps < 1) return fail(CMBL_ERR_ARG, "reps must be >= 1");
+    if (nside < 1 || !is_pow2(nside) || lmax < 2 || lmax > 4 * nside)
+        return fail(CMBL_ERR_ARG, "nside must be a power of two and lmax in [2, 4*nside]");
+    const cmbl_tuning rule{choose_k(2 * nside, 2), choose_k(2 * nside, 4), 4096};
+    double t[11];
+    for (double& v : t) v = -1.0;
+    // ring pairs per lane: candidates 1, 2, 4, 8 for both Legendre stages
+    const int ks[4] = {1, 2, 4, 8};
+    for (int c = 0; c < 4; ++c) {
+        double ring = 0;
+        CMBL_TRY(time_candidate(nside, lmax, cmbl_tuning{ks[c], ks[c], rule.split_m}, reps, &t[c], &ring, &t[4 + c]));
+        t[9] = ring;
+    }
+    auto order_from = [&](int k) {
+        std::vector<int> o{k == 1 ? 0 : k == 2 ? 1 : k == 4 ? 2 : 3};
+        for (int c = 0; c < 4; ++c)
+            if (c != o[0]) o.push_back(c);
+        return o;
+    };
+    cmbl_tuning choice{ks[pick(order_from(rule.k_synth), t)], 0, rule.split_m};
+    std::vector<int> oa = order_from(rule.k_adj);
+    for (int& i : oa) i += 4;
+    choice.k_adj = ks[pick(oa, t) - 4];
+    // split length: only rings of nph > 1024 (nside >= 512) are Bluestein
+    // transforms longer than 2048
+    if (nside >= 512) {
+        const int splits[3] = {2048, 4096, 8192};
+        for (int c = 0; c < 3; ++c) {
+            double sy = 0, ad = 0;
+            CMBL_TRY(time_candidate(nside, lmax, cmbl_tuning{choice.k_synth, choice.k_adj, splits[c]}, reps, &sy,
+                                    &t[8 + c], &ad));
+        }
+        choice.split_m = splits[pick({9, 8, 10}, t) - 8];
+    }
+    if (times)
+        for (int i = 0; i < 11; ++i) times[i] = t[i];
+    *best = choice;
+    return cmbl_tuning_set(nside, lmax, &choice);
+}
+
+int cmbl_tuning_set(int nside, int lmax, const cmbl_tuning* t)
+{
+    if (!t || !tuning_valid(*t)) return fail(CMBL_ERR_ARG, "invalid tuning");
+    const std::string arch = device_arch();
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    tuning_load_default(arch);
+    for (TuneEntry& e : g_tune)
+        if (e.arch == arch && e.nside == nside && e.lmax == lmax) {
+            e.t = *t;
+            return CMBL_OK;
+        }
+    g_tune.push_back(TuneEntry{arch, nside, lmax, *t});
     return CMBL_OK;
 }
 

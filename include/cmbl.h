@@ -52,6 +52,38 @@ const char* cmbl_version(void);
  * device.  1 <= nside (power of two), 0 <= lmax <= 4*nside, lmax >= 2. */
 int cmbl_map_create(cmbl_map** out, int nside, int lmax);
 void cmbl_map_destroy(cmbl_map* map);
+
+/* Launch shapes of a map's kernels, chosen per geometry by the autotuner
+ * (no counterpart in the reference, whose transforms are HEALPix's):
+ *   k_synth, k_adj  ring pairs per lane of the Legendre synthesis and
+ *                   adjoint (1, 2, 4 or 8; a work item is one degree m of
+ *                   64 k ring pairs);
+ *   split_m         Bluestein ring transforms longer than this (2048, 4096
+ *                   or 8192) run as two half-length transforms.
+ * cmbl_map_create takes the geometry's entry of the tuning table (the file
+ * tuning_<arch>.txt next to the library, or CMBL_TUNING_FILE, read at the
+ * first map creation, plus cmbl_tune / cmbl_tuning_set / cmbl_tuning_load
+ * results), else a fixed rule.  Results do not depend on the tuning beyond
+ * rounding (the parity bar holds for every choice). */
+typedef struct cmbl_tuning {
+    int k_synth;
+    int k_adj;
+    int split_m;
+} cmbl_tuning;
+int cmbl_map_create_tuned(cmbl_map** out, int nside, int lmax, const cmbl_tuning* tuning);
+int cmbl_map_get_tuning(const cmbl_map* map, cmbl_tuning* tuning);
+/* Autotuner: times reps evaluations (after two untimed ones) of candidate
+ * maps of this geometry on the current device -- k_synth = k_adj = 1, 2, 4, 8
+ * (each stage timed separately), then, from nside 512, split_m = 2048, 4096,
+ * 8192 -- and picks per parameter the first candidate within 2% of the
+ * fastest (the fixed rule's choice first, so timing noise does not flip the
+ * choice).  The choice enters the tuning table.  times (may be NULL) receives
+ * 11 average stage times in ms: synthesis at k = 1, 2, 4, 8; adjoint at
+ * k = 1, 2, 4, 8; ring stage at split_m = 2048, 4096, 8192 (-1 if not timed). */
+int cmbl_tune(int nside, int lmax, int reps, cmbl_tuning* best, double* times);
+int cmbl_tuning_set(int nside, int lmax, const cmbl_tuning* tuning);
+int cmbl_tuning_load(const char* path);
+int cmbl_tuning_save(const char* path);
 int cmbl_map_nside(const cmbl_map* map);
 int cmbl_map_lmax(const cmbl_map* map);
 long cmbl_map_npix(const cmbl_map* map);

@@ -14,6 +14,7 @@ Reference interface mirrored (aslanyan/cosmopp 3.1):
   mask_regions                         Utils::maskRegions                    source/utils.cpp:235-253
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -111,6 +112,15 @@ _SIGS = {
     "cmbl_gibbs_solve": ([_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "cmbl_gibbs_solve_device": ([_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp],
                                 ctypes.c_int),
+    "cmbl_gibbs_set_cl": ([_vp, _vp], ctypes.c_int),
+    "cmbl_gibbs_sampler_create": ([ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.c_double,
+                                   ctypes.c_double, _vp, _vp, ctypes.c_long], ctypes.c_int),
+    "cmbl_gibbs_sampler_destroy": ([_vp], None),
+    "cmbl_gibbs_sampler_step": ([_vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "cmbl_gibbs_sampler_cl": ([_vp, _vp], ctypes.c_int),
+    "cmbl_gibbs_sampler_sigma_l": ([_vp, _vp], ctypes.c_int),
+    "cmbl_gibbs_sampler_signal": ([_vp, _vp], ctypes.c_int),
+    "cmbl_gibbs_sampler_dipole": ([_vp, _vp], ctypes.c_int),
     "cmbl_simulate_alm": ([_vp, ctypes.c_int, ctypes.c_long, _vp], ctypes.c_int),
     "cmbl_simulate_white_noise": ([ctypes.c_int, ctypes.c_double, ctypes.c_long, _vp], ctypes.c_int),
     "cmbl_beam": ([ctypes.c_int, ctypes.c_double, _vp], ctypes.c_int),
@@ -605,6 +615,155 @@ class Gibbs:
         it = ctypes.c_int(0)
         _check(lib().cmbl_gibbs_solve(self._h, _p(b), _p(x), float(precision), int(max_iter), ctypes.byref(it)))
         return x, it.value
+
+
+    def set_cl(self, cl):
+        """Replace C_l (D_l = B_l sqrt(C_l) and the preconditioner follow)."""
+        _check(lib().cmbl_gibbs_set_cl(self._h, _p(_arr(cl, self.lmax + 1))))
+
+
+class GibbsSampler:
+    """CMBGibbsSampler (include/cmb_gibbs.hpp:12-100, source/cmb_gibbs.cpp:63-680)
+    with its maps, a_lm and solves on the GPU.
+
+    Arguments in the order of the reference constructor (:63): map, noise_map
+    and mask are RING-ordered arrays (reorder NESTED ones with
+    :func:`reorder`); fwhm in degrees; seed 0 takes the time.  beam, if
+    given, replaces the Gaussian B_l (the reference multiplies it by the
+    HEALPix pixel window, read from HEALPix's data files)."""
+
+    def __init__(self, map, noise_map, mask, pixel_noise, lmax, fwhm, starting_cl, seed=0, beam=None):
+        map = np.ascontiguousarray(map, dtype=np.float64)
+        npix = map.size
+        nside = int(round(math.sqrt(npix / 12)))
+        if 12 * nside * nside != npix:
+            raise CmblError(CMBL_ERR_ARG, "map size %d is not 12 nside^2" % npix)
+        if len(starting_cl) < lmax + 1:
+            raise CmblError(CMBL_ERR_ARG, "starting_cl needs lmax + 1 values")
+        self.nside, self.lmax, self.npix = nside, lmax, npix
+        h = _vp()
+        b = None if beam is None else _arr(beam, lmax + 1)
+        _check(lib().cmbl_gibbs_sampler_create(ctypes.byref(h), nside, lmax, _p(map), _p(_arr(noise_map, npix)),
+                                               _p(_arr(mask, npix)), float(pixel_noise), float(fwhm),
+                                               None if b is None else _p(b),
+                                               _p(_arr(np.asarray(starting_cl)[:lmax + 1], lmax + 1)), int(seed)))
+        self._h = h
+        self.iterations = 0
+
+    def close(self):
+        if self._h:
+            lib().cmbl_gibbs_sampler_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, precision=1e-20, max_iter=100000):
+        """CMBGibbsSampler::step (source/cmb_gibbs.cpp:533-538); the signal
+        CG's iteration count goes to self.iterations."""
+        it = ctypes.c_int(0)
+        _check(lib().cmbl_gibbs_sampler_step(self._h, float(precision), int(max_iter), ctypes.byref(it)))
+        self.iterations = it.value
+
+    def _get(self, fn, n):
+        out = np.zeros(n)
+        _check(fn(self._h, _p(out)))
+        return out
+
+    def get_cl(self):
+        return self._get(lib().cmbl_gibbs_sampler_cl, self.lmax + 1)
+
+    def get_sigma_l(self):
+        return self._get(lib().cmbl_gibbs_sampler_sigma_l, self.lmax + 1)
+
+    def signal(self):
+        """Current s, packed a_lm ((lmax+1)^2)."""
+        return self._get(lib().cmbl_gibbs_sampler_signal, (self.lmax + 1) ** 2)
+
+    def dipole(self):
+        """a00, a1-1, a10, a11 of the last generateW."""
+        return self._get(lib().cmbl_gibbs_sampler_dipole, 4)
+
+    getCl = get_cl
+    getSigmaL = get_sigma_l
+
+    def generate_chain(self, n_samples, burn_in=10):
+        """generateChain (source/cmb_gibbs.cpp:572-596): sigma_l after each
+        post-burn-in step."""
+        if burn_in < 0 or n_samples < 1:
+            raise CmblError(CMBL_ERR_ARG, "invalid burn_in %d / n_samples %d" % (burn_in, n_samples))
+        for _ in range(burn_in):
+            self.step()
+        chain = []
+        for _ in range(n_samples):
+            self.step()
+            chain.append(self.get_sigma_l())
+        return chain
+
+    @staticmethod
+    def calculate_likelihood(cl, sigma_l_or_chain, lmax):
+        """-2lnL of C_l given one sigma_l (source/cmb_gibbs.cpp:551-570), or
+        the Blackwell-Rao average over a chain (:598-624)."""
+        if lmax < 2 or len(cl) < lmax + 1:
+            raise CmblError(CMBL_ERR_ARG, "lmax must be >= 2 and cl must hold lmax + 1 values")
+        s = np.asarray(sigma_l_or_chain, dtype=np.float64)
+        if s.ndim == 1:
+            if s.size < lmax + 1:
+                raise CmblError(CMBL_ERR_ARG, "sigma_l must hold lmax + 1 values")
+            res = 0.0
+            for l in range(2, lmax + 1):
+                if not cl[l] > 0:
+                    raise CmblError(CMBL_ERR_ARG, "C_%d must be > 0" % l)
+                sig = s[l] / float(2 * l + 1)
+                res -= (2 * l - 1) * math.log(sig)
+                res += (2 * l + 1) * math.log(cl[l])
+                res += (2 * l + 1) * sig / cl[l]
+            return res
+        if s.shape[0] == 0:
+            raise CmblError(CMBL_ERR_ARG, "the chain is empty")
+        like = sorted(GibbsSampler.calculate_likelihood(cl, row, lmax) for row in s)
+        acc = 0.0
+        for v in like:
+            if v - like[0] < 20:
+                acc += math.exp(-(v - like[0]) / 2.0)
+        acc /= len(like)
+        return like[0] - 2 * math.log(acc)
+
+    @staticmethod
+    def write_chain(chain, path):
+        """writeChainIntoFile (source/cmb_gibbs.cpp:626-648): int32 size,
+        int32 lmax, then size x (lmax+1) doubles, native byte order."""
+        c = np.asarray(chain, dtype=np.float64)
+        if c.ndim != 2 or c.shape[0] == 0 or c.shape[1] == 0:
+            raise CmblError(CMBL_ERR_ARG, "the chain is empty")
+        with open(path, "wb") as f:
+            f.write(np.array([c.shape[0], c.shape[1] - 1], dtype=np.int32).tobytes())
+            f.write(np.ascontiguousarray(c).tobytes())
+
+    @staticmethod
+    def read_chain(path):
+        """readChainFromFile (source/cmb_gibbs.cpp:650-680)."""
+        try:
+            with open(path, "rb") as f:
+                raw = f.read()
+        except OSError as e:
+            raise CmblError(CMBL_ERR_ARG, "Cannot read the input file %s." % path) from e
+        if len(raw) < 8:
+            raise CmblError(CMBL_ERR_ARG, "The input file %s is too short." % path)
+        size, lmax = (int(v) for v in np.frombuffer(raw[:8], dtype=np.int32))
+        if size <= 0:
+            raise CmblError(CMBL_ERR_ARG, "The input file %s contains an invalid chain. The size of the chain is %d, "
+                            "it needs to be positive." % (path, size))
+        if lmax < 2:
+            raise CmblError(CMBL_ERR_ARG, "The input file %s contains an invalid chain. l_max = %d, it needs to be "
+                            ">= 2." % (path, lmax))
+        body = np.frombuffer(raw[8:], dtype=np.float64)
+        if body.size < size * (lmax + 1):
+            raise CmblError(CMBL_ERR_ARG, "The input file %s holds fewer samples than its header says." % path)
+        return [body[i * (lmax + 1):(i + 1) * (lmax + 1)].copy() for i in range(size)]
 
 
 CMBL_LOOP_EAGER = 1

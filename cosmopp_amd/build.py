@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcosmopp_amd.so")
 TEST_BIN = os.path.join(HERE, "cmbl_cpp_tests")
 
-HIP_SOURCES = ["cmbl_kernels.hip", "cmbl_map.hip", "cmbl_loops.hip"]
+HIP_SOURCES = ["cmbl_kernels.hip", "cmbl_map.hip", "cmbl_loops.hip", "cmbl_sampler.hip"]
 CXX_SOURCES = ["cmbl_io.cpp", "cmbl_master.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

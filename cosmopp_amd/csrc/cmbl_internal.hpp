@@ -382,6 +382,7 @@ struct cmbl_gibbs {
     int L = 0;
     long n = 0, npix = 0, good = 0;
     double sigma = 0.0;
+    std::vector<double> beam;  // B_l
     double* d_dl = nullptr;  // [L+1] preconditioner divisors 1 + C_l B_l^2 / N_l
     double *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_ap = nullptr;  // [n] CG vectors
     double *d_b = nullptr, *d_x = nullptr;                                   // [n] host-call staging

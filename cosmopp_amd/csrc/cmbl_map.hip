@@ -1161,6 +1161,7 @@ int cmbl_gibbs_create(cmbl_gibbs** out, int nside, int lmax, const double* cl, c
     gb->n = gb->mp->nparam;
     gb->npix = gb->mp->npix;
     gb->sigma = pixel_noise;
+    gb->beam.assign(beam, beam + lmax + 1);
     // D_l = B_l sqrt(C_l) (source/cmb_gibbs.cpp:370-375); w = mask / sigma^2 (:381-387)
     std::vector<double> dl(lmax + 1), w(gb->npix), zero(gb->npix, 0.0);
     for (int l = 0; l <= lmax; ++l) dl[l] = beam[l] * std::sqrt(cl[l]);
@@ -1204,6 +1205,23 @@ int cmbl_gibbs_create(cmbl_gibbs** out, int nside, int lmax, const double* cl, c
 static int check_gibbs(const cmbl_gibbs* gb)
 {
     if (!gb || !gb->mp) return fail(CMBL_ERR_ARG, "gibbs context is null");
+    return CMBL_OK;
+}
+
+int cmbl_gibbs_set_cl(cmbl_gibbs* gb, const double* cl)
+{
+    CMBL_TRY(check_gibbs(gb));
+    if (!cl) return fail(CMBL_ERR_ARG, "cl is null");
+    const int L = gb->L;
+    for (int l = 0; l <= L; ++l)
+        if (!(cl[l] >= 0.0) || !std::isfinite(cl[l])) return fail(CMBL_ERR_ARG, "C_l must be finite and >= 0");
+    // D_l = B_l sqrt(C_l) and the preconditioner divisors, as at creation
+    std::vector<double> dl(L + 1);
+    for (int l = 0; l <= L; ++l) dl[l] = gb->beam[l] * std::sqrt(cl[l]);
+    CMBL_TRY(cmbl_map_set_beam(gb->mp, dl.data()));
+    const double nl = gb->sigma * gb->sigma * 4 * 3.141592653589793 / (double)gb->good;
+    for (int l = 0; l <= L; ++l) dl[l] = 1.0 + cl[l] * gb->beam[l] * gb->beam[l] / nl;
+    CMBL_HIP(hipMemcpy(gb->d_dl, dl.data(), sizeof(double) * (L + 1), hipMemcpyHostToDevice));
     return CMBL_OK;
 }
 

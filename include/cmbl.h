@@ -195,6 +195,41 @@ int cmbl_gibbs_solve(cmbl_gibbs* gibbs, const double* b, double* x, double preci
                      int* iterations);
 int cmbl_gibbs_solve_device(cmbl_gibbs* gibbs, const double* d_b, double* d_x, double precision, int max_iter,
                             int* iterations, void* stream);
+/* Replace C_l (lmax+1 values, finite, >= 0): D_l = B_l sqrt(C_l) and the
+ * preconditioner's divisors follow, as CmbGibbsCGTreats reads cl_ by
+ * reference (source/cmb_gibbs.cpp:306, 372, 414) after generateCl changes it. */
+int cmbl_gibbs_set_cl(cmbl_gibbs* gibbs, const double* cl);
+
+/* ---------------------------------------------------------------- Gibbs sampler */
+/* CMBGibbsSampler (source/cmb_gibbs.cpp:63-549) with its maps, a_lm and
+ * solves on the GPU.  map, noise_map and mask are RING-ordered (npix each;
+ * cmbl_reorder converts NESTED maps, as the constructor's swap_scheme does);
+ * the data are map + noise_map; s starts as map2alm(map, weight 1) / B_l.
+ * beam (lmax+1 values, nonzero) may be NULL: then B_l is the Gaussian of
+ * fwhm_deg (the reference multiplies it by the HEALPix pixel window read
+ * from HEALPix's data files, source/utils.cpp:67-130; pass that product as
+ * beam to match).  starting_cl: lmax+1 values >= 0.  seed 0 takes the time
+ * (source/cmb_gibbs.cpp:172-173).  One host generator (mt19937 + standard
+ * normal, GaussianGenerator of include/random.hpp:32-52) supplies every draw
+ * in the reference's order: 4 per pixel (generateW), 2 per pixel
+ * (generateSignal), then 2l - 1 per l = 2..lmax (generateCl). */
+typedef struct cmbl_gibbs_sampler cmbl_gibbs_sampler;
+int cmbl_gibbs_sampler_create(cmbl_gibbs_sampler** out, int nside, int lmax, const double* map,
+                              const double* noise_map, const double* mask, double pixel_noise, double fwhm_deg,
+                              const double* beam, const double* starting_cl, long seed);
+void cmbl_gibbs_sampler_destroy(cmbl_gibbs_sampler* sampler);
+/* CMBGibbsSampler::step (source/cmb_gibbs.cpp:533-538): generateW,
+ * generateSignal, generateCl.  The signal's CG stops at |r|^2 <= precision^2
+ * (the reference passes 1e-20) or after max_iter iterations (the reference
+ * has no bound; CMBL_ERR_NOCONV then, and the step still completes);
+ * *iterations (may be NULL) receives its count. */
+int cmbl_gibbs_sampler_step(cmbl_gibbs_sampler* sampler, double precision, int max_iter, int* iterations);
+int cmbl_gibbs_sampler_cl(const cmbl_gibbs_sampler* sampler, double* cl);           /* getCl, lmax+1 */
+int cmbl_gibbs_sampler_sigma_l(const cmbl_gibbs_sampler* sampler, double* sigma_l); /* getSigmaL, lmax+1 */
+/* The current signal s (packed a_lm, (lmax+1)^2) and the monopole / dipole
+ * amplitudes a00, a1-1, a10, a11 of the last generateW. */
+int cmbl_gibbs_sampler_signal(const cmbl_gibbs_sampler* sampler, double* s_packed);
+int cmbl_gibbs_sampler_dipole(const cmbl_gibbs_sampler* sampler, double* a4);
 
 /* ---------------------------------------------------------------- device-resident loops */
 /* The reference's optimiser and sampler loops with every vector on the GPU

@@ -54,6 +54,7 @@ def _load():
     lib.orc_beam.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
     lib.orc_mask_latitude.argtypes = [ctypes.c_int, ctypes.c_double, _dp]
     lib.orc_mask_regions.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
+    lib.orc_gauss.argtypes = [ctypes.c_long, ctypes.c_long, _dp]
     lib.orc_num_threads.restype = ctypes.c_int
     lib.orc_set_threads.argtypes = [ctypes.c_int]
     return lib
@@ -343,6 +344,107 @@ def cg_solve(apply, precond, b, precision=1e-7, max_iter=None):
         it += 1
         if eps <= precision * precision or (max_iter is not None and it >= max_iter):
             return x, it
+
+
+# ---------------------------------------------------------------- Gibbs sampler (source/cmb_gibbs.cpp)
+
+def gauss(seed, n):
+    """n draws of Math::GaussianGenerator(seed, 0, 1) (include/random.hpp:32-52)."""
+    out = np.zeros(int(n))
+    lib().orc_gauss(int(seed), int(n), out)
+    return out
+
+
+class _Draws:
+    """One GaussianGenerator's stream, consumed in order."""
+
+    def __init__(self, seed):
+        self.seed, self.buf, self.pos = int(seed), np.zeros(0), 0
+
+    def take(self, n):
+        if self.pos + n > self.buf.size:
+            self.buf = gauss(self.seed, max(2 * self.buf.size, self.pos + n, 1 << 16))
+        out = self.buf[self.pos:self.pos + n]
+        self.pos += n
+        return out
+
+
+def sigma_l(s, L):
+    """calculateSigmaL (source/cmb_gibbs.cpp:541-549): |s_l0|^2 + 2 sum_m |s_lm|^2."""
+    s = np.asarray(s, dtype=np.float64)
+    return np.array([s[l * l] ** 2 + 2 * np.sum(s[l * l + 1:(l + 1) * (l + 1)] ** 2) for l in range(L + 1)])
+
+
+class GibbsSampler:
+    """CMBGibbsSampler (source/cmb_gibbs.cpp:63-549) on the oracle's transforms.
+
+    map, noise and mask RING-ordered; beam B_l (the reference's pixel window
+    times the Gaussian, source/cmb_gibbs.cpp:116-117, is the caller's)."""
+
+    def __init__(self, nside, L, m, noise, mask, sigma, beam, starting_cl, seed):
+        self.nside, self.L, self.sigma = nside, L, float(sigma)
+        self.npix = 12 * nside * nside
+        m = np.asarray(m, dtype=np.float64)
+        self.data = m + np.asarray(noise, dtype=np.float64)          # :92-93
+        self.mask = np.asarray(mask, dtype=np.float64)
+        self.good = self.mask > 0.5
+        self.beam = np.asarray(beam, dtype=np.float64)[:L + 1].copy()
+        self.cl = np.asarray(starting_cl, dtype=np.float64)[:L + 1].copy()
+        th = np.zeros(self.npix)
+        ph = np.zeros(self.npix)
+        for p in range(self.npix):
+            th[p], ph[p] = pix2ang_ring(nside, p)
+        f = np.sqrt(3.0 / (4.0 * np.pi))                               # :133-143
+        self.Y = np.stack([np.full(self.npix, 1.0 / np.sqrt(4.0 * np.pi)), f * np.sin(th) * np.sin(ph),
+                           f * np.cos(th), f * np.sin(th) * np.cos(ph)])
+        self.s = quadrature(nside, L, m) / self.beam[packed_l(L)]     # :145-159
+        self.sigma_l = sigma_l(self.s, L)
+        self.a = np.zeros(4)
+        self.draws = _Draws(seed)
+        self.iterations = 0
+
+    def _w_operator(self, u):
+        """CmbGibbsMDTreats::multiplyByMatrix (source/cmb_gibbs.cpp:202-230)."""
+        mm = u[0] * self.Y[0] + u[1] * self.Y[1] + u[2] * self.Y[2] + u[3] * self.Y[3]
+        mm = np.where(self.good, mm / (self.sigma * self.sigma), 0.0)
+        return self.Y @ mm
+
+    def generate_w(self):
+        """source/cmb_gibbs.cpp:240-301."""
+        v = self.data - synthesis(self.nside, self.L, self.s, self.beam)
+        o = self.draws.take(4 * self.npix).reshape(self.npix, 4)
+        v = np.where(self.good, v / (self.sigma * self.sigma), 0.0)
+        o = np.where(self.good[:, None], o / self.sigma, 0.0)
+        v = v + (((o[:, 0] + o[:, 1]) + o[:, 2]) + o[:, 3])
+        b = self.Y @ v
+        self.a, _ = cg_solve(self._w_operator, lambda r: r.copy(), b, precision=1e-20)
+
+    def generate_signal(self, precision=1e-20, max_iter=None):
+        """source/cmb_gibbs.cpp:437-531."""
+        o = self.draws.take(2 * self.npix).reshape(self.npix, 2)
+        d = self.data - self.a[0] * self.Y[0] - self.a[1] * self.Y[1] - self.a[2] * self.Y[2] - self.a[3] * self.Y[3]
+        args = (self.cl, self.beam, self.mask, self.sigma)
+        b = gibbs_rhs(self.nside, self.L, d, o[:, 0], o[:, 1], *args)
+        x, self.iterations = cg_solve(lambda v: gibbs_apply(self.nside, self.L, v, *args),
+                                      lambda v: gibbs_precondition(self.L, v, *args), b, precision, max_iter)
+        self.s = x * np.sqrt(self.cl)[packed_l(self.L)]
+        self.sigma_l = sigma_l(self.s, self.L)
+
+    def generate_cl(self):
+        """source/cmb_gibbs.cpp:178-193."""
+        self.cl[0] = self.cl[1] = 0.0
+        for l in range(2, self.L + 1):
+            r = self.draws.take(2 * l - 1)
+            rho = 0.0
+            for v in r:
+                rho += v * v
+            self.cl[l] = self.sigma_l[l] / rho
+
+    def step(self, precision=1e-20, max_iter=None):
+        """source/cmb_gibbs.cpp:533-538."""
+        self.generate_w()
+        self.generate_signal(precision, max_iter)
+        self.generate_cl()
 
 
 # ---------------------------------------------------------------- reference shim

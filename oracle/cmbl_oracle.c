@@ -40,6 +40,13 @@
  *   orc_mask_latitude / orc_mask_regions  source/test_like_high.cpp:123-130,
  *       Utils::maskRegions (source/utils.cpp:235-253).  Pinned: brute-force
  *       pixel counts and spherical-cap area.
+ *   orc_gauss  Math::GaussianGenerator(seed, 0, 1) draws (include/random.hpp:
+ *       32-52: std::mt19937 + std::normal_distribution<double>), restated as
+ *       MT19937 (Matsumoto & Nishimura 1998) feeding 53-bit uniforms made of
+ *       two 32-bit words, then the Marsaglia polar method returning the
+ *       second variate of each pair first.  Pinned: the reference's own draws
+ *       for fixed seeds (tests/golden/rng.npz) and, where oracle/_ref builds,
+ *       the reference generator run live.
  * Bit-level parity with libsharp's summation order is not reproducible here
  * (HEALPix is absent); agreement is asserted to 1e-10 relative.
  */
@@ -70,6 +77,78 @@ void orc_set_threads(int n)
 #else
     (void)n;
 #endif
+}
+
+/* ------------------------------------------------------------------ */
+/* Gaussian draws                                                      */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    unsigned int mt[624];
+    int idx;
+    int have_saved;
+    double saved;
+} orc_rng;
+
+static void rng_seed(orc_rng* g, unsigned int seed)
+{
+    g->mt[0] = seed;
+    for (int i = 1; i < 624; ++i)
+        g->mt[i] = 1812433253u * (g->mt[i - 1] ^ (g->mt[i - 1] >> 30)) + (unsigned int)i;
+    g->idx = 624;
+    g->have_saved = 0;
+}
+
+static unsigned int rng_u32(orc_rng* g)
+{
+    if (g->idx >= 624) {
+        for (int i = 0; i < 624; ++i) {
+            const unsigned int y = (g->mt[i] & 0x80000000u) | (g->mt[(i + 1) % 624] & 0x7fffffffu);
+            g->mt[i] = g->mt[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        g->idx = 0;
+    }
+    unsigned int y = g->mt[g->idx++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+/* Uniform in [0, 1) from two 32-bit words, low word first. */
+static double rng_unit(orc_rng* g)
+{
+    const double lo = (double)rng_u32(g);
+    const double hi = (double)rng_u32(g);
+    double u = (lo + hi * 4294967296.0) / 18446744073709551616.0;
+    if (u >= 1.0) u = nextafter(1.0, 0.0);
+    return u;
+}
+
+static double rng_normal(orc_rng* g)
+{
+    if (g->have_saved) {
+        g->have_saved = 0;
+        return g->saved;
+    }
+    double x, y, r2;
+    do {
+        x = 2.0 * rng_unit(g) - 1.0;
+        y = 2.0 * rng_unit(g) - 1.0;
+        r2 = x * x + y * y;
+    } while (r2 > 1.0 || r2 == 0.0);
+    const double f = sqrt(-2.0 * log(r2) / r2);
+    g->saved = x * f;
+    g->have_saved = 1;
+    return y * f;
+}
+
+void orc_gauss(long seed, long n, double* out)
+{
+    orc_rng g;
+    rng_seed(&g, (unsigned int)(int)seed);
+    for (long i = 0; i < n; ++i) out[i] = rng_normal(&g);
 }
 
 /* ------------------------------------------------------------------ */

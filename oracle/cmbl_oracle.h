@@ -81,6 +81,8 @@ void orc_mask_latitude(int nside, double cut_rad, double* mask);
 void orc_mask_regions(int nside, int nreg, const double* theta, const double* phi,
                       const double* angle, double* mask);
 
+/* n draws of Math::GaussianGenerator(seed, 0, 1) (include/random.hpp:32-52). */
+void orc_gauss(long seed, long n, double* out);
 int orc_num_threads(void);
 /* OpenMP threads of the calling thread's later parallel regions (timing). */
 void orc_set_threads(int n);

@@ -136,3 +136,59 @@ def device_case(cfg, seed):
     mp.set_data(d, w)
     x = a_true + 0.1 * cm.simulate_alm(cl, lmax, seed + 2)
     return mp, dict(cl=cl, beam=b, d=d, w=w, x=x)
+
+
+# The Gibbs sampler subtests of source/test_cmb_gibbs.cpp:64-99 (0: nside 16,
+# Gibbs lmax 30, FWHM 10 deg, sigma 1; 1: nside 32, lmax 50, FWHM 5 deg,
+# sigma 2; 4: as 1 with the reference's mask1.fits).  The simulated sky has
+# lmax 64 / 128 as there; its spectrum is C_l = 2pi 1000 / (l(l+1)) (the
+# reference's CLASS LCDM spectrum is not available), the beam is the Gaussian
+# of the FWHM (the reference's also carries HEALPix's pixel window).
+GIBBS_SUBTESTS = {
+    0: dict(nside=16, lsim=64, lmax=30, fwhm=10.0, sigma=1.0, scale=0.4, mask=False),
+    1: dict(nside=32, lsim=128, lmax=50, fwhm=5.0, sigma=2.0, scale=0.4, mask=False),
+    4: dict(nside=32, lsim=128, lmax=50, fwhm=5.0, sigma=2.0, scale=5.0, mask=True),
+}
+
+
+def gibbs_case(i, seed=21, ring_mask=None):
+    """Inputs of Gibbs subtest i: dict(nside, lmax, map, noise, mask, sigma,
+    fwhm, beam, cl0, cl_original, seed, scale).  ring_mask supplies mask1 in
+    RING order for subtest 4."""
+    import oracle
+
+    c = GIBBS_SUBTESTS[i]
+    nside, lsim, lmax = c["nside"], c["lsim"], c["lmax"]
+    cl_true = np.zeros(lsim + 1)
+    l = np.arange(2, lsim + 1)
+    cl_true[2:] = 2 * np.pi * 1000.0 / (l * (l + 1))
+    a = cm.simulate_alm(cl_true, lsim, seed)
+    m = oracle.synthesis(nside, lsim, a, cm.beam(lsim, c["fwhm"]))
+    noise = cm.simulate_white_noise(nside, c["sigma"], seed + 1)
+    mask = np.ones(12 * nside * nside) if not c["mask"] else np.asarray(ring_mask, dtype=np.float64)
+    cl0 = np.full(lmax + 1, 1000.0)                  # source/test_cmb_gibbs.cpp:143-146
+    ll = np.arange(1, lmax + 1)
+    cl0[1:] *= 2 * np.pi / (ll * (ll + 1))
+    cl0[0] = cl0[1]
+    orig = np.array([(a[k * k] ** 2 + 2 * np.sum(a[k * k + 1:(k + 1) ** 2] ** 2)) / (2 * k + 1)
+                     for k in range(lmax + 1)])
+    return dict(nside=nside, lmax=lmax, map=m, noise=noise, mask=mask, sigma=c["sigma"], fwhm=c["fwhm"],
+                beam=cm.beam(lmax, c["fwhm"]), cl0=cl0, cl_original=orig, seed=seed + 2, scale=c["scale"])
+
+
+def gibbs_acceptance(case, sigma_l_chain):
+    """source/test_cmb_gibbs.cpp:206-229: for l = 2..lmax the chain's C_l
+    must match the realisation's within the relative scale (Math::areEqual,
+    include/numerics.hpp:17-25: |est - orig| / |orig| < scale).
+    The chain's C_l estimate is the mean over samples of sigma_l / (2l+1), the
+    mode of the inverse-gamma draw of generateCl, in place of the peak of the
+    smoothed posterior.  Returns the list of failing l."""
+    s = np.asarray(sigma_l_chain, dtype=np.float64)
+    L = case["lmax"]
+    bad = []
+    for l in range(2, L + 1):
+        est = s[:, l].mean() / (2 * l + 1)
+        orig = case["cl_original"][l]
+        if not abs(est - orig) < case["scale"] * abs(orig):
+            bad.append((l, orig, est))
+    return bad

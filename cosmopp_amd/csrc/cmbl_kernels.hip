@@ -62,6 +62,46 @@ __device__ __forceinline__ void wave_sync()
 // which the compiler must order after earlier LDS and global stores.
 typedef __attribute__((address_space(1))) double gdouble;
 typedef const __attribute__((address_space(1))) double gcdouble;
+typedef double __attribute__((ext_vector_type(2))) dvec2;
+
+// Views of double2 arrays in global memory (16-byte loads and stores).
+struct GCView2 {
+    const __attribute__((address_space(1))) dvec2* p;
+    __device__ __forceinline__ double2 operator[](long i) const
+    {
+        const dvec2 v = p[i];
+        return make_double2(v.x, v.y);
+    }
+    __device__ __forceinline__ GCView2 operator+(long k) const { return GCView2{p + k}; }
+};
+struct GView2 {
+    __attribute__((address_space(1))) dvec2* p;
+    struct Ref {
+        __attribute__((address_space(1))) dvec2* q;
+        __device__ __forceinline__ void operator=(double2 a) const
+        {
+            dvec2 v;
+            v.x = a.x;
+            v.y = a.y;
+            *q = v;
+        }
+        __device__ __forceinline__ operator double2() const
+        {
+            const dvec2 v = *q;
+            return make_double2(v.x, v.y);
+        }
+    };
+    __device__ __forceinline__ Ref operator[](long i) const { return Ref{p + i}; }
+    __device__ __forceinline__ GView2 operator+(long k) const { return GView2{p + k}; }
+};
+
+__device__ __forceinline__ gcdouble* gview(const double* p) { return (gcdouble*)p; }
+__device__ __forceinline__ gdouble* gview(double* p) { return (gdouble*)p; }
+__device__ __forceinline__ GCView2 gview(const double2* p)
+{
+    return GCView2{(const __attribute__((address_space(1))) dvec2*)p};
+}
+__device__ __forceinline__ GView2 gview(double2* p) { return GView2{(__attribute__((address_space(1))) dvec2*)p}; }
 
 // ================================================================== k_pre / k_post
 // The packed parameter vector (almToVector: l-major, index l^2 + 2m - 1 + c)
@@ -1510,7 +1550,7 @@ __device__ __forceinline__ int zp(int e) { return e + (e >> 4); }
 
 __device__ __forceinline__ double2 tw_get(const double2* __restrict__ tw, int t, int sign)
 {
-    double2 w = tw[t];
+    double2 w = gview(tw)[t];
     if (sign > 0) w.y = -w.y;
     return w;
 }
@@ -1566,7 +1606,9 @@ __device__ __forceinline__ constexpr int brev_c(int q)
 // already applied: butterfly i (k = i mod p) reads x[i + r M/R], twiddles by
 // e^{SIGN 2 pi i r k/(p R)}, transforms, and writes y[(i - k) R + k + q p].
 // Every thread loads all its butterflies before the first barrier and stores
-// after it, so the pass works in place.
+// after it, so the pass works in place.  The twiddle loads (from the table in
+// global memory, mostly L2 hits) are issued first, so their latency runs
+// alongside the LDS loads and the barrier instead of after them.
 template <int R, int LR, int SIGN, int MAXM>
 __device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, const double2* __restrict__ tw)
 {
@@ -1575,6 +1617,14 @@ __device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, const do
     const int tid = threadIdx.x;
     const int nb = M / R;
     double2 u[B][R];
+    double2 w1v[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int i = tid + b * NT;
+        const int k = i & (p - 1);
+        w1v[b] = make_double2(1.0, 0.0);
+        if (i < nb && k) w1v[b] = tw_get(tw, k * (kMaxFFT / (p * R)), SIGN);
+    }
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int i = tid + b * NT;
@@ -1590,7 +1640,7 @@ __device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, const do
         if (i < nb) {
             const int k = i & (p - 1);
             if (k) {
-                const double2 w1 = tw_get(tw, k * (kMaxFFT / (p * R)), SIGN);
+                const double2 w1 = w1v[b];
                 double2 w = w1;
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
@@ -1630,12 +1680,16 @@ __device__ void fft(double2* Z, int M, int logM, const double2* __restrict__ tw)
 // Bluestein DFT of z[0..n) (natural order, zero-padded by this call to M):
 // result in natural order in z[0..n).  chirp[j] = e^{-i pi j^2/n};
 // filt = FFT_M of e^{+i pi j^2/n} (wrapped), natural order.  For SIGN > 0 the
-// chirps are conjugated and the filter is read reversed and conjugated.
+// chirps are conjugated and the filter is read reversed and conjugated.  The
+// loops are unrolled by four so that four table loads are in flight at once.
 template <int SIGN, int MAXM>
-__device__ void bluestein(double2* z, int n, int M, int logM, const double2* __restrict__ chirp,
-                          const double2* __restrict__ filt, const double2* __restrict__ tw)
+__device__ void bluestein(double2* z, int n, int M, int logM, const double2* __restrict__ chirp_,
+                          const double2* __restrict__ filt_, const double2* __restrict__ tw)
 {
+    auto chirp = gview(chirp_);
+    auto filt = gview(filt_);
     const int tid = threadIdx.x, nt = blockDim.x;
+#pragma unroll 4
     for (int j = tid; j < M; j += nt) {
         if (j < n) {
             double2 c = chirp[j];
@@ -1647,6 +1701,7 @@ __device__ void bluestein(double2* z, int n, int M, int logM, const double2* __r
     }
     __syncthreads();
     fft<-1, MAXM>(z, M, logM, tw);
+#pragma unroll 4
     for (int r = tid; r < M; r += nt) {
         double2 f;
         if (SIGN < 0) {
@@ -1660,6 +1715,7 @@ __device__ void bluestein(double2* z, int n, int M, int logM, const double2* __r
     __syncthreads();
     fft<+1, MAXM>(z, M, logM, tw);
     const double inv = 1.0 / (double)M;
+#pragma unroll 4
     for (int k = tid; k < n; k += nt) {
         double2 c = chirp[k];
         if (SIGN > 0) c.y = -c.y;
@@ -1882,6 +1938,11 @@ __global__ __launch_bounds__(ring_threads(MAXM), MAXM > 4096 ? 1 : 4) void k_rin
     const double2* filt = pow2 ? nullptr : md.bs_filt + md.ring_bsofs[p];
     const double2* chirp = pow2 ? nullptr : md.bs_chirp + md.ring_chofs[p];
     const int tid = threadIdx.x;
+    constexpr int CNT = (MAXM + NT - 1) / NT;  // ring samples per thread, at most
+    gdouble* out = gview(map_out);
+    gcdouble* in = gview(map_in);
+    gcdouble* dd = gview(md.d);
+    gcdouble* ww = gview(md.w);
     // for n < NT the aliases of each bin are split over R thread
     // groups (reads stay coalesced over k), partial sums added in fixed order
     const int R = n < NT ? NT / n : 1;
@@ -1889,7 +1950,48 @@ __global__ __launch_bounds__(ring_threads(MAXM), MAXM > 4096 ? 1 : 4) void k_rin
     if (MODE != kRingAna) {
         const double2* row = reinterpret_cast<const double2*>(md.P + (long)p * (L + 1) * 4);
         // fold: Z_k = G_N,k + i G_S,k
-        if (R == 1) {
+        if (R == 1 && n > L) {
+            // at most the alias k (k <= L) and the mirror n - k (n - k <= L)
+            // per bin, and both only where n - L <= k <= L: one load per bin
+            // issued for every bin of the thread before the sums, the rare
+            // second one where the sums need it
+            auto rw = gview(row);
+            double2 e[CNT], o[CNT];
+#pragma unroll
+            for (int t = 0; t < CNT; ++t) {
+                const int k = tid + t * NT;
+                const int m = k <= L ? k : n - k;
+                if (k < n && (k <= L || (k >= 1 && n - k <= L))) {
+                    e[t] = rw[2 * m];
+                    o[t] = rw[2 * m + 1];
+                }
+            }
+            const double sg = shifted ? -1.0 : 1.0;  // (-1)^{j shift} at j = 1
+#pragma unroll
+            for (int t = 0; t < CNT; ++t) {
+                const int k = tid + t * NT;
+                if (k >= n) continue;
+                double2 gN = make_double2(0.0, 0.0), gS = make_double2(0.0, 0.0);
+                const bool mirror = k >= 1 && n - k <= L;
+                if (k <= L) {
+                    gN.x += 1.0 * (e[t].x + o[t].x);
+                    gN.y += 1.0 * (e[t].y + o[t].y);
+                    gS.x += 1.0 * (e[t].x - o[t].x);
+                    gS.y += 1.0 * (e[t].y - o[t].y);
+                    if (mirror) {
+                        e[t] = rw[2 * (n - k)];
+                        o[t] = rw[2 * (n - k) + 1];
+                    }
+                }
+                if (mirror) {
+                    gN.x += sg * (e[t].x + o[t].x);
+                    gN.y -= sg * (e[t].y + o[t].y);
+                    gS.x += sg * (e[t].x - o[t].x);
+                    gS.y -= sg * (e[t].y - o[t].y);
+                }
+                fold_store(Z, k, k, n, shifted, oS < 0, gN, gS);
+            }
+        } else if (R == 1) {
             for (int k = tid; k < n; k += NT) {
                 double2 gN, gS;
                 fold_aliases(row, L, n, k, 0, 1, shifted, gN, gS);
@@ -1921,23 +2023,40 @@ __global__ __launch_bounds__(ring_threads(MAXM), MAXM > 4096 ? 1 : 4) void k_rin
     if (MODE == kRingSyn) {
         for (int j = tid; j < n; j += NT) {
             const double2 v = Z[zp(j)];
-            map_out[oN + j] = v.x;
-            if (oS >= 0) map_out[oS + j] = v.y;
+            out[oN + j] = v.x;
+            if (oS >= 0) out[oS + j] = v.y;
         }
         return;
     }
 
+    // per-pixel inputs of the thread's samples, all loaded before use
     if (MODE == kRingLike) {
+        double dN[CNT], wN[CNT], dS[CNT], wS[CNT];
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int j = tid + t * NT;
+            if (j < n) {
+                dN[t] = dd[oN + j];
+                wN[t] = ww[oN + j];
+                if (oS >= 0) {
+                    dS[t] = dd[oS + j];
+                    wS[t] = ww[oS + j];
+                }
+            }
+        }
         double chi = 0.0;
-        for (int j = tid; j < n; j += NT) {
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int j = tid + t * NT;
+            if (j >= n) continue;
             const double2 v = Z[zp(j)];
-            const double rN = md.d[oN + j] - v.x;
-            const double fN = md.w[oN + j] * rN;
+            const double rN = dN[t] - v.x;
+            const double fN = wN[t] * rN;
             chi = fma(fN, rN, chi);
             double fS = 0.0;
             if (oS >= 0) {
-                const double rS = md.d[oS + j] - v.y;
-                fS = md.w[oS + j] * rS;
+                const double rS = dS[t] - v.y;
+                fS = wS[t] * rS;
                 chi = fma(fS, rS, chi);
             }
             Z[zp(j)] = make_double2(fN, fS);
@@ -1945,14 +2064,35 @@ __global__ __launch_bounds__(ring_threads(MAXM), MAXM > 4096 ? 1 : 4) void k_rin
         const double t = block_sum<NT>(chi, sh);
         if (tid == 0) md.chi[p] = t;
     } else if (MODE == kRingWeight) {
-        for (int j = tid; j < n; j += NT) {
+        double wN[CNT], wS[CNT];
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int j = tid + t * NT;
+            if (j < n) {
+                wN[t] = ww[oN + j];
+                wS[t] = oS >= 0 ? ww[oS + j] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int j = tid + t * NT;
+            if (j >= n) continue;
             const double2 v = Z[zp(j)];
-            Z[zp(j)] = make_double2(md.w[oN + j] * v.x, oS >= 0 ? md.w[oS + j] * v.y : 0.0);
+            Z[zp(j)] = make_double2(wN[t] * v.x, oS >= 0 ? wS[t] * v.y : 0.0);
         }
         __syncthreads();
     } else {  // kRingAna: load the map to analyse
-        for (int j = tid; j < n; j += NT)
-            Z[zp(j)] = make_double2(map_in[oN + j], oS >= 0 ? map_in[oS + j] : 0.0);
+        double2 mv[CNT];
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int j = tid + t * NT;
+            if (j < n) mv[t] = make_double2(in[oN + j], oS >= 0 ? in[oS + j] : 0.0);
+        }
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int j = tid + t * NT;
+            if (j < n) Z[zp(j)] = mv[t];
+        }
         __syncthreads();
     }
     ring_dft<-1, MAXM>(Z, n, M, logM, pow2, chirp, filt, tw);

@@ -19,6 +19,7 @@
 // x / y), so the loops produce the same iterates, bit for bit, as the
 // reference's LBFGS_General / HMCGeneral running on cmbl::DeviceLargeVector.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <future>
@@ -894,14 +895,20 @@ struct cmbl_hmc {
     long trajectories = 0, accepted = 0, evaluations = 0, gradCalls = 0;
     // momentum draws: pinned host buffers filled by worker threads kSlots
     // trajectories ahead, uploaded on a copy stream into the device buffer of
-    // the trajectory's parity while the previous trajectory runs
+    // the trajectory's parity while the previous trajectory runs.  The
+    // workers keep drawing ahead between runs (the seeds of later
+    // trajectories are known), so a run does not start by waiting for draws.
     static constexpr int kSlots = 12;
     std::vector<double*> slot;
     std::vector<hipEvent_t> slotEv;  // upload from the slot complete
+    std::vector<std::future<int>> fut;  // worker filling each slot
+    std::vector<long> slotSeed;         // seed drawn (or being drawn) into each slot
     hipStream_t copy = nullptr;
     hipEvent_t upDone[2] = {nullptr, nullptr}, used[2] = {nullptr, nullptr};
     ~cmbl_hmc()
     {
+        for (auto& f : fut)
+            if (f.valid()) f.wait();
         cache.clear();
         if (stream) (void)hipStreamSynchronize(stream);
         if (copy) (void)hipStreamSynchronize(copy);
@@ -991,6 +998,8 @@ int cmbl_hmc_create(cmbl_hmc** out, cmbl_map* map, const double* d_start, const 
             return bail(fail(CMBL_ERR_HIP, "hipEventCreate failed"));
         hm->slotEv.push_back(e);
     }
+    hm->fut.resize(cmbl_hmc::kSlots);
+    hm->slotSeed.assign(cmbl_hmc::kSlots, LONG_MIN);
     // x = start, M = masses, sqrt(M) as the reference forms it (pow(M, 0.5))
     // copies on this context's stream, ordered before the square root (a
     // device-to-device hipMemcpy does not wait for completion, and this
@@ -1031,25 +1040,35 @@ int cmbl_hmc_run(cmbl_hmc* hm, int max_iters, cmbl_hmc_callback cb, void* user)
     // refilled once its upload has completed), uploaded on the copy stream
     // into noise[k % 2] once trajectory k - 2 has finished with it
     const int W = cmbl_hmc::kSlots;
-    const int seed0 = hm->gaussSeed;
-    std::vector<std::future<int>> fut(W);
-    auto produce = [hm, n, seed0](int k) -> int {
-        const int sl = k % cmbl_hmc::kSlots;
+    const long seed0 = hm->gaussSeed;
+    auto slot_of = [W](long seed) { return (int)(((seed % W) + W) % W); };
+    auto produce = [hm, n](long seed, int sl) -> int {
         if (hipEventSynchronize(hm->slotEv[sl]) != hipSuccess) return CMBL_ERR_HIP;
-        return cmbl_whitenoise(seed0 + k, 1.0, n, hm->slot[sl]);
+        return cmbl_whitenoise((int)seed, 1.0, n, hm->slot[sl]);
+    };
+    auto schedule = [&](long seed) {
+        const int sl = slot_of(seed);
+        if (hm->fut[sl].valid()) (void)hm->fut[sl].get();  // an unused earlier draw in that slot
+        hm->slotSeed[sl] = seed;
+        hm->fut[sl] = std::async(std::launch::async, produce, seed, sl);
     };
     auto upload = [&](int k) -> int {
-        const int sl = k % W, par = k & 1;
-        const int rc = fut[sl].get();
+        const long seed = seed0 + k;
+        const int sl = slot_of(seed), par = k & 1;
+        if (hm->slotSeed[sl] != seed || !hm->fut[sl].valid()) return fail(CMBL_ERR_STATE, "momentum draw missing");
+        const int rc = hm->fut[sl].get();
+        hm->slotSeed[sl] = LONG_MIN;
         if (rc != CMBL_OK) return fail(rc, "momentum draw failed");
         CMBL_HIP(hipStreamWaitEvent(cs, hm->used[par], 0));
         CMBL_HIP(hipMemcpyAsync(d.noise[par], hm->slot[sl], bytes, hipMemcpyHostToDevice, cs));
         CMBL_HIP(hipEventRecord(hm->upDone[par], cs));
         CMBL_HIP(hipEventRecord(hm->slotEv[sl], cs));
-        if (k + W < max_iters) fut[sl] = std::async(std::launch::async, produce, k + W);
+        schedule(seed + W);  // also past this run: the next run starts with its draws made
         return CMBL_OK;
     };
-    for (int k = 0; k < std::min(W, max_iters); ++k) fut[k] = std::async(std::launch::async, produce, k);
+    // draws left by the previous run are kept; the rest of the window starts now
+    for (int k = 0; k < W; ++k)
+        if (hm->slotSeed[slot_of(seed0 + k)] != seed0 + k) schedule(seed0 + k);
     int rc = upload(0);
     for (int iter = 0; iter < max_iters && rc == CMBL_OK; ++iter) {
         double tau;
@@ -1110,9 +1129,8 @@ int cmbl_hmc_run(cmbl_hmc* hm, int max_iters, cmbl_hmc_callback cb, void* user)
         }
         if (cb) cb(user, d.x, hm->like);
     }
-    // outstanding draws and uploads finish before the buffers can go
-    for (auto& f : fut)
-        if (f.valid()) (void)f.get();
+    // uploads finish before the call returns; the workers drawing ahead keep
+    // going (the destructor waits for them)
     if (hipStreamSynchronize(cs) != hipSuccess && rc == CMBL_OK) rc = fail(CMBL_ERR_HIP, "momentum uploads failed");
     return rc;
 }

@@ -1764,14 +1764,33 @@ __device__ __forceinline__ void fold_aliases(const double2* __restrict__ row, in
     }
 }
 
-// Bin k of the packed pair spectrum (natural order): phase e^{i pi k/n} for
-// shifted rings, no south ring at the equator.
-__device__ __forceinline__ void fold_store(double2* Z, int slot, int k, int n, bool shifted, bool no_south,
+// e^{i pi (k0 + t D) / n} for t = 0, 1, 2, ...: the first two factors from
+// sincospi, each later term by one complex multiply (a thread's bins and
+// samples step by a fixed D; the rounding grows by about an ulp per step, over
+// at most 16 steps).  A phase by sincospi costs a 64-bit remainder, an FP64
+// division and the polynomial, about a hundred instructions.
+struct PhaseSeq {
+    double2 cur, step;
+    __device__ __forceinline__ void init(long k0, long D, long n)
+    {
+        cur = cis_pi_frac(k0, n);
+        step = cis_pi_frac(D, n);
+    }
+    __device__ __forceinline__ double2 next()
+    {
+        const double2 c = cur;
+        cur = cmul(cur, step);
+        return c;
+    }
+};
+
+// Bin of the packed pair spectrum (natural order): phase ph = e^{i pi k/n}
+// (bin k) for shifted rings, no south ring at the equator.
+__device__ __forceinline__ void fold_store(double2* Z, int slot, double2 ph, bool shifted, bool no_south,
                                            double2 gN, double2 gS)
 {
     if (no_south) gS = make_double2(0.0, 0.0);
     if (shifted) {
-        const double2 ph = cis_pi_frac(k, n);
         gN = cmul(gN, ph);
         gS = cmul(gS, ph);
     }
@@ -1779,36 +1798,35 @@ __device__ __forceinline__ void fold_store(double2* Z, int slot, int k, int n, b
 }
 
 // Unfold bin k and its mirror n - k of Y into Q (the aliases m = k + j n,
-// j = r, r + R, ...): Psi_X,m = e^{-i pi k/n} (-1)^j H_X,k.
-__device__ __forceinline__ void unfold_bin(double2* qrow, int L, int n, int k, int r, int R, bool shifted, double2 y,
-                                           double2 z2)
+// j = r, r + R, ...): Psi_X,m = e^{-i pi k/n} (-1)^j H_X,k; ph = e^{i pi k/n}
+// (used for shifted rings).
+__device__ __forceinline__ void unfold_bin(double2* qrow, int L, int n, int k, int r, int R, bool shifted,
+                                           double2 ph, double2 y, double2 z2)
 {
     const double2 yc = make_double2(z2.x, -z2.y);
     double2 hN = make_double2(0.5 * (y.x + yc.x), 0.5 * (y.y + yc.y));
     double2 hS = make_double2(0.5 * (y.y - yc.y), -0.5 * (y.x - yc.x));
     if (shifted) {
-        double2 ph = cis_pi_frac(k, n);
         ph.y = -ph.y;
         hN = cmul(hN, ph);
         hS = cmul(hS, ph);
     }
     const double2 pe = cadd(hN, hS), po = csub(hN, hS);
+    auto q = gview(qrow);
     for (int j = r; k + (long)j * n <= L; j += R) {
         const int mm = k + j * n;
         const double sg = (shifted && (j & 1)) ? -1.0 : 1.0;
-        qrow[2 * mm] = make_double2(sg * pe.x, sg * pe.y);
-        qrow[2 * mm + 1] = make_double2(sg * po.x, sg * po.y);
+        q[2 * mm] = make_double2(sg * pe.x, sg * pe.y);
+        q[2 * mm + 1] = make_double2(sg * po.x, sg * po.y);
     }
 }
 
 // Y_k of a split ring from its two half transforms: SE (global scratch) and
-// Z (LDS), Y_k = E_{k mod h} + e^{-2 pi i k/n} O_{k mod h}.
-__device__ __forceinline__ double2 split_bin(const double2* SE, const double2* Z, int k, int n, int h)
+// Z (LDS), Y_k = E_{k mod h} + w O_{k mod h}, w = e^{-2 pi i k/n}.
+__device__ __forceinline__ double2 split_bin(const double2* SE, const double2* Z, int k, int h, double2 w)
 {
-    double2 w = cis_pi_frac(2 * (long)k % (2L * n), n);
-    w.y = -w.y;
     const int kk = k % h;
-    return cadd(SE[kk], cmul(Z[zp(kk)], w));
+    return cadd(gview(SE)[kk], cmul(Z[zp(kk)], w));
 }
 
 // A ring pair whose length-n Bluestein transform would not fit in LDS (n = 4i
@@ -1837,11 +1855,13 @@ __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, boo
     if (MODE != kRingAna) {
         const double2* row = reinterpret_cast<const double2*>(md.P + (long)p * (L + 1) * 4);
         for (int r = 0; r < 2; ++r) {
+            PhaseSeq ps;
+            if (shifted) ps.init(2 * tid + r, 2 * NT, n);
             for (int kk = tid; kk < h; kk += NT) {
                 const int k = 2 * kk + r;
                 double2 gN, gS;
                 fold_aliases(row, L, n, k, 0, 1, shifted, gN, gS);
-                fold_store(Z, kk, k, n, shifted, oS < 0, gN, gS);
+                fold_store(Z, kk, shifted ? ps.next() : make_double2(1.0, 0.0), shifted, oS < 0, gN, gS);
             }
             __syncthreads();
             bluestein<+1, MAXM>(Z, h, M, logM, chirp, filt, tw);
@@ -1851,9 +1871,11 @@ __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, boo
             }
         }
         double chi = 0.0;
+        PhaseSeq tw2;  // e^{2 pi i j / n}
+        tw2.init(2 * tid, 2 * NT, n);
         for (int j = tid; j < h; j += NT) {
             const double2 e = SE[j];
-            const double2 o = cmul(Z[zp(j)], cis_pi_frac(2 * j, n));
+            const double2 o = cmul(Z[zp(j)], tw2.next());
             const double2 sv[2] = {cadd(e, o), csub(e, o)};
 #pragma unroll
             for (int half = 0; half < 2; ++half) {
@@ -1901,9 +1923,16 @@ __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, boo
     }
 
     double2* qrow = reinterpret_cast<double2*>(md.Q + (long)p * (L + 1) * 4);
+    PhaseSeq wk, pk;  // e^{2 pi i k/n} and e^{i pi k/n}
+    wk.init(2 * tid, 2 * NT, n);
+    if (shifted) pk.init(tid, NT, n);
     for (int k = tid; k < n && k <= L; k += NT) {
         const int nk = (k == 0) ? 0 : n - k;
-        unfold_bin(qrow, L, n, k, 0, 1, shifted, split_bin(SE, Z, k, n, h), split_bin(SE, Z, nk, n, h));
+        const double2 a = wk.next();  // e^{-2 pi i (n-k)/n} = a, e^{-2 pi i k/n} = conj(a)
+        const double2 ph = shifted ? pk.next() : make_double2(1.0, 0.0);
+        unfold_bin(qrow, L, n, k, 0, 1, shifted, ph, split_bin(md.ring_scr + md.ring_scrofs[p], Z, k, h,
+                                                                 make_double2(a.x, -a.y)),
+                   split_bin(md.ring_scr + md.ring_scrofs[p], Z, nk, h, a));
     }
 }
 
@@ -1967,10 +1996,13 @@ __global__ __launch_bounds__(ring_threads(MAXM), MAXM > 4096 ? 1 : 4) void k_rin
                 }
             }
             const double sg = shifted ? -1.0 : 1.0;  // (-1)^{j shift} at j = 1
+            PhaseSeq ps;
+            if (shifted) ps.init(tid, NT, n);
 #pragma unroll
             for (int t = 0; t < CNT; ++t) {
                 const int k = tid + t * NT;
                 if (k >= n) continue;
+                const double2 ph = shifted ? ps.next() : make_double2(1.0, 0.0);
                 double2 gN = make_double2(0.0, 0.0), gS = make_double2(0.0, 0.0);
                 const bool mirror = k >= 1 && n - k <= L;
                 if (k <= L) {
@@ -1989,13 +2021,15 @@ __global__ __launch_bounds__(ring_threads(MAXM), MAXM > 4096 ? 1 : 4) void k_rin
                     gS.x += sg * (e[t].x - o[t].x);
                     gS.y -= sg * (e[t].y - o[t].y);
                 }
-                fold_store(Z, k, k, n, shifted, oS < 0, gN, gS);
+                fold_store(Z, k, ph, shifted, oS < 0, gN, gS);
             }
         } else if (R == 1) {
+            PhaseSeq ps;
+            if (shifted) ps.init(tid, NT, n);
             for (int k = tid; k < n; k += NT) {
                 double2 gN, gS;
                 fold_aliases(row, L, n, k, 0, 1, shifted, gN, gS);
-                fold_store(Z, k, k, n, shifted, oS < 0, gN, gS);
+                fold_store(Z, k, shifted ? ps.next() : make_double2(1.0, 0.0), shifted, oS < 0, gN, gS);
             }
         } else {
             double2 gN = make_double2(0.0, 0.0), gS = make_double2(0.0, 0.0);
@@ -2014,7 +2048,8 @@ __global__ __launch_bounds__(ring_threads(MAXM), MAXM > 4096 ? 1 : 4) void k_rin
                 }
             }
             __syncthreads();
-            if (tid < n) fold_store(Z, tid, tid, n, shifted, oS < 0, gN, gS);
+            if (tid < n)
+                fold_store(Z, tid, shifted ? cis_pi_frac(tid, n) : make_double2(1.0, 0.0), shifted, oS < 0, gN, gS);
         }
         __syncthreads();
         ring_dft<+1, MAXM>(Z, n, M, logM, pow2, chirp, filt, tw);
@@ -2099,11 +2134,22 @@ __global__ __launch_bounds__(ring_threads(MAXM), MAXM > 4096 ? 1 : 4) void k_rin
 
     // unfold, with the aliases of bin k split over R thread groups as in the fold
     double2* qrow = reinterpret_cast<double2*>(md.Q + (long)p * (L + 1) * 4);
-    for (int t = tid; t < R * n; t += NT) {
-        const int k = t % n, r = t / n;
-        if (k > L) continue;
-        const int nk = (k == 0) ? 0 : n - k;
-        unfold_bin(qrow, L, n, k, r, R, shifted, Z[zp(k)], Z[zp(nk)]);
+    if (R == 1) {
+        PhaseSeq ps;
+        if (shifted) ps.init(tid, NT, n);
+        for (int k = tid; k < n; k += NT) {
+            const double2 ph = shifted ? ps.next() : make_double2(1.0, 0.0);
+            if (k > L) continue;
+            const int nk = (k == 0) ? 0 : n - k;
+            unfold_bin(qrow, L, n, k, 0, 1, shifted, ph, Z[zp(k)], Z[zp(nk)]);
+        }
+    } else if (tid < R * n) {  // R n <= NT: one (bin, group) per thread
+        const int k = tid % n, r = tid / n;
+        if (k <= L) {
+            const int nk = (k == 0) ? 0 : n - k;
+            unfold_bin(qrow, L, n, k, r, R, shifted, shifted ? cis_pi_frac(k, n) : make_double2(1.0, 0.0), Z[zp(k)],
+                       Z[zp(nk)]);
+        }
     }
 }
 

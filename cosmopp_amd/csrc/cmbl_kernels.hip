@@ -1609,8 +1609,11 @@ __device__ __forceinline__ constexpr int brev_c(int q)
 // after it, so the pass works in place.  The twiddle loads (from the table in
 // global memory, mostly L2 hits) are issued first, so their latency runs
 // alongside the LDS loads and the barrier instead of after them.
+constexpr int kLogMaxFFT = 13;
+static_assert((1 << kLogMaxFFT) == kMaxFFT, "twiddle table length");
+
 template <int R, int LR, int SIGN, int MAXM>
-__device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, const double2* __restrict__ tw)
+__device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, int lp, const double2* __restrict__ tw)
 {
     constexpr int NT = ring_threads(MAXM);
     constexpr int B = (MAXM / R + NT - 1) / NT;  // butterflies per thread, at most
@@ -1623,7 +1626,7 @@ __device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, const do
         const int i = tid + b * NT;
         const int k = i & (p - 1);
         w1v[b] = make_double2(1.0, 0.0);
-        if (i < nb && k) w1v[b] = tw_get(tw, k * (kMaxFFT / (p * R)), SIGN);
+        if (i < nb && k) w1v[b] = tw_get(tw, k << (kLogMaxFFT - lp - LR), SIGN);  // k kMaxFFT / (p R)
     }
 #pragma unroll
     for (int b = 0; b < B; ++b) {
@@ -1667,14 +1670,14 @@ __device__ void fft(double2* Z, int M, int logM, const double2* __restrict__ tw)
     // radix 16 where the class runs one workgroup per CU anyway (LDS-bound),
     // radix 8 below, whose halved register set lets two workgroups share a CU
     constexpr int LR = MAXM > 4096 ? 4 : 3;
-    int p = 1, rem = logM;
-    for (; rem >= LR; rem -= LR, p <<= LR) stockham_pass<1 << LR, LR, SIGN, MAXM>(Z, M, p, tw);
+    int p = 1, lp = 0, rem = logM;
+    for (; rem >= LR; rem -= LR, p <<= LR, lp += LR) stockham_pass<1 << LR, LR, SIGN, MAXM>(Z, M, p, lp, tw);
     if (rem == 3)
-        stockham_pass<8, 3, SIGN, MAXM>(Z, M, p, tw);
+        stockham_pass<8, 3, SIGN, MAXM>(Z, M, p, lp, tw);
     else if (rem == 2)
-        stockham_pass<4, 2, SIGN, MAXM>(Z, M, p, tw);
+        stockham_pass<4, 2, SIGN, MAXM>(Z, M, p, lp, tw);
     else if (rem == 1)
-        stockham_pass<2, 1, SIGN, MAXM>(Z, M, p, tw);
+        stockham_pass<2, 1, SIGN, MAXM>(Z, M, p, lp, tw);
 }
 
 // Bluestein DFT of z[0..n) (natural order, zero-padded by this call to M):
@@ -1688,7 +1691,8 @@ __device__ void bluestein(double2* z, int n, int M, int logM, const double2* __r
 {
     auto chirp = gview(chirp_);
     auto filt = gview(filt_);
-    const int tid = threadIdx.x, nt = blockDim.x;
+    constexpr int nt = ring_threads(MAXM);  // the ring kernels' workgroup size
+    const int tid = threadIdx.x;
 #pragma unroll 4
     for (int j = tid; j < M; j += nt) {
         if (j < n) {
@@ -1714,7 +1718,7 @@ __device__ void bluestein(double2* z, int n, int M, int logM, const double2* __r
     }
     __syncthreads();
     fft<+1, MAXM>(z, M, logM, tw);
-    const double inv = 1.0 / (double)M;
+    const double inv = ldexp(1.0, -logM);  // 1 / M, M = 2^logM
 #pragma unroll 4
     for (int k = tid; k < n; k += nt) {
         double2 c = chirp[k];

@@ -324,6 +324,10 @@ enum { kVecCopy = 0, kVecSet, kVecAxpy, kVecMul, kVecDiv, kVecPow };
 int launch_vec(int op, double* y, const double* x, double c, long n, hipStream_t s);
 int launch_dot(const double* x, const double* y, long n, double* d_out, double* d_work, hipStream_t s);
 
+// Device-resident CG state of a Gibbs system (defined with the loops).
+struct GibbsCg;
+void gibbs_cg_free(GibbsCg* cg);
+
 }  // namespace cmbl
 
 // Opaque handles of the C ABI.
@@ -387,6 +391,7 @@ struct cmbl_gibbs {
     double *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_ap = nullptr;  // [n] CG vectors
     double *d_b = nullptr, *d_x = nullptr;                                   // [n] host-call staging
     double *d_m0 = nullptr, *d_m1 = nullptr, *d_m2 = nullptr;               // [npix] maps
+    cmbl::GibbsCg* cg = nullptr;  // device-resident CG, made at the first solve
     std::vector<void*> allocs;
 };
 

@@ -1164,6 +1164,291 @@ int cmbl_hmc_position(cmbl_hmc* hm, double* d_out)
 
 }  // extern "C"
 
+// ====================================================================== Gibbs CG
+// Math::ConjugateGradient::solve (include/conjugate_gradient.hpp:45-102) on
+// a Gibbs system with every vector and scalar on the device: each iteration
+// is the operator, then three fused passes (p . Ap; the x and r updates with
+// r . r, the preconditioner and r . z; the new direction), with alpha, beta
+// and the stopping test formed by the passes' final kernels.  Blocks of
+// kCgBlock iterations are captured in one HIP graph and replayed; the host
+// reads the iteration count and the stop flag after each block.  Once the
+// stop flag is set (|r|^2 <= precision^2, the iteration limit, or a
+// breakdown), the rest of the block leaves x, r, z and p alone, so the
+// result and the count are those of a loop that checks after every
+// iteration.  The passes run the plain dot product's order and the BLAS-1
+// operations of the host-driven loop, so the iterates are the same, bit for
+// bit.
+namespace cmbl {
+
+constexpr int kCgBlock = 8;
+
+struct CgScalars {
+    double rz, alpha, beta, eps, prec2;
+    int done, error, iters, max_iter;
+};
+
+struct CgView {
+    double *x, *r, *z, *p, *ap;
+    const double* dl;
+    CgScalars* st;    // device copy
+    CgScalars* host;  // host-mapped mirror, written after each iteration
+};
+
+__device__ __forceinline__ int packed_l_of(long p)
+{
+    long l = (long)sqrt((double)p);
+    while (l * l > p) --l;
+    while ((l + 1) * (l + 1) <= p) ++l;
+    return (int)l;
+}
+
+// x = 0, r = b - A 0 = b, z = M^-1 r, p = z, rz = r . z; sets the limits.
+struct OpCgStart {
+    static constexpr int kDots = 1;
+    CgView c;
+    long n;
+    const double* b;
+    double prec2;
+    int max_iter;
+    struct In {
+        double b;
+    };
+    __device__ void load() {}
+    __device__ In fetch(long e) const { return In{b[e]}; }
+    __device__ void apply(long e, const In& v, double* a)
+    {
+        const double r = 1.0 * v.b;
+        const double z = r / c.dl[packed_l_of(e)];
+        c.x[e] = 0.0;
+        c.r[e] = r;
+        c.z[e] = z;
+        c.p[e] = 1.0 * z;
+        a[0] = fma(r, z, a[0]);
+    }
+    __device__ void finish(const double* r)
+    {
+        CgScalars& st = *c.st;
+        st.rz = r[0];
+        st.alpha = st.beta = st.eps = 0.0;
+        st.prec2 = prec2;
+        st.done = st.error = st.iters = 0;
+        st.max_iter = max_iter;
+        *c.host = st;
+    }
+};
+
+// p . Ap, alpha = rz / p . Ap (a zero p . Ap stops the solve as a breakdown).
+struct OpCgPap {
+    static constexpr int kDots = 1;
+    CgView c;
+    long n;
+    struct In {
+        double p, ap;
+    };
+    __device__ void load() {}
+    __device__ In fetch(long e) const { return In{c.p[e], c.ap[e]}; }
+    __device__ void apply(long, const In& v, double* a) { a[0] = fma(v.p, v.ap, a[0]); }
+    __device__ void finish(const double* r)
+    {
+        CgScalars& st = *c.st;
+        if (st.done) return;
+        if (r[0] == 0.0) {
+            st.error = 1;
+            st.done = 1;
+            *c.host = st;
+            return;
+        }
+        st.alpha = st.rz / r[0];
+    }
+};
+
+// x += alpha p, r -= alpha Ap, r . r, z = M^-1 r, r . z; then beta and the
+// stopping test.
+struct OpCgUpdate {
+    static constexpr int kDots = 2;
+    CgView c;
+    long n;
+    double alpha;
+    int skip;
+    struct In {
+        double x, p, r, ap;
+    };
+    __device__ void load()
+    {
+        skip = c.st->done;
+        alpha = c.st->alpha;
+    }
+    __device__ In fetch(long e) const
+    {
+        if (skip) return In{0.0, 0.0, 0.0, 0.0};
+        return In{c.x[e], c.p[e], c.r[e], c.ap[e]};
+    }
+    __device__ void apply(long e, const In& v, double* a)
+    {
+        if (skip) return;
+        const double x = fma(alpha, v.p, v.x);
+        const double r = fma(-alpha, v.ap, v.r);
+        a[0] = fma(r, r, a[0]);
+        const double z = r / c.dl[packed_l_of(e)];
+        a[1] = fma(r, z, a[1]);
+        c.x[e] = x;
+        c.r[e] = r;
+        c.z[e] = z;
+    }
+    __device__ void finish(const double* r)
+    {
+        CgScalars& st = *c.st;
+        if (st.done) return;
+        st.eps = r[0];
+        st.iters += 1;
+        if (!isfinite(r[0])) {
+            st.error = 2;
+            st.done = 1;
+        } else if (st.rz == 0.0) {
+            st.error = 3;
+            st.done = 1;
+        } else {
+            st.beta = r[1] / st.rz;
+            st.rz = r[1];
+            if (!(r[0] > st.prec2) || st.iters >= st.max_iter) st.done = 1;
+        }
+        *c.host = st;
+    }
+};
+
+// p = z + beta p (skipped once the solve has stopped: p is not used again).
+struct OpCgDir {
+    static constexpr int kDots = 0;
+    CgView c;
+    long n;
+    double beta;
+    int skip;
+    struct In {
+        double z, p;
+    };
+    __device__ void load()
+    {
+        skip = c.st->done;
+        beta = c.st->beta;
+    }
+    __device__ In fetch(long e) const
+    {
+        if (skip) return In{0.0, 0.0};
+        return In{c.z[e], c.p[e]};
+    }
+    __device__ void apply(long e, const In& v, double*)
+    {
+        if (skip) return;
+        c.p[e] = fma(1.0, v.z, beta * v.p);
+    }
+    __device__ void finish(const double*) {}
+};
+
+struct GibbsCg {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    LoopAllocs mem;
+    PassSync ps{};
+    CgView v{};
+    CgScalars* h = nullptr;  // host view of v.host
+    GraphCache cache;
+    ~GibbsCg()
+    {
+        cache.clear();
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (ev_in) (void)hipEventDestroy(ev_in);
+        if (ev_out) (void)hipEventDestroy(ev_out);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+void gibbs_cg_free(GibbsCg* cg) { delete cg; }
+
+static int gibbs_cg_make(cmbl_gibbs* gb)
+{
+    if (gb->cg) return CMBL_OK;
+    GibbsCg* cg = new GibbsCg();
+    auto bail = [&](int rc) {
+        const std::string msg = cmbl_last_error();
+        delete cg;
+        set_error(msg);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&cg->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&cg->ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&cg->ev_out, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(CMBL_ERR_HIP, "Gibbs CG stream set-up failed"));
+    const size_t n = (size_t)gb->n;
+    int rc;
+    CgView& v = cg->v;
+    if ((rc = cg->mem.dalloc(&v.x, n)) || (rc = cg->mem.dalloc(&v.st, 1)) ||
+        (rc = cg->mem.dalloc(&cg->ps.part, (size_t)kMaxPassDots * kDotParts)) ||
+        (rc = cg->mem.halloc(&cg->h, &v.host, 1)))
+        return bail(rc);
+    v.r = gb->d_r;
+    v.z = gb->d_z;
+    v.p = gb->d_p;
+    v.ap = gb->d_ap;
+    v.dl = gb->d_dl;
+    gb->cg = cg;
+    return CMBL_OK;
+}
+
+static int gibbs_cg_block(cmbl_gibbs* gb, hipStream_t s)
+{
+    const CgView& v = gb->cg->v;
+    const long n = gb->n;
+    for (int i = 0; i < kCgBlock; ++i) {
+        CMBL_TRY(cmbl_gibbs_apply_device(gb, v.p, v.ap, s));
+        CMBL_TRY(run_pass(OpCgPap{v, n}, gb->cg->ps, s));
+        CMBL_TRY(run_pass(OpCgUpdate{v, n, 0.0, 0}, gb->cg->ps, s));
+        CMBL_TRY(run_pass(OpCgDir{v, n, 0.0, 0}, gb->cg->ps, s));
+    }
+    return CMBL_OK;
+}
+
+}  // namespace cmbl
+
+extern "C" {
+
+int cmbl_gibbs_solve_device(cmbl_gibbs* gb, const double* b, double* x, double precision, int max_iter,
+                            int* iterations, void* stream)
+{
+    if (!gb || !gb->mp) return fail(CMBL_ERR_ARG, "gibbs context is null");
+    if (!b || !x) return fail(CMBL_ERR_ARG, "null pointer");
+    if (!(precision > 0.0)) return fail(CMBL_ERR_ARG, "precision must be > 0");
+    if (max_iter < 1) return fail(CMBL_ERR_ARG, "max_iter must be >= 1");
+    if (iterations) *iterations = 0;
+    CMBL_TRY(gibbs_cg_make(gb));
+    GibbsCg* cg = gb->cg;
+    hipStream_t caller = (hipStream_t)stream;
+    hipStream_t s = cg->stream;
+    CMBL_HIP(hipEventRecord(cg->ev_in, caller));
+    CMBL_HIP(hipStreamWaitEvent(s, cg->ev_in, 0));
+    OpCgStart st0{cg->v, gb->n, b, precision * precision, max_iter};
+    CMBL_TRY(run_pass(st0, cg->ps, s));
+    for (;;) {
+        CMBL_TRY(cg->cache.run(true, kCgBlock, s, [&](hipStream_t q) { return gibbs_cg_block(gb, q); }));
+        CMBL_HIP(hipStreamSynchronize(s));
+        if (__atomic_load_n(&cg->h->done, __ATOMIC_ACQUIRE)) break;
+    }
+    const CgScalars r = *cg->h;
+    CMBL_HIP(hipMemcpyAsync(x, cg->v.x, sizeof(double) * gb->n, hipMemcpyDeviceToDevice, s));
+    CMBL_HIP(hipEventRecord(cg->ev_out, s));
+    CMBL_HIP(hipStreamWaitEvent(caller, cg->ev_out, 0));
+    CMBL_HIP(hipStreamSynchronize(s));
+    if (iterations) *iterations = r.iters;
+    if (r.error == 1) return fail(CMBL_ERR_STATE, "CG breakdown: p.Ap = 0");
+    if (r.error == 2) return fail(CMBL_ERR_NONFINITE, "CG residual is not finite");
+    if (r.error == 3) return fail(CMBL_ERR_STATE, "CG breakdown: r.z = 0");
+    if (r.eps > r.prec2)
+        return fail(CMBL_ERR_NOCONV,
+                    "CG reached " + std::to_string(max_iter) + " iterations, |r|^2 = " + std::to_string(r.eps));
+    return CMBL_OK;
+}
+
+}  // extern "C"
+
 // ====================================================================== line search
 struct cmbl_line_search {
     cmbl::MoreThuente mt;

@@ -1134,6 +1134,8 @@ void cmbl_gibbs_destroy(cmbl_gibbs* gb)
 {
     if (!gb) return;
     (void)hipDeviceSynchronize();
+    gibbs_cg_free(gb->cg);
+    gb->cg = nullptr;
     for (void* p : gb->allocs) (void)hipFree(p);
     cmbl_map_destroy(gb->mp);
     delete gb;
@@ -1266,52 +1268,6 @@ int cmbl_gibbs_rhs_device(cmbl_gibbs* gb, const double* d, const double* omega0,
     CMBL_TRY(launch_adj(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, s));
     CMBL_TRY(launch_post(mp->desc, mp->d_desc, nullptr, b, kPostAddAlm, s,
                          std::sqrt(4.0 * 3.141592653589793 / (double)gb->npix)));
-    return CMBL_OK;
-}
-
-int cmbl_gibbs_solve_device(cmbl_gibbs* gb, const double* b, double* x, double precision, int max_iter,
-                            int* iterations, void* stream)
-{
-    CMBL_TRY(check_gibbs(gb));
-    if (!b || !x) return fail(CMBL_ERR_ARG, "null pointer");
-    if (!(precision > 0.0)) return fail(CMBL_ERR_ARG, "precision must be > 0");
-    if (max_iter < 1) return fail(CMBL_ERR_ARG, "max_iter must be >= 1");
-    hipStream_t s = (hipStream_t)stream;
-    const long n = gb->n;
-    double *r = gb->d_r, *z = gb->d_z, *p = gb->d_p, *ap = gb->d_ap;
-    if (iterations) *iterations = 0;
-    // x = 0, r = b - A x, z = M^-1 r, p = z (include/conjugate_gradient.hpp:51-58)
-    CMBL_TRY(cmbl_vec_set(x, 0.0, n, s));
-    CMBL_TRY(cmbl_gibbs_apply_device(gb, x, r, s));
-    CMBL_TRY(cmbl_vec_copy(r, r, -1.0, n, s));
-    CMBL_TRY(cmbl_vec_axpy(r, b, 1.0, n, s));
-    CMBL_TRY(cmbl_gibbs_precondition_device(gb, r, z, s));
-    CMBL_TRY(cmbl_vec_copy(p, z, 1.0, n, s));
-    double eps = 0.0;
-    int it = 0;
-    do {  // :64-97
-        double rz = 0.0, pap = 0.0, rz_new = 0.0;
-        CMBL_TRY(cmbl_vec_dot(r, z, n, &rz, s));
-        CMBL_TRY(cmbl_gibbs_apply_device(gb, p, ap, s));
-        CMBL_TRY(cmbl_vec_dot(p, ap, n, &pap, s));
-        if (pap == 0.0) return fail(CMBL_ERR_STATE, "CG breakdown: p.Ap = 0");
-        const double alpha = rz / pap;
-        CMBL_TRY(cmbl_vec_axpy(x, p, alpha, n, s));
-        CMBL_TRY(cmbl_vec_axpy(r, ap, -alpha, n, s));
-        CMBL_TRY(cmbl_vec_dot(r, r, n, &eps, s));
-        CMBL_TRY(cmbl_gibbs_precondition_device(gb, r, z, s));
-        CMBL_TRY(cmbl_vec_dot(r, z, n, &rz_new, s));
-        if (rz == 0.0) return fail(CMBL_ERR_STATE, "CG breakdown: r.z = 0");
-        const double beta = rz_new / rz;
-        CMBL_TRY(cmbl_vec_copy(p, p, beta, n, s));
-        CMBL_TRY(cmbl_vec_axpy(p, z, 1.0, n, s));
-        ++it;
-        if (iterations) *iterations = it;
-        if (!std::isfinite(eps)) return fail(CMBL_ERR_NONFINITE, "CG residual is not finite");
-    } while (eps > precision * precision && it < max_iter);
-    if (eps > precision * precision)
-        return fail(CMBL_ERR_NOCONV, "CG reached " + std::to_string(max_iter) + " iterations, |r|^2 = " +
-                                         std::to_string(eps));
     return CMBL_OK;
 }
 

@@ -267,10 +267,81 @@ def bench_loops(args):
                   "evaluations": evs, "accepted": st["accepted"] - st0["accepted"], "seconds": round(t_hmc, 4),
                   "ms_per_leapfrog_step": round(t_hmc / 500 * 1e3, 4),
                   "overhead_vs_evaluations": round(t_hmc / (evs * eval_s) - 1, 4),
-                  "momenta": "reference draws (whitenoise seeds) on host threads, 8 trajectories ahead"}
+                  "momenta": "reference draws (whitenoise seeds) on 12 host threads, up to 12 trajectories ahead"}
     hm.close()
     log("bench: hmc 500 steps %.3f s" % t_hmc)
     mp.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_gibbs(args):
+    """CMBGibbsSampler steps on the GPU at the reference sampler test's sizes
+    (source/test_cmb_gibbs.cpp:64-99: nside 16 / Gibbs lmax 30 with a uniform
+    mask, nside 32 / lmax 50 with the reference's mask1.fits), and the Gibbs
+    CG at nside 1024 / lmax 2048 (ms per iteration).  The oracle's sampler is
+    timed beside the first two on the host (a bounded number of steps)."""
+    import numpy as np
+
+    import cosmopp_amd as cm
+    from cases import gibbs_case
+
+    def synth(nside, L, a, beam):
+        mp = cm.Map(nside, L)
+        mp.set_beam(beam)
+        m = mp.forward(a)
+        mp.close()
+        return m
+
+    m, ns, nested = cm.read_map(os.path.join(ROOT, "tests", "golden", "mask1.fits"))
+    ring = cm.reorder(m, ns, to_ring=True) if nested else m
+    out = {"metric": "Gibbs sampler steps/s (CMBGibbsSampler::step) and Gibbs CG ms/iteration, FP64",
+           "n_gpus": 1, "dtype": "f64", "data": "synthetic sky, white noise; mask1.fits from the reference's tests"}
+    for i, steps, cpu_steps in ((0, 40, 20), (4, 6, 1)):
+        c = gibbs_case(i, ring_mask=ring if i == 4 else None, synth=synth)
+        log("bench: gibbs subtest %d nside=%d lmax=%d" % (i, c["nside"], c["lmax"]))
+        gs = cm.GibbsSampler(c["map"], c["noise"], c["mask"], c["sigma"], c["lmax"], c["fwhm"], c["cl0"],
+                             seed=c["seed"])
+        gs.step()
+        t0 = time.perf_counter()
+        its = 0
+        for _ in range(steps):
+            gs.step()
+            its += gs.iterations
+        dt = (time.perf_counter() - t0) / steps
+        gs.close()
+        rec = {"nside": c["nside"], "lmax": c["lmax"], "mask": "mask1.fits" if i == 4 else "uniform",
+               "steps_per_s": round(1.0 / dt, 3), "ms_per_step": round(dt * 1e3, 3),
+               "cg_iterations_per_step": its / steps, "ms_per_cg_iteration": round(dt * 1e3 / (its / steps), 4)}
+        if not args.no_cpu_baseline:
+            import oracle
+
+            og = oracle.GibbsSampler(c["nside"], c["lmax"], c["map"], c["noise"], c["mask"], c["sigma"], c["beam"],
+                                     c["cl0"], c["seed"])
+            t0 = time.perf_counter()
+            for _ in range(cpu_steps):
+                og.step()
+            cdt = (time.perf_counter() - t0) / cpu_steps
+            rec["cpu_baseline"] = {"steps_per_s": round(1.0 / cdt, 4), "kind": "port", "cores": oracle.num_threads(),
+                                   "sample": "%d step(s) of the oracle's CMBGibbsSampler restatement" % cpu_steps}
+        out["sampler_subtest%d" % i] = rec
+    # Gibbs CG at the headline geometry: 300 iterations
+    nside, lmax, sigma = 1024, 2048, 20.0
+    log("bench: gibbs CG nside=%d lmax=%d" % (nside, lmax))
+    g = cm.Gibbs(nside, lmax, cm.cl_shape(lmax), cm.beam(lmax, 10.0 / 60.0),
+                 cm.mask_latitude(nside, math.radians(20.0)), sigma)
+    rng = np.random.default_rng(3)
+    npix = 12 * nside * nside
+    b = g.rhs(rng.standard_normal(npix) * 50, rng.standard_normal(npix), rng.standard_normal(npix))
+    for its in (16, 300):
+        t0 = time.perf_counter()
+        try:
+            g.solve(b, precision=1e-30, max_iter=its)  # runs the full count
+        except cm.CmblError as e:
+            if e.code != cm.CMBL_ERR_NOCONV:
+                raise
+        dt = time.perf_counter() - t0
+    g.close()
+    out["gibbs_cg_n1024_l2048"] = {"iterations": 300, "ms_per_iteration": round(dt * 1e3 / 300, 4)}
     print(json.dumps(out), flush=True)
 
 
@@ -371,6 +442,8 @@ def main():
                     help="only time the CPU oracle at configs 1, 2, 4 on one and on all threads")
     ap.add_argument("--loops", action="store_true",
                     help="time the device-resident L-BFGS (200 iterations) and HMC (500 leapfrog steps) at config 4")
+    ap.add_argument("--gibbs", action="store_true",
+                    help="time Gibbs sampler steps (nside 16 and 32, the reference test's sizes) and the Gibbs CG")
     ap.add_argument("--host-buffers", action="store_true",
                     help="time cmbl_value_and_grad with x and g in host memory (PCIe-inclusive rate)")
     ap.add_argument("--cpu-budget", type=float, default=40.0, help="seconds for the CPU baseline leg")
@@ -411,6 +484,9 @@ def main():
         return
     if args.loops:
         bench_loops(args)
+        return
+    if args.gibbs:
+        bench_gibbs(args)
         return
     cfg = CONFIGS[args.config]
     if args.pol:

@@ -151,19 +151,22 @@ GIBBS_SUBTESTS = {
 }
 
 
-def gibbs_case(i, seed=21, ring_mask=None):
+def gibbs_case(i, seed=21, ring_mask=None, synth=None):
     """Inputs of Gibbs subtest i: dict(nside, lmax, map, noise, mask, sigma,
     fwhm, beam, cl0, cl_original, seed, scale).  ring_mask supplies mask1 in
-    RING order for subtest 4."""
-    import oracle
+    RING order for subtest 4; synth(nside, L, a, beam) makes the sky map
+    (default: the oracle's synthesis)."""
+    if synth is None:
+        import oracle
 
+        synth = oracle.synthesis
     c = GIBBS_SUBTESTS[i]
     nside, lsim, lmax = c["nside"], c["lsim"], c["lmax"]
     cl_true = np.zeros(lsim + 1)
     l = np.arange(2, lsim + 1)
     cl_true[2:] = 2 * np.pi * 1000.0 / (l * (l + 1))
     a = cm.simulate_alm(cl_true, lsim, seed)
-    m = oracle.synthesis(nside, lsim, a, cm.beam(lsim, c["fwhm"]))
+    m = synth(nside, lsim, a, cm.beam(lsim, c["fwhm"]))
     noise = cm.simulate_white_noise(nside, c["sigma"], seed + 1)
     mask = np.ones(12 * nside * nside) if not c["mask"] else np.asarray(ring_mask, dtype=np.float64)
     cl0 = np.full(lmax + 1, 1000.0)                  # source/test_cmb_gibbs.cpp:143-146

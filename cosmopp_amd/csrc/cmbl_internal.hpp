@@ -220,6 +220,8 @@ struct WorkItem {
     int mcnt;
     int rb;
     int lfirst;
+    int lend;  // last l of the item (the adjoint's segments end before L)
+    int sidx;  // start-state slot of an adjoint segment (-1: indexed by m)
     int pad_;
 };
 

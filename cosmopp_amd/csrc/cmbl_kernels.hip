@@ -1358,7 +1358,7 @@ __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __re
     if (it.mcnt == 0) return;
     const MapDesc& md = maps[it.map];
     const int L = md.L;
-    const int npad = md.npair_pad;
+    const int lend = it.lend;  // last l of this segment
     const int lane = threadIdx.x;
     const int m = it.m0;
     const int l0 = it.lfirst;
@@ -1377,7 +1377,7 @@ __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __re
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int p = src.pbase + k * 64 + lane;
-        const long q = (long)m * npad + p;
+        const long q = (long)it.sidx * 64 * K + k * 64 + lane;  // the segment's start state
         st.x[k] = md.ring_x[p];
         st.mu1[k] = md.s5_cur[q];
         st.mu2[k] = md.s5_prev[q];
@@ -1410,22 +1410,22 @@ __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __re
     // chunk ends with one unpipelined row reduction, so longer chunks drain
     // the pipeline less often)
     int cb = l0 + 1;
-    double tA0 = (cb + lane <= L) ? A[cb + lane] : 0.0;
-    double tA1 = (cb + 64 + lane <= L) ? A[cb + 64 + lane] : 0.0;
+    double tA0 = (cb + lane <= lend) ? A[cb + lane] : 0.0;
+    double tA1 = (cb + 64 + lane <= lend) ? A[cb + 64 + lane] : 0.0;
     sA[0][lane] = tA0;
     sA[0][64 + lane] = tA1;
     wave_sync();
     int buf = 0;
     int l = cb;
     // two loops so that the scaled-path state is dead in the second
-    while (cb <= L && !st.all) {
+    while (cb <= lend && !st.all) {
         const int nb = cb + kAdjChunk;
-        if (nb + lane <= L) tA0 = A[nb + lane];
-        if (nb + 64 + lane <= L) tA1 = A[nb + 64 + lane];
-        adj_chunk_scaled<K>(st, in, rd, l, min(L, nb - 1), cb, m, sA[buf], src);
+        if (nb + lane <= lend) tA0 = A[nb + lane];
+        if (nb + 64 + lane <= lend) tA1 = A[nb + 64 + lane];
+        adj_chunk_scaled<K>(st, in, rd, l, min(lend, nb - 1), cb, m, sA[buf], src);
         // every lane turned live inside the chunk: its rest runs as live rows
-        if (st.all && l <= min(L, nb - 1)) adj_chunk_live<K>(st, in, rd, l, min(L, nb - 1), cb, m, sA[buf]);
-        if (nb <= L) {
+        if (st.all && l <= min(lend, nb - 1)) adj_chunk_live<K>(st, in, rd, l, min(lend, nb - 1), cb, m, sA[buf]);
+        if (nb <= lend) {
             sA[buf ^ 1][lane] = tA0;
             sA[buf ^ 1][64 + lane] = tA1;
         }
@@ -1433,12 +1433,12 @@ __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __re
         buf ^= 1;
         cb = nb;
     }
-    while (cb <= L) {
+    while (cb <= lend) {
         const int nb = cb + kAdjChunk;
-        if (nb + lane <= L) tA0 = A[nb + lane];
-        if (nb + 64 + lane <= L) tA1 = A[nb + 64 + lane];
-        adj_chunk_live<K>(st, in, rd, l, min(L, nb - 1), cb, m, sA[buf]);
-        if (nb <= L) {
+        if (nb + lane <= lend) tA0 = A[nb + lane];
+        if (nb + 64 + lane <= lend) tA1 = A[nb + 64 + lane];
+        adj_chunk_live<K>(st, in, rd, l, min(lend, nb - 1), cb, m, sA[buf]);
+        if (nb <= lend) {
             sA[buf ^ 1][lane] = tA0;
             sA[buf ^ 1][64 + lane] = tA1;
         }
@@ -1492,7 +1492,8 @@ __global__ __launch_bounds__(64) void k_scan(const WorkItem* __restrict__ items,
     }
 }
 
-// k_capture: the state (mu_{l-1}, mu_l, scale count) at l = item.lfirst.
+// k_capture: the state (mu_{l-1}, mu_l, scale count) at l = item.lfirst, in
+// slot sidx (adjoint segments) or at (m, pair).
 template <int K>
 __global__ __launch_bounds__(64) void k_capture(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps,
                                                 double* __restrict__ prev, double* __restrict__ cur,
@@ -1522,7 +1523,7 @@ __global__ __launch_bounds__(64) void k_capture(const WorkItem* __restrict__ ite
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const long q = (long)m * npad + p[k];
+        const long q = it.sidx >= 0 ? (long)it.sidx * 64 * K + k * 64 + lane : (long)m * npad + p[k];
         prev[q] = st.mu2[k];
         cur[q] = st.mu1[k];
         kout[q] = st.sc[k];

@@ -157,9 +157,41 @@ static void build_items(const cmbl_map* mp, int K, const std::vector<int>& lstar
             }
             if (lmin > L) continue;
             const double cost = K * (3.0 * (lmax - lmin) + 4.0 * (L - lmax + 1)) + 64.0 * K;
-            out->push_back(ItemCost{WorkItem{map_index, m, 1, rb, lmin, 0}, cost});
+            out->push_back(ItemCost{WorkItem{map_index, m, 1, rb, lmin, L, -1, 0}, cost});
         }
     }
+}
+
+// Adjoint items cut along l into segments of at most S steps, each starting
+// from its own captured recurrence state (slot sidx): the adjoint's outputs
+// are per l, so segments of one (m, ring block) write disjoint parts of its
+// row, and the long low-m items no longer set the launch's tail.
+static int adj_segment_len(int L) { return std::max(256, ((L / 4 + 127) / 128) * 128); }
+// Segments only where the adjoint has few items: below this count a launch
+// is a few waves per SIMD (three fit at once) and its longest items set its
+// length; at nside 512 / lmax 1024 (4,100 items) segments took the adjoint
+// from 0.153 to 0.122 ms, at nside 1024 / lmax 2048 (16,000) they changed
+// nothing and at nside 2048 (65,000) cost up to 4%.
+static const size_t kSegmentBelow = 12288;
+
+static void segment_items(std::vector<ItemCost>* v, int L, int S, int K, std::vector<ItemCost>* whole)
+{
+    std::vector<ItemCost> out;
+    whole->clear();
+    for (const ItemCost& c : *v) {
+        whole->push_back(c);
+        whole->back().it.sidx = (int)out.size();  // the first segment's start state is the item's
+        const double len = (double)(L - c.it.lfirst + 1);
+        for (int l0 = c.it.lfirst; l0 <= L; l0 += S) {
+            ItemCost sg = c;
+            sg.it.lfirst = l0;
+            sg.it.lend = std::min(L, l0 + S - 1);
+            sg.it.sidx = (int)out.size();
+            sg.cost = (c.cost - 64.0 * K) * (double)(sg.it.lend - l0 + 1) / len + 64.0 * K;
+            out.push_back(sg);
+        }
+    }
+    *v = out;
 }
 
 // XCD-aware order.  Workgroup i runs on XCD i % 8, so the list is built as
@@ -205,7 +237,7 @@ static std::vector<WorkItem> order_items(std::vector<ItemCost>& v)
         std::stable_sort(l.begin(), l.end(), [](const ItemCost& a, const ItemCost& b) { return a.cost > b.cost; });
         len = std::max(len, l.size());
     }
-    out.resize(len * kXcds, WorkItem{0, 0, 0, 0, 0, 0});
+    out.resize(len * kXcds, WorkItem{0, 0, 0, 0, 0, 0, -1, 0});
     for (int x = 0; x < kXcds; ++x)
         for (size_t j = 0; j < lists[x].size(); ++j) out[j * kXcds + x] = lists[x][j].it;
     return out;
@@ -560,7 +592,7 @@ static int map_create(cmbl_map* mp, int nside, int lmax, const cmbl_tuning& tune
     // ---- scan: first live l per (m, pair); work lists; start states
     std::vector<WorkItem> all;
     for (int rb = 0; rb < mp->nrb1; ++rb)
-        for (int m = 0; m <= L; m += 4) all.push_back(WorkItem{0, m, std::min(4, L + 1 - m), rb, m, 0});
+        for (int m = 0; m <= L; m += 4) all.push_back(WorkItem{0, m, std::min(4, L + 1 - m), rb, m, L, -1, 0});
     WorkItem* d_all;
     int* d_ls;
     CMBL_TRY(dupload(mp, &d_all, all));
@@ -575,8 +607,15 @@ static int map_create(cmbl_map* mp, int nside, int lmax, const cmbl_tuning& tune
     std::vector<ItemCost> c1, c5;
     build_items(mp, mp->K1, lstart, 0, &c1);
     build_items(mp, mp->K5, lstart, 0, &c5);
+    std::vector<ItemCost> c5whole;  // batches take whole items: they have items enough
+    if (c5.size() < kSegmentBelow) {
+        segment_items(&c5, L, adj_segment_len(L), mp->K5, &c5whole);
+    } else {
+        for (size_t i = 0; i < c5.size(); ++i) c5[i].it.sidx = (int)i;  // whole items, one slot each
+        c5whole = c5;
+    }
     mp->cost1 = c1;
-    mp->cost5 = c5;
+    mp->cost5 = c5whole;
     mp->items1 = order_items(c1);
     mp->items5 = order_items(c5);
     mp->nitems1 = (int)mp->items1.size();
@@ -585,14 +624,15 @@ static int map_create(cmbl_map* mp, int nside, int lmax, const cmbl_tuning& tune
     CMBL_TRY(dupload(mp, &mp->d_items5, mp->items5));
     {
         const size_t nst = (size_t)(L + 1) * npad;
+        const size_t nst5 = std::max<size_t>(c5.size(), 1) * 64 * mp->K5;  // one slot per segment
         double *p1, *c1p, *p5, *c5p;
         int *k1, *k5;
         CMBL_TRY(dalloc(mp, &p1, nst));
         CMBL_TRY(dalloc(mp, &c1p, nst));
         CMBL_TRY(dalloc(mp, &k1, nst));
-        CMBL_TRY(dalloc(mp, &p5, nst));
-        CMBL_TRY(dalloc(mp, &c5p, nst));
-        CMBL_TRY(dalloc(mp, &k5, nst));
+        CMBL_TRY(dalloc(mp, &p5, nst5));
+        CMBL_TRY(dalloc(mp, &c5p, nst5));
+        CMBL_TRY(dalloc(mp, &k5, nst5));
         CMBL_TRY(launch_capture(mp->K1, mp->d_items1, mp->nitems1, mp->d_desc, p1, c1p, k1, nullptr));
         CMBL_TRY(launch_capture(mp->K5, mp->d_items5, mp->nitems5, mp->d_desc, p5, c5p, k5, nullptr));
         d.s1_prev = p1;

@@ -258,8 +258,8 @@ __global__ __launch_bounds__(256) void k_post_b(const MapDesc* __restrict__ mds,
 
 // y_p = x_p / d_l over the packed layout (l = floor(sqrt(p))): the Gibbs
 // preconditioner (source/cmb_gibbs.cpp:407-423 divides every a_lm by d_l).
-__global__ __launch_bounds__(256) void k_ldiv(double* __restrict__ y, const double* __restrict__ x,
-                                              const double* __restrict__ dl, int L)
+// y may be x (in place).
+__global__ __launch_bounds__(256) void k_ldiv(double* y, const double* x, const double* __restrict__ dl, int L)
 {
     const long n = (long)(L + 1) * (L + 1);
     for (long p = blockIdx.x * 256L + threadIdx.x; p < n; p += (long)gridDim.x * 256L) {

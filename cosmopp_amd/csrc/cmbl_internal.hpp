@@ -279,6 +279,9 @@ int hip_fail(hipError_t e, const char* what);
 // (defined in cmbl_kernels.hip)
 int launch_pre(const MapDesc& md, const MapDesc* d_md, const double* d_x, hipStream_t s);
 int launch_synth(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, hipStream_t s);
+// Shared-recurrence synthesis of groups of G maps of one geometry (G = 2, 4).
+int launch_synth_group(int K, int G, const WorkItem* items, int nitems, const MapDesc* d_maps, const int* d_groups,
+                       hipStream_t s);
 int launch_adj(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, hipStream_t s);
 int lm_tiles(int L);  // blocks of k_pre / k_post (prior partial sums)
 int launch_ldiv(double* y, const double* x, const double* dl, int L, hipStream_t s);
@@ -404,6 +407,15 @@ struct cmbl_batch {
     cmbl::WorkItem* d_items1[4] = {nullptr, nullptr, nullptr, nullptr};
     cmbl::WorkItem* d_items5[4] = {nullptr, nullptr, nullptr, nullptr};
     int n1[4] = {0, 0, 0, 0}, n5[4] = {0, 0, 0, 0};
+    // shared-recurrence synthesis: groups of G maps of one geometry, their
+    // items by (K, G) (index log2 K, G = 2 or 4) and the group table
+    struct SynGroupList {
+        int K = 0, G = 0, n = 0;
+        cmbl::WorkItem* d_items = nullptr;
+    };
+    std::vector<SynGroupList> syn_groups;
+    int* d_gmaps = nullptr;
+    int ngroups = 0;
     cmbl::RingItem* d_ring_items = nullptr;
     int nring_items = 0;
     int lds_ring = 0;

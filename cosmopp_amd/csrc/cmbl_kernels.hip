@@ -750,6 +750,266 @@ __global__ __launch_bounds__(64) void k_synth(const WorkItem* __restrict__ items
     }
 }
 
+// -------------------------------------------------------------- shared-recurrence synthesis
+// G maps of one geometry (same nside, lmax and ring pairs per lane, so the
+// same ring tables, recurrence coefficients, work items and start states)
+// in one wave: the recurrence runs once per (l, ring pair) and feeds the G
+// maps' parity sums, 2K + 2KG FP64 instructions per l-step against 4KG for
+// the maps one by one.  Item.map indexes the group table (rows of four map
+// indices, the first G used);
+// the first map supplies the geometry.  Each map's sums are formed by the
+// same operations in the same order as in k_synth, so its P is the same.
+// Compile-time loops over the maps of a group: every index is a constant
+// when the accumulators are laid out, so they stay in registers (a runtime
+// loop, even one that is unrolled later, left them in scratch).
+template <class F, int... I>
+__device__ __forceinline__ void for_each_map(std::integer_sequence<int, I...>, F&& f)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+
+template <int K, int G, bool ODD>
+__device__ __forceinline__ void syng_step(LegState<K>& st, SynAcc<K> (&ac)[G], double Al, const double2 (&av)[G])
+{
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double nu = fma(Al * st.x[k], st.mu1[k], -st.mu2[k]);
+        st.mu2[k] = st.mu1[k];
+        st.mu1[k] = nu;
+        for_each_map(std::make_integer_sequence<int, G>{}, [&](auto gi) {
+            constexpr int g = decltype(gi)::value;
+            if (ODD) {
+                ac[g].orr[k] = fma(nu, av[g].x, ac[g].orr[k]);
+                ac[g].oi[k] = fma(nu, av[g].y, ac[g].oi[k]);
+            } else {
+                ac[g].er[k] = fma(nu, av[g].x, ac[g].er[k]);
+                ac[g].ei[k] = fma(nu, av[g].y, ac[g].ei[k]);
+            }
+        });
+    }
+}
+
+template <int K, int G, bool ODD>
+__device__ __forceinline__ void syng_stepj(LegState<K>& st, SynAcc<K> (&ac)[G], int j, const double* cA,
+                                           const double2 (*ca)[kSynChunk / 2 + kChunkPad])
+{
+    double2 av[G];
+    for_each_map(std::make_integer_sequence<int, G>{}, [&](auto gi) {
+        constexpr int g = decltype(gi)::value;
+        av[g] = ca[g][j];
+    });
+    syng_step<K, G, ODD>(st, ac, cA[j], av);
+}
+
+// Rescale check: as syn_rescale, a joining lane clears every map's sums.
+template <int K, int G>
+__device__ __forceinline__ void syng_rescale(LegState<K>& st, SynAcc<K> (&ac)[G])
+{
+    int need = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) need |= (st.sc[k] > 0) & (fabs(st.mu1[k]) > kScaleHi);
+    if (!__any(need)) return;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const bool r = st.sc[k] > 0 && fabs(st.mu1[k]) > kScaleHi;
+        st.mu1[k] = r ? st.mu1[k] * kScaleDn : st.mu1[k];
+        st.mu2[k] = r ? st.mu2[k] * kScaleDn : st.mu2[k];
+        st.sc[k] -= r ? 1 : 0;
+        const bool join = r && st.sc[k] == 0;
+        for_each_map(std::make_integer_sequence<int, G>{}, [&](auto gi) {
+            constexpr int g = decltype(gi)::value;
+            ac[g].er[k] = join ? 0.0 : ac[g].er[k];
+            ac[g].ei[k] = join ? 0.0 : ac[g].ei[k];
+            ac[g].orr[k] = join ? 0.0 : ac[g].orr[k];
+            ac[g].oi[k] = join ? 0.0 : ac[g].oi[k];
+        });
+    }
+    st.all = all_live<K>(st.sc);
+}
+
+// One step of either parity, the parity by value (both parities' sums take an
+// fma, the other one's with zero coefficients: fma(nu, 0, s) = s for s != -0).
+template <int K, int G>
+__device__ __forceinline__ void syng_step_any(LegState<K>& st, SynAcc<K> (&ac)[G], bool odd, int j, const double* cA,
+                                              const double2 (*ca)[kSynChunk / 2 + kChunkPad])
+{
+    const double Al = cA[j];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double nu = fma(Al * st.x[k], st.mu1[k], -st.mu2[k]);
+        st.mu2[k] = st.mu1[k];
+        st.mu1[k] = nu;
+    }
+    for_each_map(std::make_integer_sequence<int, G>{}, [&](auto gi) {
+        constexpr int g = decltype(gi)::value;
+        const double2 a = ca[g][j];
+        const double er = odd ? 0.0 : a.x, ei = odd ? 0.0 : a.y, orr = odd ? a.x : 0.0, oi = odd ? a.y : 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            ac[g].er[k] = fma(st.mu1[k], er, ac[g].er[k]);
+            ac[g].ei[k] = fma(st.mu1[k], ei, ac[g].ei[k]);
+            ac[g].orr[k] = fma(st.mu1[k], orr, ac[g].orr[k]);
+            ac[g].oi[k] = fma(st.mu1[k], oi, ac[g].oi[k]);
+        }
+    });
+}
+
+// Steps l..cend of one staged chunk.  Every run of steps starts at an even
+// l - m (an odd first step is taken on its own, parity by value), so all
+// paired steps are (even, odd) in one code path: two paths that differ only
+// in which parity's sums a step updates get merged by the compiler into a
+// choice of sums by address, which keeps them in scratch.  While some lane
+// is scaled: rows of eight steps with a rescale check after each, and pairs
+// with a check after each near the chunk's end (a lane joining inside a row
+// drops terms below 2^-55 of O(1), as in k_synth); then live pairs.
+template <int K, int G>
+__device__ __forceinline__ void syng_chunk(LegState<K>& st, SynAcc<K> (&ac)[G], int& l, int cend, int cb, int m,
+                                           const double* cA, const double2 (*ca)[kSynChunk / 2 + kChunkPad])
+{
+    while (l <= cend) {
+        if ((l - m) & 1) {
+            syng_step_any<K, G>(st, ac, true, l - cb, cA, ca);
+            ++l;
+            if (!st.all) syng_rescale<K, G>(st, ac);
+            continue;
+        }
+        if (!st.all) {
+            const int n = (l + 7 <= cend) ? 8 : 2;
+#pragma unroll 1
+            for (int q = 0; q < n && l + 1 <= cend; q += 2, l += 2) {
+                syng_stepj<K, G, false>(st, ac, l - cb, cA, ca);
+                syng_stepj<K, G, true>(st, ac, l + 1 - cb, cA, ca);
+            }
+            if (l == cend && n == 2) {
+                syng_stepj<K, G, false>(st, ac, l - cb, cA, ca);
+                ++l;
+            }
+            syng_rescale<K, G>(st, ac);
+            continue;
+        }
+#pragma unroll 1
+        for (; l + 1 <= cend; l += 2) {
+            syng_stepj<K, G, false>(st, ac, l - cb, cA, ca);
+            syng_stepj<K, G, true>(st, ac, l + 1 - cb, cA, ca);
+        }
+        if (l <= cend) {
+            syng_stepj<K, G, false>(st, ac, l - cb, cA, ca);
+            ++l;
+        }
+    }
+}
+
+template <int K, int G>
+__global__ __launch_bounds__(64) void k_synth_g(const WorkItem* __restrict__ items, const MapDesc* __restrict__ maps,
+                                                const int* __restrict__ gmaps)
+{
+    constexpr int CH = kSynChunk / 2;  // 64 steps: the G coefficient rows share the LDS budget
+    __shared__ double sA[2][CH + kChunkPad];
+    __shared__ double2 sa[2][G][CH + kChunkPad];
+    const WorkItem it = items[blockIdx.x];
+    if (it.mcnt == 0) return;
+    const int* gm = gmaps + (long)it.map * 4;  // group table rows of four entries
+    const MapDesc& md = maps[gm[0]];
+    const int L = md.L;
+    const int npad = md.npair_pad;
+    const int lane = threadIdx.x;
+    const int m = it.m0;
+    const int l0 = it.lfirst;
+    const long base = moff(L, m) - m;
+    gcdouble* __restrict__ A = (gcdouble*)(md.A + base);
+    gcdouble* a[G];
+    for_each_map(std::make_integer_sequence<int, G>{}, [&](auto gi) {
+        constexpr int g = decltype(gi)::value;
+        a[g] = (gcdouble*)(maps[gm[g]].alm + base);
+    });
+
+    LegState<K> st;
+    SynAcc<K> ac[G];
+    int p[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        p[k] = it.rb * 64 * K + k * 64 + lane;
+        const long q = (long)m * npad + p[k];
+        st.x[k] = md.ring_x[p[k]];
+        st.mu1[k] = md.s1_cur[q];
+        st.mu2[k] = md.s1_prev[q];
+        st.sc[k] = md.s1_k[q];
+        for_each_map(std::make_integer_sequence<int, G>{}, [&](auto gi) {
+            constexpr int g = decltype(gi)::value;
+            ac[g].er[k] = ac[g].ei[k] = ac[g].orr[k] = ac[g].oi[k] = 0.0;
+        });
+    }
+    st.all = all_live<K>(st.sc);
+    // the item's first l, its parity by value (a branch here would be merged
+    // into a choice of accumulator by address, which keeps the sums in
+    // scratch); fma(mu, 0, +0) = +0 leaves the other parity's sums as they are
+    const bool odd0 = ((l0 - m) & 1) != 0;
+    for_each_map(std::make_integer_sequence<int, G>{}, [&](auto gi) {
+        constexpr int g = decltype(gi)::value;
+        const double ar = a[g][2 * l0], ai = a[g][2 * l0 + 1];
+        const double er = odd0 ? 0.0 : ar, ei = odd0 ? 0.0 : ai, orr = odd0 ? ar : 0.0, oi = odd0 ? ai : 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            ac[g].er[k] = fma(st.mu1[k], er, ac[g].er[k]);
+            ac[g].ei[k] = fma(st.mu1[k], ei, ac[g].ei[k]);
+            ac[g].orr[k] = fma(st.mu1[k], orr, ac[g].orr[k]);
+            ac[g].oi[k] = fma(st.mu1[k], oi, ac[g].oi[k]);
+        }
+    });
+    int cb = l0 + 1;
+    double tA = 0.0;
+    double2 ta[G];
+    {
+        const int j = cb + lane;
+        tA = j <= L ? A[j] : 0.0;
+        for_each_map(std::make_integer_sequence<int, G>{}, [&](auto gi) {
+            constexpr int g = decltype(gi)::value;
+            ta[g] = j <= L ? make_double2(a[g][2 * j], a[g][2 * j + 1]) : make_double2(0.0, 0.0);
+            sa[0][g][lane] = ta[g];
+        });
+        sA[0][lane] = tA;
+    }
+    wave_sync();
+    int buf = 0;
+    int l = cb;
+    while (cb <= L) {
+        const int nb = cb + CH;
+        const int j = nb + lane;
+        if (j <= L) {
+            tA = A[j];
+            for_each_map(std::make_integer_sequence<int, G>{}, [&](auto gi) {
+                constexpr int g = decltype(gi)::value;
+                ta[g] = make_double2(a[g][2 * j], a[g][2 * j + 1]);
+            });
+        }
+        syng_chunk<K, G>(st, ac, l, min(L, nb - 1), cb, m, sA[buf], sa[buf]);
+        if (nb <= L) {
+            sA[buf ^ 1][lane] = tA;
+            for_each_map(std::make_integer_sequence<int, G>{}, [&](auto gi) {
+                constexpr int g = decltype(gi)::value;
+                sa[buf ^ 1][g][lane] = ta[g];
+            });
+        }
+        wave_sync();
+        buf ^= 1;
+        cb = nb;
+    }
+    // lanes still scaled at L store zeros
+    for_each_map(std::make_integer_sequence<int, G>{}, [&](auto gi) {
+        constexpr int g = decltype(gi)::value;
+        const MapDesc& mg = maps[gm[g]];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (p[k] < md.npair) {
+                const bool z = st.sc[k] != 0;
+                double2* dst = reinterpret_cast<double2*>(mg.P + ((long)p[k] * (L + 1) + m) * 4);
+                dst[0] = make_double2(z ? 0.0 : ac[g].er[k], z ? 0.0 : ac[g].ei[k]);
+                dst[1] = make_double2(z ? 0.0 : ac[g].orr[k], z ? 0.0 : ac[g].oi[k]);
+            }
+        }
+    });
+}
+
 // -------------------------------------------------------------- fused spin-2 synthesis
 // Accumulators of one ring pair, per slot (0: steps at even offsets from the
 // item's first l, 1: odd offsets), 18 doubles: T (re, im), then per field f
@@ -2294,6 +2554,33 @@ static int scan_k(const WorkItem* items, int n, const MapDesc* d_maps, int* d_ls
     hipLaunchKernelGGL(k_scan<K>, dim3(n), dim3(64), 0, s, items, d_maps, d_lstart);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
+}
+
+template <int K, int G>
+static int synth_g(const WorkItem* items, int n, const MapDesc* d_maps, const int* d_groups, hipStream_t s)
+{
+    if (n <= 0) return CMBL_OK;
+    hipLaunchKernelGGL((k_synth_g<K, G>), dim3(n), dim3(64), 0, s, items, d_maps, d_groups);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+int launch_synth_group(int K, int G, const WorkItem* items, int n, const MapDesc* d_maps, const int* d_groups,
+                       hipStream_t s)
+{
+    if (G == 4) {
+        switch (K) {
+        case 1: return synth_g<1, 4>(items, n, d_maps, d_groups, s);
+        case 2: return synth_g<2, 4>(items, n, d_maps, d_groups, s);
+        }
+    } else if (G == 2) {
+        switch (K) {
+        case 1: return synth_g<1, 2>(items, n, d_maps, d_groups, s);
+        case 2: return synth_g<2, 2>(items, n, d_maps, d_groups, s);
+        case 4: return synth_g<4, 2>(items, n, d_maps, d_groups, s);
+        }
+    }
+    return fail(CMBL_ERR_ARG, "unsupported shared-synthesis shape");
 }
 
 int launch_synth(int K, const WorkItem* items, int n, const MapDesc* d_maps, hipStream_t s)

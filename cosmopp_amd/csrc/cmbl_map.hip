@@ -7,9 +7,11 @@
 #include <cstring>
 #include <dlfcn.h>
 #include <fstream>
+#include <map>
 #include <mutex>
 #include <sstream>
 #include <random>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -1371,6 +1373,34 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
     std::vector<MapDesc> descs;
     std::vector<ItemCost> w1[4], w5[4];
     std::vector<RingItem> rings;
+    // maps of one geometry (nside, lmax, synthesis ring pairs per lane) share
+    // the synthesis recurrence in groups of four (then two)
+    std::vector<int> group_of(nmaps, -1);
+    std::vector<int> gtable;                   // G map indices per group
+    std::vector<std::pair<int, int>> gshape;   // (K, G) per group
+    {
+        std::map<std::tuple<int, int, int>, std::vector<int>> byg;
+        for (int i = 0; i < nmaps; ++i)
+            if (maps[i]) byg[std::make_tuple(maps[i]->nside, maps[i]->L, maps[i]->K1)].push_back(i);
+        for (auto& kv : byg) {
+            const std::vector<int>& v = kv.second;
+            const int K = std::get<2>(kv.first);
+            size_t at = 0;
+            for (int G : {4, 2}) {
+                const bool ok = (G == 4 && (K == 1 || K == 2)) || (G == 2 && K <= 4);
+                for (; ok && at + G <= v.size(); at += G) {
+                    const int gid = (int)gshape.size();
+                    gshape.push_back({K, G});
+                    for (int j = 0; j < G; ++j) {
+                        gtable.push_back(v[at + j]);
+                        group_of[v[at + j]] = gid;
+                    }
+                    gtable.resize(gtable.size() + 4 - G, -1);  // table rows of 4
+                }
+            }
+        }
+    }
+    std::map<std::pair<int, int>, std::vector<ItemCost>> wg;  // (K, G) -> items
     for (int i = 0; i < nmaps; ++i) {
         cmbl_map* mp = maps[i];
         if (!mp) return cleanup(fail(CMBL_ERR_ARG, "null map in batch"));
@@ -1383,9 +1413,18 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
         int k1 = 0, k5 = 0;
         while ((1 << k1) < mp->K1) ++k1;
         while ((1 << k5) < mp->K5) ++k5;
-        for (ItemCost ic : mp->cost1) {
-            ic.it.map = i;
-            w1[k1].push_back(ic);
+        const int gid = group_of[i];
+        if (gid < 0) {
+            for (ItemCost ic : mp->cost1) {
+                ic.it.map = i;
+                w1[k1].push_back(ic);
+            }
+        } else if (gtable[4 * gid] == i) {  // the group's first map brings the group's items
+            for (ItemCost ic : mp->cost1) {
+                ic.it.map = gid;
+                ic.cost *= 0.5 + 0.5 * gshape[gid].second;  // recurrence once, sums G times
+                wg[gshape[gid]].push_back(ic);
+            }
         }
         for (ItemCost ic : mp->cost5) {
             ic.it.map = i;
@@ -1421,6 +1460,28 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
                 b->n1[k] = (int)w.size();
             }
         }
+    }
+    for (auto& kv : wg) {
+        const std::vector<WorkItem> w = order_items(kv.second);
+        if (hipMalloc(&q, sizeof(WorkItem) * w.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
+        b->allocs.push_back(q);
+        if (hipMemcpy(q, w.data(), sizeof(WorkItem) * w.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(CMBL_ERR_HIP, "batch upload"));
+        cmbl_batch::SynGroupList gl;
+        gl.K = kv.first.first;
+        gl.G = kv.first.second;
+        gl.n = (int)w.size();
+        gl.d_items = (WorkItem*)q;
+        b->syn_groups.push_back(gl);
+    }
+    if (!gtable.empty()) {
+        // rows of four entries; a G = 2 group reads the first two of its row
+        if (hipMalloc(&q, sizeof(int) * gtable.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
+        b->allocs.push_back(q);
+        if (hipMemcpy(q, gtable.data(), sizeof(int) * gtable.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(CMBL_ERR_HIP, "batch upload"));
+        b->d_gmaps = (int*)q;
+        b->ngroups = (int)gshape.size();
     }
     b->ring_classes = ring_classes(&rings, [&](const RingItem& r) { return maps[r.map]->ring_M[r.pair]; });
     if (hipMalloc(&q, sizeof(RingItem) * rings.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
@@ -1485,6 +1546,8 @@ int cmbl_batch_value_and_grad_device(cmbl_batch* b, const double* const* x, doub
     for (size_t q = 0; q < groups.size(); ++q)
         CMBL_TRY(launch_pre_batch(b->d_descs + q * kBatchGroup, groups[q], s));
     for (int k = 0; k < 4; ++k) CMBL_TRY(launch_synth(1 << k, b->d_items1[k], b->n1[k], b->d_descs, s));
+    for (const cmbl_batch::SynGroupList& gl : b->syn_groups)
+        CMBL_TRY(launch_synth_group(gl.K, gl.G, gl.d_items, gl.n, b->d_descs, b->d_gmaps, s));
     CMBL_TRY(run_ring(kRingLike, b->d_ring_items, b->ring_classes, b->d_descs, nullptr, nullptr, b->d_tw, s));
     for (int k = 0; k < 4; ++k) CMBL_TRY(launch_adj(1 << k, b->d_items5[k], b->n5[k], b->d_descs, s));
     for (size_t q = 0; q < groups.size(); ++q) {

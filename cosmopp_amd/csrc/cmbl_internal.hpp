@@ -146,10 +146,12 @@ constexpr double kScaleUp = 0x1p+1000;
 constexpr double kScaleLo = 0x1p-100;
 
 constexpr int kWave = 64;
-constexpr int kRingThreads = 512;  // threads of a ring workgroup (largest)
+constexpr int kRingThreads = 512;  // threads of a ring workgroup up to M = 4096
 // Threads of a ring workgroup for transforms up to maxm: 256 up to 2048, so
-// that four such workgroups share a CU where two would leave most threads idle.
-constexpr int ring_threads(int maxm) { return maxm <= 2048 ? 256 : kRingThreads; }
+// that four such workgroups share a CU where two would leave most threads
+// idle; 1024 at 8192, whose LDS (139 KB) allows one workgroup per CU, so that
+// the CU still holds sixteen waves.
+constexpr int ring_threads(int maxm) { return maxm <= 2048 ? 256 : maxm <= 4096 ? kRingThreads : 1024; }
 constexpr int kMaxFFT = 8192;          // largest LDS FFT (complex doubles)
 // LDS bytes of a sequence of n complex doubles in the ring stage's padded
 // layout (one spare slot per 16 values).

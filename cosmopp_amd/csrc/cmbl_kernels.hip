@@ -1873,8 +1873,40 @@ __device__ __forceinline__ constexpr int brev_c(int q)
 constexpr int kLogMaxFFT = 13;
 static_assert((1 << kLogMaxFFT) == kMaxFFT, "twiddle table length");
 
-template <int R, int LR, int SIGN, int MAXM>
-__device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, int lp, const double2* __restrict__ tw)
+// Pointwise factors of a Bluestein transform fused into the first and last
+// pass of its two FFTs: kFuseChirpIn multiplies each loaded input by the
+// chirp (inputs at n and beyond are zero and not read), kFuseFilt each output
+// by the chirp filter, kFuseChirpOut each output by 1/M and the chirp (outputs
+// at n and beyond are not stored).  OS is the sign of the ring transform: for
+// OS > 0 the chirps are conjugated and the filter is read reversed and
+// conjugated.
+enum { kFuseNone = 0, kFuseChirpIn = 1, kFuseFilt = 2, kFuseChirpOut = 3 };
+struct BsFuse {
+    const double2* chirp;
+    const double2* filt;
+    int n;
+    double inv;  // 1 / M
+};
+
+template <int OS>
+__device__ __forceinline__ double2 bs_chirp(BsFuse f, int j)
+{
+    double2 c = gview(f.chirp)[j];
+    if (OS > 0) c.y = -c.y;
+    return c;
+}
+
+template <int OS>
+__device__ __forceinline__ double2 bs_filt(BsFuse f, int r, int M)
+{
+    if (OS < 0) return gview(f.filt)[r];
+    const double2 g = gview(f.filt)[(M - r) & (M - 1)];
+    return make_double2(g.x, -g.y);
+}
+
+template <int R, int LR, int SIGN, int MAXM, int PRE = kFuseNone, int POST = kFuseNone, int OS = -1>
+__device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, int lp, const double2* __restrict__ tw,
+                                              BsFuse fz = BsFuse{})
 {
     constexpr int NT = ring_threads(MAXM);
     constexpr int B = (MAXM / R + NT - 1) / NT;  // butterflies per thread, at most
@@ -1893,8 +1925,22 @@ __device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, int lp, 
     for (int b = 0; b < B; ++b) {
         const int i = tid + b * NT;
         if (i < nb) {
+            if (PRE == kFuseChirpIn) {
+                // chirps first (global loads in flight), then the LDS loads
 #pragma unroll
-            for (int r = 0; r < R; ++r) u[b][r] = Z[zp(i + r * nb)];
+                for (int r = 0; r < R; ++r) {
+                    const int e = i + r * nb;
+                    u[b][r] = e < fz.n ? bs_chirp<OS>(fz, e) : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int e = i + r * nb;
+                    if (e < fz.n) u[b][r] = cmul(Z[zp(e)], u[b][r]);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; ++r) u[b][r] = Z[zp(i + r * nb)];
+            }
         }
     }
     __syncthreads();
@@ -1915,7 +1961,17 @@ __device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, int lp, 
             dft_reg<R, SIGN>(u[b]);
             const int j = (i - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; ++q) Z[zp(j + q * p)] = u[b][brev_c<LR>(q)];
+            for (int q = 0; q < R; ++q) {
+                const int e = j + q * p;
+                double2 v = u[b][brev_c<LR>(q)];
+                if (POST == kFuseFilt) {
+                    v = cmul(v, bs_filt<OS>(fz, e, M));
+                } else if (POST == kFuseChirpOut) {
+                    if (e >= fz.n) continue;
+                    v = cmul(make_double2(v.x * fz.inv, v.y * fz.inv), bs_chirp<OS>(fz, e));
+                }
+                Z[zp(e)] = v;
+            }
         }
     }
     __syncthreads();
@@ -1923,71 +1979,55 @@ __device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, int lp, 
 
 // Unnormalised DFT of Z[0..M) in place, natural order in and out:
 // Z_q <- sum_r Z_r e^{SIGN 2 pi i r q / M}, M = 2^logM <= MAXM <= kMaxFFT, M >= 2.
-// Radix-16 passes, then one radix-8/4/2 pass for the remaining factor.  MAXM
+// Radix-8 passes, then one radix-4/2 pass for the remaining factor.  MAXM
 // bounds the butterflies per thread and so the registers.
 template <int SIGN, int MAXM>
 __device__ void fft(double2* Z, int M, int logM, const double2* __restrict__ tw)
 {
-    // radix 16 where the class runs one workgroup per CU anyway (LDS-bound),
-    // radix 8 below, whose halved register set lets two workgroups share a CU
-    constexpr int LR = MAXM > 4096 ? 4 : 3;
+    // radix 8: one butterfly per thread at M = 8 x threads, within the 128
+    // VGPRs that four waves per SIMD allow (radix 16 spills there)
+    constexpr int LR = 3;
     int p = 1, lp = 0, rem = logM;
     for (; rem >= LR; rem -= LR, p <<= LR, lp += LR) stockham_pass<1 << LR, LR, SIGN, MAXM>(Z, M, p, lp, tw);
-    if (rem == 3)
-        stockham_pass<8, 3, SIGN, MAXM>(Z, M, p, lp, tw);
-    else if (rem == 2)
+    if (rem == 2)
         stockham_pass<4, 2, SIGN, MAXM>(Z, M, p, lp, tw);
     else if (rem == 1)
         stockham_pass<2, 1, SIGN, MAXM>(Z, M, p, lp, tw);
 }
 
-// Bluestein DFT of z[0..n) (natural order, zero-padded by this call to M):
-// result in natural order in z[0..n).  chirp[j] = e^{-i pi j^2/n};
-// filt = FFT_M of e^{+i pi j^2/n} (wrapped), natural order.  For SIGN > 0 the
-// chirps are conjugated and the filter is read reversed and conjugated.  The
-// loops are unrolled by four so that four table loads are in flight at once.
-template <int SIGN, int MAXM>
-__device__ void bluestein(double2* z, int n, int M, int logM, const double2* __restrict__ chirp_,
-                          const double2* __restrict__ filt_, const double2* __restrict__ tw)
+// The FFT of fft() with pointwise factors fused into its first (PRE) and last
+// (POST) pass; M >= 8.
+template <int SIGN, int MAXM, int PRE, int POST, int OS>
+__device__ void fft_fused(double2* Z, int M, int logM, const double2* __restrict__ tw, BsFuse fz)
 {
-    auto chirp = gview(chirp_);
-    auto filt = gview(filt_);
-    constexpr int nt = ring_threads(MAXM);  // the ring kernels' workgroup size
-    const int tid = threadIdx.x;
-#pragma unroll 4
-    for (int j = tid; j < M; j += nt) {
-        if (j < n) {
-            double2 c = chirp[j];
-            if (SIGN > 0) c.y = -c.y;
-            z[zp(j)] = cmul(z[zp(j)], c);
-        } else {
-            z[zp(j)] = make_double2(0.0, 0.0);
-        }
+    if (logM == 3) {
+        stockham_pass<8, 3, SIGN, MAXM, PRE, POST, OS>(Z, M, 1, 0, tw, fz);
+        return;
     }
-    __syncthreads();
-    fft<-1, MAXM>(z, M, logM, tw);
-#pragma unroll 4
-    for (int r = tid; r < M; r += nt) {
-        double2 f;
-        if (SIGN < 0) {
-            f = filt[r];
-        } else {
-            const double2 g = filt[(M - r) & (M - 1)];
-            f = make_double2(g.x, -g.y);
-        }
-        z[zp(r)] = cmul(z[zp(r)], f);
-    }
-    __syncthreads();
-    fft<+1, MAXM>(z, M, logM, tw);
-    const double inv = ldexp(1.0, -logM);  // 1 / M, M = 2^logM
-#pragma unroll 4
-    for (int k = tid; k < n; k += nt) {
-        double2 c = chirp[k];
-        if (SIGN > 0) c.y = -c.y;
-        const double2 v = z[zp(k)];
-        z[zp(k)] = cmul(make_double2(v.x * inv, v.y * inv), c);
-    }
-    __syncthreads();
+    stockham_pass<8, 3, SIGN, MAXM, PRE, kFuseNone, OS>(Z, M, 1, 0, tw, fz);
+    int p = 8, lp = 3, rem = logM - 3;
+    for (; rem > 3; rem -= 3, p <<= 3, lp += 3) stockham_pass<8, 3, SIGN, MAXM>(Z, M, p, lp, tw);
+    if (rem == 3)
+        stockham_pass<8, 3, SIGN, MAXM, kFuseNone, POST, OS>(Z, M, p, lp, tw, fz);
+    else if (rem == 2)
+        stockham_pass<4, 2, SIGN, MAXM, kFuseNone, POST, OS>(Z, M, p, lp, tw, fz);
+    else
+        stockham_pass<2, 1, SIGN, MAXM, kFuseNone, POST, OS>(Z, M, p, lp, tw, fz);
+}
+
+// Bluestein DFT of z[0..n) (natural order; M >= 2n, so M >= 8): result in
+// natural order in z[0..n).  chirp[j] = e^{-i pi j^2/n}; filt = FFT_M of
+// e^{+i pi j^2/n} (wrapped), natural order.  For SIGN > 0 the chirps are
+// conjugated and the filter is read reversed and conjugated.  The chirp
+// products, the zero padding, the filter product and the final chirp and 1/M
+// ride on the first and last passes of the two FFTs.
+template <int SIGN, int MAXM>
+__device__ void bluestein(double2* z, int n, int M, int logM, const double2* __restrict__ chirp,
+                          const double2* __restrict__ filt, const double2* __restrict__ tw)
+{
+    const BsFuse fz{chirp, filt, n, ldexp(1.0, -logM)};  // 1 / M, M = 2^logM
+    fft_fused<-1, MAXM, kFuseChirpIn, kFuseFilt, SIGN>(z, M, logM, tw, fz);
+    fft_fused<+1, MAXM, kFuseNone, kFuseChirpOut, SIGN>(z, M, logM, tw, fz);
 }
 
 // Length-n DFT in LDS (natural order): the FFT for power-of-two n, Bluestein otherwise.
@@ -2421,7 +2461,7 @@ __global__ __launch_bounds__(ring_threads(MAXM), MAXM > 4096 ? 1 : 4) void k_rin
 // Bluestein set-up for one ring pair of Bluestein length n (nph, or nph/2 for
 // a split ring): chirp[j] = e^{-i pi j^2/n} (j < n) and filt = FFT_M of
 // b_j = e^{+i pi j^2/n} for |j| < n wrapped mod M, natural order.
-__global__ __launch_bounds__(kRingThreads) void k_bluestein_setup(const int* __restrict__ pairs,
+__global__ __launch_bounds__(ring_threads(kMaxFFT)) void k_bluestein_setup(const int* __restrict__ pairs,
                                                                   const MapDesc* __restrict__ md_,
                                                                   double2* __restrict__ filt_out,
                                                                   double2* __restrict__ chirp_out,
@@ -2690,7 +2730,7 @@ int launch_bluestein_setup(const MapDesc* d_md, const int* d_pairs, int npairs_b
     if (npairs_bs <= 0) return CMBL_OK;
     static std::atomic<unsigned long long> lds_set{0};
     CMBL_TRY(allow_dynamic_lds((const void*)k_bluestein_setup, lds_set));
-    hipLaunchKernelGGL(k_bluestein_setup, dim3(npairs_bs), dim3(kRingThreads), lds, s, d_pairs, d_md, d_out, d_chirp,
+    hipLaunchKernelGGL(k_bluestein_setup, dim3(npairs_bs), dim3(ring_threads(kMaxFFT)), lds, s, d_pairs, d_md, d_out, d_chirp,
                        tw);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;

@@ -178,8 +178,8 @@ struct MapDesc {
     const int* ring_M;        // FFT length used (nph if a power of two, else Bluestein length)
     const long* ring_bsofs;   // offset of the Bluestein filter in bs_filt (-1 if none)
     const long* ring_chofs;   // offset of the Bluestein chirp in bs_chirp (-1 if none)
-    const double2* bs_filt;   // FFT_M of e^{+i pi j^2/b}, natural order, per non-power-of-two ring
-    const double2* bs_chirp;  // e^{-i pi j^2/b}, j < b, per non-power-of-two ring
+    const double2* bs_filt;   // FFT_M of e^{+i pi j^2/b}, entries 0..M/2, per non-power-of-two ring
+    const double2* bs_chirp;  // e^{-i pi j^2/b}, j <= b/2, per non-power-of-two ring
     // Rings whose Bluestein length for nph would exceed kMaxFFT are split into
     // two transforms of b = nph/2 (even and odd samples); the others use b = nph.
     const long* ring_scrofs;  // offset of the split ring's scratch in ring_scr (-1 if not split)

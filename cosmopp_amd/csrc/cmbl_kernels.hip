@@ -1888,10 +1888,14 @@ struct BsFuse {
     double inv;  // 1 / M
 };
 
+// The tables hold half of each sequence: the chirp of even length n has
+// c_{n-j} = c_j ((n - j)^2 = j^2 mod 2n), and the filter, the FFT of a
+// sequence even in j (b_{M-j} = b_j), has F_{M-r} = F_r; entries 0..n/2 and
+// 0..M/2 are stored.
 template <int OS>
 __device__ __forceinline__ double2 bs_chirp(BsFuse f, int j)
 {
-    double2 c = gview(f.chirp)[j];
+    double2 c = gview(f.chirp)[min(j, f.n - j)];
     if (OS > 0) c.y = -c.y;
     return c;
 }
@@ -1899,9 +1903,9 @@ __device__ __forceinline__ double2 bs_chirp(BsFuse f, int j)
 template <int OS>
 __device__ __forceinline__ double2 bs_filt(BsFuse f, int r, int M)
 {
-    if (OS < 0) return gview(f.filt)[r];
-    const double2 g = gview(f.filt)[(M - r) & (M - 1)];
-    return make_double2(g.x, -g.y);
+    double2 g = gview(f.filt)[min(r, M - r)];
+    if (OS > 0) g.y = -g.y;  // F_{M-r} conjugated, F_{M-r} = F_r
+    return g;
 }
 
 template <int R, int LR, int SIGN, int MAXM, int PRE = kFuseNone, int POST = kFuseNone, int OS = -1>
@@ -2016,9 +2020,9 @@ __device__ void fft_fused(double2* Z, int M, int logM, const double2* __restrict
 }
 
 // Bluestein DFT of z[0..n) (natural order; M >= 2n, so M >= 8): result in
-// natural order in z[0..n).  chirp[j] = e^{-i pi j^2/n}; filt = FFT_M of
-// e^{+i pi j^2/n} (wrapped), natural order.  For SIGN > 0 the chirps are
-// conjugated and the filter is read reversed and conjugated.  The chirp
+// natural order in z[0..n).  chirp[j] = e^{-i pi j^2/n}, j <= n/2; filt =
+// FFT_M of e^{+i pi j^2/n} (wrapped), entries 0..M/2.  For SIGN > 0 the chirps
+// are conjugated and the filter is read reversed and conjugated.  The chirp
 // products, the zero padding, the filter product and the final chirp and 1/M
 // ride on the first and last passes of the two FFTs.
 template <int SIGN, int MAXM>
@@ -2459,8 +2463,9 @@ __global__ __launch_bounds__(ring_threads(MAXM), MAXM > 4096 ? 1 : 4) void k_rin
 }
 
 // Bluestein set-up for one ring pair of Bluestein length n (nph, or nph/2 for
-// a split ring): chirp[j] = e^{-i pi j^2/n} (j < n) and filt = FFT_M of
-// b_j = e^{+i pi j^2/n} for |j| < n wrapped mod M, natural order.
+// a split ring; even): chirp[j] = e^{-i pi j^2/n} for j <= n/2 and filt =
+// FFT_M of b_j = e^{+i pi j^2/n} for |j| < n wrapped mod M, entries 0..M/2 (the
+// rest follow by symmetry).
 __global__ __launch_bounds__(ring_threads(kMaxFFT)) void k_bluestein_setup(const int* __restrict__ pairs,
                                                                   const MapDesc* __restrict__ md_,
                                                                   double2* __restrict__ filt_out,
@@ -2483,7 +2488,7 @@ __global__ __launch_bounds__(ring_threads(kMaxFFT)) void k_bluestein_setup(const
         Z[zp(j)] = jj >= 0 ? cis_pi_frac(((long)jj * jj) % (2L * n), n) : make_double2(0.0, 0.0);
     }
     double2* ch = chirp_out + md.ring_chofs[p];
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    for (int j = threadIdx.x; j <= n / 2; j += blockDim.x) {
         double2 c = cis_pi_frac(((long)j * j) % (2L * n), n);
         c.y = -c.y;
         ch[j] = c;
@@ -2491,7 +2496,7 @@ __global__ __launch_bounds__(ring_threads(kMaxFFT)) void k_bluestein_setup(const
     __syncthreads();
     fft<-1, kMaxFFT>(Z, M, logM, tw);
     double2* dst = filt_out + md.ring_bsofs[p];
-    for (int j = threadIdx.x; j < M; j += blockDim.x) dst[j] = Z[zp(j)];
+    for (int j = threadIdx.x; j <= M / 2; j += blockDim.x) dst[j] = Z[zp(j)];
 }
 
 // ================================================================== k_final

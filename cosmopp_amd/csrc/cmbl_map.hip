@@ -426,10 +426,11 @@ static int map_create(cmbl_map* mp, int nside, int lmax, const cmbl_tuning& tune
                 scr_total += 3L * b;
             }
             Mfft[p] = M;
+            // half tables (the chirp and the filter are symmetric)
             bsofs[p] = bs_total;
-            bs_total += M;
+            bs_total += M / 2 + 1;
             chofs[p] = ch_total;
-            ch_total += b;
+            ch_total += b / 2 + 1;
             bs_pairs.push_back(p);
         }
         maxM = std::max(maxM, Mfft[p]);

@@ -238,9 +238,14 @@ struct ItemCost {
     double cost;
 };
 
+// A ring pair of a map; with pair2 >= 0 also a second ring pair (of map2) of
+// the same power-of-two length, half the launch's largest, in the same
+// workgroup.
 struct RingItem {
     int map;
     int pair;
+    int map2 = -1;
+    int pair2 = -1;
 };
 
 // kRingLike: s -> f = w (d - s) and chi partials -> analysis;  kRingSyn: s -> map;
@@ -357,6 +362,7 @@ struct cmbl_map {
     int nitems1 = 0, nitems5 = 0, nring_items = 0;
     std::vector<cmbl::RingClass> ring_classes;
     std::vector<int> ring_M;  // transform length per pair
+    std::vector<int> ring_two;  // n for power-of-two unsplit pairs with n > lmax (two may share a workgroup), else 0
     std::vector<cmbl::WorkItem> items1, items5;   // launch order (XCD-interleaved)
     std::vector<cmbl::ItemCost> cost1, cost5;      // live items with cost estimates
     std::vector<cmbl::RingItem> ring_items;

@@ -1981,6 +1981,74 @@ __device__ __forceinline__ void stockham_pass(double2* Z, int M, int p, int lp, 
     __syncthreads();
 }
 
+// stockham_pass over two transforms of length M = 2^logM side by side in
+// LDS (transform t at Z[t M ..)): butterfly g belongs to transform g / (M/R).
+template <int R, int LR, int SIGN, int MAXM>
+__device__ __forceinline__ void stockham_pass_two(double2* Z, int logM, int p, int lp, const double2* __restrict__ tw)
+{
+    constexpr int NT = ring_threads(MAXM);
+    constexpr int B = (MAXM / R + NT - 1) / NT;  // butterflies per thread, at most (2 M <= MAXM)
+    const int tid = threadIdx.x;
+    const int lnb = logM - LR;  // log2 of the butterflies per transform
+    const int nb = 1 << lnb;
+    double2 u[B][R];
+    double2 w1v[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int g = tid + b * NT;
+        const int i = g & (nb - 1);
+        const int k = i & (p - 1);
+        w1v[b] = make_double2(1.0, 0.0);
+        if (g < 2 * nb && k) w1v[b] = tw_get(tw, k << (kLogMaxFFT - lp - LR), SIGN);
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int g = tid + b * NT;
+        if (g < 2 * nb) {
+            const int base = (g >> lnb) << logM;
+            const int i = g & (nb - 1);
+#pragma unroll
+            for (int r = 0; r < R; ++r) u[b][r] = Z[zp(base + i + r * nb)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int g = tid + b * NT;
+        if (g < 2 * nb) {
+            const int base = (g >> lnb) << logM;
+            const int i = g & (nb - 1);
+            const int k = i & (p - 1);
+            if (k) {
+                const double2 w1 = w1v[b];
+                double2 w = w1;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    u[b][r] = cmul(u[b][r], w);
+                    if (r + 1 < R) w = cmul(w, w1);
+                }
+            }
+            dft_reg<R, SIGN>(u[b]);
+            const int j = (i - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) Z[zp(base + j + q * p)] = u[b][brev_c<LR>(q)];
+        }
+    }
+    __syncthreads();
+}
+
+// fft() of two transforms of length M = 2^logM side by side (2 M <= MAXM).
+template <int SIGN, int MAXM>
+__device__ void fft_two(double2* Z, int logM, const double2* __restrict__ tw)
+{
+    int p = 1, lp = 0, rem = logM;
+    for (; rem >= 3; rem -= 3, p <<= 3, lp += 3) stockham_pass_two<8, 3, SIGN, MAXM>(Z, logM, p, lp, tw);
+    if (rem == 2)
+        stockham_pass_two<4, 2, SIGN, MAXM>(Z, logM, p, lp, tw);
+    else if (rem == 1)
+        stockham_pass_two<2, 1, SIGN, MAXM>(Z, logM, p, lp, tw);
+}
+
 // Unnormalised DFT of Z[0..M) in place, natural order in and out:
 // Z_q <- sum_r Z_r e^{SIGN 2 pi i r q / M}, M = 2^logM <= MAXM <= kMaxFFT, M >= 2.
 // Radix-8 passes, then one radix-4/2 pass for the remaining factor.  MAXM
@@ -2245,6 +2313,189 @@ __device__ void ring_split(const MapDesc& md, int p, int n, int M, int logM, boo
     }
 }
 
+// Two power-of-two rings of length n = MAXM / 2 in one workgroup (each longer
+// than its map's lmax and unsplit; the pair of a RingItem with pair2 >= 0):
+// ring r's spectrum and samples live at Z[r n ..), a thread holds CNT / 2
+// samples of each, and every step is the single ring's (one-alias fold,
+// transform, residual or weights, transform, unfold) with the two transforms
+// side by side.  Rings of half the launch's length would otherwise leave half
+// of their workgroup's threads idle in every FFT pass.
+constexpr int clog2(int v) { return v <= 1 ? 0 : 1 + clog2(v / 2); }
+
+template <int MODE, int MAXM>
+__device__ void ring_two(const RingItem& it, const MapDesc* __restrict__ maps, double* __restrict__ map_out,
+                         const double* __restrict__ map_in, const double2* __restrict__ tw, double2* Z, double* sh)
+{
+    constexpr int NT = ring_threads(MAXM);
+    constexpr int n = MAXM / 2;
+    constexpr int logn = clog2(n);
+    constexpr int CNT = MAXM / NT;  // samples per thread, both rings
+    constexpr int CH = CNT / 2;     // per ring
+    static_assert(CNT % 2 == 0 && (1 << logn) == n, "two rings per workgroup");
+    const int tid = threadIdx.x;
+    const MapDesc* mdr[2] = {&maps[it.map], &maps[it.map2]};
+    const int pr[2] = {it.pair, it.pair2};
+    bool shf[2];
+    long oN[2], oS[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        shf[r] = mdr[r]->ring_shift[pr[r]] != 0;
+        oN[r] = mdr[r]->ring_ofsN[pr[r]];
+        oS[r] = mdr[r]->ring_ofsS[pr[r]];
+    }
+
+    if (MODE != kRingAna) {
+        // fold, n > L: at most the alias k and the mirror n - k per bin
+        double2 e[CNT], o[CNT];
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int r = t / CH;
+            const int L = mdr[r]->L;
+            auto rw = gview(reinterpret_cast<const double2*>(mdr[r]->P + (long)pr[r] * (L + 1) * 4));
+            const int k = tid + (t - r * CH) * NT;
+            const int m = k <= L ? k : n - k;
+            if (k <= L || (k >= 1 && n - k <= L)) {
+                e[t] = rw[2 * m];
+                o[t] = rw[2 * m + 1];
+            }
+        }
+        PhaseSeq ps;
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int r = t / CH;
+            const int L = mdr[r]->L;
+            auto rw = gview(reinterpret_cast<const double2*>(mdr[r]->P + (long)pr[r] * (L + 1) * 4));
+            const int k = tid + (t - r * CH) * NT;
+            if (t % CH == 0 && shf[r]) ps.init(tid, NT, n);
+            const double sg = shf[r] ? -1.0 : 1.0;  // (-1)^{j shift} at j = 1
+            const double2 ph = shf[r] ? ps.next() : make_double2(1.0, 0.0);
+            double2 gN = make_double2(0.0, 0.0), gS = make_double2(0.0, 0.0);
+            const bool mirror = k >= 1 && n - k <= L;
+            if (k <= L) {
+                gN.x += 1.0 * (e[t].x + o[t].x);
+                gN.y += 1.0 * (e[t].y + o[t].y);
+                gS.x += 1.0 * (e[t].x - o[t].x);
+                gS.y += 1.0 * (e[t].y - o[t].y);
+                if (mirror) {
+                    e[t] = rw[2 * (n - k)];
+                    o[t] = rw[2 * (n - k) + 1];
+                }
+            }
+            if (mirror) {
+                gN.x += sg * (e[t].x + o[t].x);
+                gN.y -= sg * (e[t].y + o[t].y);
+                gS.x += sg * (e[t].x - o[t].x);
+                gS.y -= sg * (e[t].y - o[t].y);
+            }
+            fold_store(Z, r * n + k, ph, shf[r], oS[r] < 0, gN, gS);
+        }
+        __syncthreads();
+        fft_two<+1, MAXM>(Z, logn, tw);
+    }
+
+    if (MODE == kRingSyn) {
+        gdouble* out = gview(map_out);
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int r = t / CH;
+            const int j = tid + (t - r * CH) * NT;
+            const double2 v = Z[zp(r * n + j)];
+            out[oN[r] + j] = v.x;
+            if (oS[r] >= 0) out[oS[r] + j] = v.y;
+        }
+        return;
+    }
+
+    if (MODE == kRingLike) {
+        double dN[CNT], wN[CNT], dS[CNT], wS[CNT];
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int r = t / CH;
+            gcdouble* dd = gview(mdr[r]->d);
+            gcdouble* ww = gview(mdr[r]->w);
+            const int j = tid + (t - r * CH) * NT;
+            dN[t] = dd[oN[r] + j];
+            wN[t] = ww[oN[r] + j];
+            if (oS[r] >= 0) {
+                dS[t] = dd[oS[r] + j];
+                wS[t] = ww[oS[r] + j];
+            }
+        }
+        double chi[2] = {0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int r = t / CH;
+            const int j = tid + (t - r * CH) * NT;
+            const double2 v = Z[zp(r * n + j)];
+            const double rN = dN[t] - v.x;
+            const double fN = wN[t] * rN;
+            chi[r] = fma(fN, rN, chi[r]);
+            double fS = 0.0;
+            if (oS[r] >= 0) {
+                const double rS = dS[t] - v.y;
+                fS = wS[t] * rS;
+                chi[r] = fma(fS, rS, chi[r]);
+            }
+            Z[zp(r * n + j)] = make_double2(fN, fS);
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const double c = block_sum<NT>(chi[r], sh);
+            if (tid == 0) mdr[r]->chi[pr[r]] = c;
+        }
+    } else if (MODE == kRingWeight) {
+        double wN[CNT], wS[CNT];
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int r = t / CH;
+            gcdouble* ww = gview(mdr[r]->w);
+            const int j = tid + (t - r * CH) * NT;
+            wN[t] = ww[oN[r] + j];
+            wS[t] = oS[r] >= 0 ? ww[oS[r] + j] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int r = t / CH;
+            const int j = tid + (t - r * CH) * NT;
+            const double2 v = Z[zp(r * n + j)];
+            Z[zp(r * n + j)] = make_double2(wN[t] * v.x, oS[r] >= 0 ? wS[t] * v.y : 0.0);
+        }
+        __syncthreads();
+    } else {  // kRingAna
+        gcdouble* in = gview(map_in);
+        double2 mv[CNT];
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int r = t / CH;
+            const int j = tid + (t - r * CH) * NT;
+            mv[t] = make_double2(in[oN[r] + j], oS[r] >= 0 ? in[oS[r] + j] : 0.0);
+        }
+#pragma unroll
+        for (int t = 0; t < CNT; ++t) {
+            const int r = t / CH;
+            const int j = tid + (t - r * CH) * NT;
+            Z[zp(r * n + j)] = mv[t];
+        }
+        __syncthreads();
+    }
+    fft_two<-1, MAXM>(Z, logn, tw);
+
+    // unfold, one alias per bin (n > L)
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int L = mdr[r]->L;
+        double2* qrow = reinterpret_cast<double2*>(mdr[r]->Q + (long)pr[r] * (L + 1) * 4);
+        PhaseSeq ps;
+        if (shf[r]) ps.init(tid, NT, n);
+        for (int k = tid; k < n; k += NT) {
+            const double2 ph = shf[r] ? ps.next() : make_double2(1.0, 0.0);
+            if (k > L) continue;
+            const int nk = (k == 0) ? 0 : n - k;
+            unfold_bin(qrow, L, n, k, 0, 1, shf[r], ph, Z[zp(r * n + k)], Z[zp(r * n + nk)]);
+        }
+    }
+}
+
 // Ring kernels up to M = 4096 are held to four waves per SIMD (128 VGPRs):
 // two workgroups of 512 threads, or four of 256 up to M = 2048, per CU.
 template <int MODE, int MAXM>
@@ -2258,6 +2509,12 @@ __global__ __launch_bounds__(ring_threads(MAXM), MAXM > 4096 ? 1 : 4) void k_rin
     extern __shared__ double2 Z[];
     __shared__ double sh[NT / 64];
     const RingItem it = items[blockIdx.x];
+    if constexpr (MAXM <= 4096) {
+        if (it.pair2 >= 0) {
+            ring_two<MODE, MAXM>(it, maps, map_out, map_in, tw, Z, sh);
+            return;
+        }
+    }
     const MapDesc& md = maps[it.map];
     const int L = md.L;
     const int p = it.pair;

@@ -250,21 +250,51 @@ static std::vector<WorkItem> order_items(std::vector<ItemCost>& v)
 // the rest.  Up to M = 4096 the ring kernels are held by their registers to
 // two workgroups per CU whatever their LDS, so one launch carries every ring
 // up to 4096 and the small ones fill its tail instead of running as launches
-// of a few hundred workgroups.
-template <typename MOf>
-static std::vector<RingClass> ring_classes(std::vector<RingItem>* items, MOf Mof)
+// of a few hundred workgroups.  In that launch, power-of-two rings of half its
+// largest length (Two(item) == n, n > lmax, unsplit) go two to a workgroup:
+// alone they would leave half of its threads idle in every FFT pass.  The
+// pairs lead their launch (each weighs about one ring of the full length).
+template <typename MOf, typename TwoOf>
+static std::vector<RingClass> ring_classes(std::vector<RingItem>* items, MOf Mof, TwoOf Two)
 {
     std::stable_sort(items->begin(), items->end(),
                      [&](const RingItem& a, const RingItem& b) { return Mof(a) > Mof(b); });
     auto cls = [](int M) { return M > 4096 ? 0 : 1; };
+    std::vector<RingItem> out_items;
     std::vector<RingClass> out;
-    for (int i = 0; i < (int)items->size(); ++i) {
-        const int M = Mof((*items)[i]);
-        if (out.empty() || cls(Mof((*items)[out.back().first])) != cls(M))
-            // at least 2 * kRingThreads entries: the fold's partial sums live in LDS
-            out.push_back(RingClass{i, 0, lds_bytes(std::max(M, 2 * kRingThreads))});
-        ++out.back().count;
+    for (size_t i = 0; i < items->size();) {
+        const int Mfirst = Mof((*items)[i]);
+        size_t j = i;
+        while (j < items->size() && cls(Mof((*items)[j])) == cls(Mfirst)) ++j;
+        // the launch's kernel template: 2048 or 4096 for the class up to 4096
+        const int maxm = Mfirst > 4096 ? 0 : (Mfirst > 2048 ? 4096 : 2048);
+        const int half = (maxm > 0 && Mfirst == maxm) ? maxm / 2 : 0;
+        const size_t start = out_items.size();
+        std::vector<RingItem> rest;
+        int pending = -1;
+        for (size_t k = i; k < j; ++k) {
+            const RingItem& r = (*items)[k];
+            if (half > 0 && Two(r) == half) {
+                if (pending < 0) {
+                    pending = (int)k;
+                } else {
+                    RingItem b = (*items)[pending];
+                    b.map2 = r.map;
+                    b.pair2 = r.pair;
+                    out_items.push_back(b);
+                    pending = -1;
+                }
+            } else {
+                rest.push_back(r);
+            }
+        }
+        if (pending >= 0) rest.insert(rest.begin(), (*items)[pending]);
+        out_items.insert(out_items.end(), rest.begin(), rest.end());
+        // at least 2 * kRingThreads entries: the fold's partial sums live in LDS
+        out.push_back(RingClass{(int)start, (int)(out_items.size() - start), lds_bytes(std::max(Mfirst, 2 * kRingThreads))});
+        i = j;
     }
+    *items = out_items;
     return out;
 }
 
@@ -649,7 +679,12 @@ static int map_create(cmbl_map* mp, int nside, int lmax, const cmbl_tuning& tune
     // ring items: largest transform first
     for (int p = 0; p < mp->npair; ++p) mp->ring_items.push_back(RingItem{0, p});
     mp->ring_M.assign(Mfft.begin(), Mfft.begin() + mp->npair);
-    mp->ring_classes = ring_classes(&mp->ring_items, [&](const RingItem& r) { return Mfft[r.pair]; });
+    mp->ring_two.assign(mp->npair, 0);
+    for (int p = 0; p < mp->npair; ++p)
+        if (Mfft[p] == nph[p] && nph[p] > L && scrofs[p] < 0) mp->ring_two[p] = nph[p];
+    mp->ring_classes = ring_classes(
+        &mp->ring_items, [&](const RingItem& r) { return Mfft[r.pair]; },
+        [&](const RingItem& r) { return mp->ring_two[r.pair]; });
     mp->nring_items = (int)mp->ring_items.size();
     CMBL_TRY(dupload(mp, &mp->d_ring_items, mp->ring_items));
     CMBL_HIP(hipDeviceSynchronize());
@@ -1431,9 +1466,10 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
             ic.it.map = i;
             w5[k5].push_back(ic);
         }
-        for (RingItem r : mp->ring_items) {
-            r.map = i;
-            rings.push_back(r);
+        // the map's own two-ring workgroups are cut again for the batch's launches
+        for (const RingItem& r : mp->ring_items) {
+            rings.push_back(RingItem{i, r.pair});
+            if (r.pair2 >= 0) rings.push_back(RingItem{i, r.pair2});
         }
         b->lds_ring = std::max(b->lds_ring, mp->lds_ring);
     }
@@ -1484,7 +1520,9 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
         b->d_gmaps = (int*)q;
         b->ngroups = (int)gshape.size();
     }
-    b->ring_classes = ring_classes(&rings, [&](const RingItem& r) { return maps[r.map]->ring_M[r.pair]; });
+    b->ring_classes = ring_classes(
+        &rings, [&](const RingItem& r) { return maps[r.map]->ring_M[r.pair]; },
+        [&](const RingItem& r) { return maps[r.map]->ring_two[r.pair]; });
     if (hipMalloc(&q, sizeof(RingItem) * rings.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
     b->allocs.push_back(q);
     b->d_ring_items = (RingItem*)q;

@@ -444,6 +444,8 @@ def main():
                     help="time the device-resident L-BFGS (200 iterations) and HMC (500 leapfrog steps) at config 4")
     ap.add_argument("--gibbs", action="store_true",
                     help="time Gibbs sampler steps (nside 16 and 32, the reference test's sizes) and the Gibbs CG")
+    ap.add_argument("--chains", action="store_true",
+                    help="only time config 5's 16 chain maps (nside 256 / lmax 512) as one batch")
     ap.add_argument("--host-buffers", action="store_true",
                     help="time cmbl_value_and_grad with x and g in host memory (PCIe-inclusive rate)")
     ap.add_argument("--cpu-budget", type=float, default=40.0, help="seconds for the CPU baseline leg")
@@ -487,6 +489,13 @@ def main():
         return
     if args.gibbs:
         bench_gibbs(args)
+        return
+    if args.chains:
+        out = {"metric": "map-level -2lnL + gradient evaluations per second over config 5's 16 chain maps "
+                         "as one batch, FP64", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+               "dtype": "f64", "data": "synthetic"}
+        out.update(bench_c5_chains(args))
+        print(json.dumps(out), flush=True)
         return
     cfg = CONFIGS[args.config]
     if args.pol:

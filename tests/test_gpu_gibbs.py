@@ -54,6 +54,18 @@ def test_gibbs_split_rings_nside2048():
     g.close()
 
 
+def test_gibbs_paired_rings_nside512():
+    """The Gibbs ring pass (f = w s) with the equatorial rings (2048 pixels,
+    half the launch's 4096) two to a workgroup."""
+    nside, lmax = 512, 256
+    print("case gibbs paired nside=%d lmax=%d" % (nside, lmax), flush=True)
+    rng, cl, beam, mask, sigma = problem(nside, lmax, 512, fwhm=30.0)
+    g = cm.Gibbs(nside, lmax, cl, beam, mask, sigma)
+    x = rng.standard_normal((lmax + 1) ** 2)
+    assert rel(g.multiply_by_matrix(x), oracle.gibbs_apply(nside, lmax, x, cl, beam, mask, sigma)) < 1e-10
+    g.close()
+
+
 @pytest.mark.parametrize("nside,lmax", [(8, 16), (32, 64)])
 def test_gibbs_solve_matches_oracle_cg(nside, lmax):
     print("case gibbs solve nside=%d lmax=%d" % (nside, lmax), flush=True)

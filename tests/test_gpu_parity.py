@@ -59,14 +59,15 @@ def test_backward_matches_oracle(nside, lmax):
     mp.close()
 
 
-@pytest.mark.parametrize("nside,lmax", [(2048, 96), (1024, 64), (256, 1024)])
+@pytest.mark.parametrize("nside,lmax", [(2048, 96), (1024, 64), (256, 1024), (512, 256)])
 def test_large_geometries_match_oracle(nside, lmax):
     """Polar rings whose Bluestein length would exceed 4096 (nph = 4i > 2048,
     nside >= 1024) take the split transform (two Bluestein halves of nph/2, one
     held in global scratch; at nside 2048 in a launch of their own), the others
     the unsplit paths in one launch; nside 256 / lmax 1024 aliases every ring
-    in the 256-thread small-ring kernels.  Forward, backward and -2lnL +
-    gradient against the oracle."""
+    in the 256-thread small-ring kernels; at nside 512 the equatorial rings
+    (2048 pixels, half the launch's 4096) run two to a workgroup.  Forward,
+    backward and -2lnL + gradient against the oracle."""
     print("case large geometry nside=%d lmax=%d" % (nside, lmax), flush=True)
     rng = np.random.default_rng(nside + lmax)
     n = (lmax + 1) ** 2

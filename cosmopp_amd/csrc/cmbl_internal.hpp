@@ -290,6 +290,19 @@ int launch_synth(int K, const WorkItem* items, int nitems, const MapDesc* d_maps
 int launch_synth_group(int K, int G, const WorkItem* items, int nitems, const MapDesc* d_maps, const int* d_groups,
                        hipStream_t s);
 int launch_adj(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, hipStream_t s);
+// Shared adjoint of groups of kAdjG maps of one geometry on the FP64 matrix
+// cores: one ring pair per lane, a workgroup of kMmaWaves waves on
+// consecutive 64-pair blocks of one m; item.sidx indexes `subs` (each wave's
+// first live l and start-state slot); item.rb counts blocks of kMmaWaves * 64
+// pairs, whose partials the maps' descriptors address (nrb5 of them).
+constexpr int kAdjG = 8;
+constexpr int kMmaWaves = 4;
+struct MmaSub {
+    int lf[kMmaWaves];    // first live l of the wave's block, -1 if it has none
+    int sidx[kMmaWaves];  // its start-state slot
+};
+int launch_adj_mma(const WorkItem* items, const MmaSub* subs, int nitems, const MapDesc* d_maps, const int* d_groups,
+                   hipStream_t s);
 int lm_tiles(int L);  // blocks of k_pre / k_post (prior partial sums)
 int launch_ldiv(double* y, const double* x, const double* dl, int L, hipStream_t s);
 // spin-2 synthesis: kPolRec complex coefficients per (l, m); parity sums of
@@ -424,6 +437,12 @@ struct cmbl_batch {
     std::vector<SynGroupList> syn_groups;
     int* d_gmaps = nullptr;
     int ngroups = 0;
+    // shared adjoint on the matrix cores: groups of kAdjG maps of one
+    // geometry with one ring pair per lane, their items and group table
+    cmbl::WorkItem* d_items_mma = nullptr;
+    cmbl::MmaSub* d_subs_mma = nullptr;
+    int n_mma = 0;
+    int* d_amaps = nullptr;
     cmbl::RingItem* d_ring_items = nullptr;
     int nring_items = 0;
     int lds_ring = 0;

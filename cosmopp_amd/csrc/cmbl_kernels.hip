@@ -1709,6 +1709,230 @@ __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __re
     if (rd.cnt > 0) rd.flush();
 }
 
+// -------------------------------------------------------------- shared adjoint on the matrix cores
+//
+// Maps of one geometry share lambda_lm(theta).  For a group of kAdjG = 8 such
+// maps the adjoint of one (m, block of 64 ring pairs) is a matrix product,
+//   U[l][c] = sum_p Lambda[l][p] Psi[p][c],  c = (map, re / im),
+// the ring-pair sum of 16 l-rows by 16 columns taking one FP64 matrix
+// instruction per 4 ring pairs (v_mfma_f64_16x16x4f64): the sum over ring
+// pairs happens inside the instruction (no cross-lane reduction) and the
+// recurrence runs once for the 8 maps.  Lane layout of that instruction:
+// A[i][k] comes from lane i + 16 k, B[k][j] from lane j + 16 k, and D[i][j]
+// lands in lane j + 16 (i % 4), register i / 4.
+// Each lane runs the recurrence of one ring pair and writes lambda_l (times a
+// gate that is 0 while the lane is scaled) into an LDS tile of kMmaSteps
+// l-steps; the 16 rows of one parity (l - m even pairs with Psi_E, odd with
+// Psi_O) are the A operands, and the Psi of the block's 64 ring pairs stay in
+// registers as B operands for the whole item.  A workgroup is kMmaWaves
+// waves on consecutive 64-pair blocks of one m, each starting at its own
+// first live l on a common grid of tiles; their products are added in LDS
+// in wave order, so one partial per kMmaWaves blocks goes to memory.
+constexpr int kMmaSteps = 32;
+constexpr int kMmaRow = 65;  // doubles per tile row: the 16 rows of one parity start 4 banks apart
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Rescale check of a scaled lane (every 8 steps), without branches.
+__device__ __forceinline__ void mma_rescale(double& mu1, double& mu2, int& sc, double& g, bool& all)
+{
+    const bool r = sc > 0 && fabs(mu1) > kScaleHi;
+    mu1 = r ? mu1 * kScaleDn : mu1;
+    mu2 = r ? mu2 * kScaleDn : mu2;
+    sc -= r ? 1 : 0;
+    g = sc == 0 ? 1.0 : 0.0;
+    all = __all(sc == 0);
+}
+
+// kMmaSteps recurrence steps of one ring pair into tile column `lane`: row s
+// holds lambda at l = c0 + s.  SCALED: times the gate, rescale checks every 8
+// steps.  Coefficients are read from LDS eight steps ahead of their use.
+template <bool SCALED>
+__device__ __forceinline__ void mma_fill(double* __restrict__ T, const double* __restrict__ cA, int lane, double x,
+                                         double& mu1, double& mu2, int& sc, double& g, bool& all)
+{
+    double c[8], cn[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) c[u] = cA[u];
+#pragma unroll
+    for (int s8 = 0; s8 < kMmaSteps; s8 += 8) {
+        if (s8 + 8 < kMmaSteps) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) cn[u] = cA[s8 + 8 + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const double nu = fma(c[u] * x, mu1, -mu2);
+            mu2 = mu1;
+            mu1 = nu;
+            T[(s8 + u) * kMmaRow + lane] = SCALED ? mu1 * g : mu1;
+        }
+        if (SCALED) mma_rescale(mu1, mu2, sc, g, all);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] = cn[u];
+    }
+}
+
+// The tile holding a wave's first live l at row r0: rows below it are zero,
+// row r0 is the start state itself, later rows advance.  (Rescale checks
+// before r0 find nothing: start states are at most 2^900.)
+__device__ __forceinline__ void mma_fill_first(double* __restrict__ T, const double* __restrict__ cA, int lane,
+                                               int r0, double x, double& mu1, double& mu2, int& sc, double& g,
+                                               bool& all)
+{
+#pragma unroll
+    for (int s = 0; s < kMmaSteps; ++s) {
+        const bool adv = s > r0;
+        const double nu = fma(cA[s] * x, mu1, -mu2);
+        mu2 = adv ? mu1 : mu2;
+        mu1 = adv ? nu : mu1;
+        T[s * kMmaRow + lane] = s >= r0 ? mu1 * g : 0.0;
+        if ((s & 7) == 7) mma_rescale(mu1, mu2, sc, g, all);
+    }
+}
+
+__global__ __launch_bounds__(64 * kMmaWaves, 2) void k_adj_mma(const WorkItem* __restrict__ items,
+                                                               const MmaSub* __restrict__ subs,
+                                                               const MapDesc* __restrict__ maps,
+                                                               const int* __restrict__ groups)
+{
+    __shared__ double T[kMmaWaves][kMmaSteps * kMmaRow];
+    __shared__ double sA[2][kMmaSteps];
+    const WorkItem it = items[blockIdx.x];
+    if (it.mcnt == 0) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int* __restrict__ gm = groups + kAdjG * it.map;
+    const MapDesc& md = maps[gm[0]];
+    const int L = md.L;
+    const int lend = it.lend;
+    const int m = it.m0;
+    const int l0 = it.lfirst;  // the earliest first live l of the workgroup's blocks
+    const long base = moff(L, m) - m;
+    gcdouble* __restrict__ A = (gcdouble*)(md.A + base);
+    const int lfw = subs[it.sidx].lf[w];  // this wave's first live l, -1: no live item
+    const int rb = it.rb * kMmaWaves + w;  // this wave's 64-pair block
+    double* __restrict__ Tw = T[w];
+
+    // the recurrence of ring pair rb * 64 + lane, from the wave's start state
+    // (a wave without a live item may lie past the ring tables: npair_pad
+    // need not be a multiple of kMmaWaves * 64)
+    double x = 0.0, mu1 = 0.0, mu2 = 0.0;
+    int sc = 1;
+    if (lfw >= 0) {
+        x = md.ring_x[rb * 64 + lane];
+        const long q = (long)subs[it.sidx].sidx[w] * 64 + lane;
+        mu1 = md.s5_cur[q];
+        mu2 = md.s5_prev[q];
+        sc = md.s5_k[q];
+    }
+    double g = sc == 0 ? 1.0 : 0.0;
+    bool all = __all(sc == 0);
+
+    // B operands: B[k][j] = Psi of ring pair rb * 64 + 4 b + k, column j =
+    // (map j / 2 of the group, re / im j % 2)
+    const int j = lane & 15, kq = lane >> 4;
+    const MapDesc& mj = maps[gm[j >> 1]];
+    double bE[16], bO[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) bE[b] = bO[b] = 0.0;
+    if (lfw >= 0) {
+        const long row = 4 * (long)(L + 1);
+        // padding pairs (past npair) read the last ring pair's row and are
+        // zeroed afterwards, so that all 32 loads are issued together
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const int pp = min(rb * 64 + 4 * b + kq, md.npair - 1);
+            gcdouble* src = (gcdouble*)(mj.Q + (long)pp * row + 4 * (long)m + (j & 1));
+            bE[b] = src[0];
+            bO[b] = src[2];
+        }
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const bool pad = rb * 64 + 4 * b + kq >= md.npair;
+            bE[b] = pad ? 0.0 : bE[b];
+            bO[b] = pad ? 0.0 : bO[b];
+        }
+    }
+    // this workgroup's partial (kMmaWaves blocks), written by wave 0
+    gdouble* __restrict__ dst = (gdouble*)(mj.upart + 2 * ((long)it.rb * md.nalm + base) + (j & 1));
+
+    // coefficients of the current tile in sA[buf], of the next one in tA
+    const int t = threadIdx.x;
+    double tA = 0.0;
+    if (t < kMmaSteps) {
+        sA[0][t] = (l0 + t <= lend) ? A[l0 + t] : 0.0;
+        tA = (l0 + kMmaSteps + t <= lend) ? A[l0 + kMmaSteps + t] : 0.0;
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int c0 = l0; c0 <= lend; c0 += kMmaSteps) {
+        const int n2 = c0 + 2 * kMmaSteps;
+        // tile rows past lend hold finite values that no stored output uses
+        const bool active = lfw >= 0 && lfw < c0 + kMmaSteps;
+        if (active) {
+            if (lfw >= c0)
+                mma_fill_first(Tw, sA[buf], lane, lfw - c0, x, mu1, mu2, sc, g, all);
+            else if (all)
+                mma_fill<false>(Tw, sA[buf], lane, x, mu1, mu2, sc, g, all);
+            else
+                mma_fill<true>(Tw, sA[buf], lane, x, mu1, mu2, sc, g, all);
+        }
+        wave_sync();
+        const int sE = (c0 - m) & 1;  // first tile row with l - m even
+        dbl4 dE = {0.0, 0.0, 0.0, 0.0}, dO = {0.0, 0.0, 0.0, 0.0};
+        if (active) {
+            // the ring-pair sums on the matrix cores, 16 l-rows of each parity
+            const double* tE = Tw + (sE + 2 * j) * kMmaRow + kq;
+            const double* tO = Tw + ((sE ^ 1) + 2 * j) * kMmaRow + kq;
+            double aE[16], aO[16];
+#pragma unroll
+            for (int b = 0; b < 16; ++b) {
+                aE[b] = tE[4 * b];
+                aO[b] = tO[4 * b];
+            }
+#pragma unroll
+            for (int b = 0; b < 16; ++b) {
+                dE = __builtin_amdgcn_mfma_f64_16x16x4f64(aE[b], bE[b], dE, 0, 0, 0);
+                dO = __builtin_amdgcn_mfma_f64_16x16x4f64(aO[b], bO[b], dO, 0, 0, 0);
+            }
+        }
+        // waves 1.. hand their sums to wave 0 through their own tile (its
+        // A operands are in registers; LDS runs a wave's operations in order)
+        if (w > 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                Tw[r * 64 + lane] = dE[r];
+                Tw[(4 + r) * 64 + lane] = dO[r];
+            }
+        }
+        __syncthreads();
+        if (w == 0) {
+#pragma unroll
+            for (int v = 1; v < kMmaWaves; ++v) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    dE[r] += T[v][r * 64 + lane];
+                    dO[r] += T[v][(4 + r) * 64 + lane];
+                }
+            }
+            // row i = kq + 4 r of each parity: l = c0 + s0 + 2 i
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int lE = c0 + sE + 2 * (kq + 4 * r);
+                const int lO = c0 + (sE ^ 1) + 2 * (kq + 4 * r);
+                if (lE <= lend) dst[2 * (long)lE] = dE[r];
+                if (lO <= lend) dst[2 * (long)lO] = dO[r];
+            }
+        }
+        if (t < kMmaSteps) {
+            sA[buf ^ 1][t] = tA;
+            tA = (n2 + t <= lend) ? A[n2 + t] : 0.0;
+        }
+        __syncthreads();
+        buf ^= 1;
+    }
+}
+
 // -------------------------------------------------------------- set-up passes
 // Both run the scaled recurrence from the seed lambda_mm.
 // k_scan: first l at which each ring pair becomes live (L+1 if never).
@@ -2846,6 +3070,14 @@ static int adj_k(const WorkItem* items, int n, const MapDesc* d_maps, hipStream_
 {
     if (n <= 0) return CMBL_OK;
     hipLaunchKernelGGL(k_adj<K>, dim3(n), dim3(64), 0, s, items, d_maps);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+int launch_adj_mma(const WorkItem* items, const MmaSub* subs, int n, const MapDesc* d_maps, const int* d_groups,
+                   hipStream_t s)
+{
+    if (n <= 0) return CMBL_OK;
+    hipLaunchKernelGGL(k_adj_mma, dim3(n), dim3(64 * kMmaWaves), 0, s, items, subs, d_maps, d_groups);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }

@@ -1436,6 +1436,27 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
             }
         }
     }
+    // maps of one geometry whose adjoint runs one ring pair per lane share it
+    // on the matrix cores in groups of kAdjG (CMBL_BATCH_MMA=0 turns this off)
+    std::vector<int> agroup_of(nmaps, -1);
+    std::vector<int> atable;  // kAdjG map indices per group
+    {
+        const char* env = getenv("CMBL_BATCH_MMA");
+        const bool on = !(env && env[0] == '0');
+        std::map<std::pair<int, int>, std::vector<int>> bya;
+        for (int i = 0; on && i < nmaps; ++i)
+            if (maps[i] && maps[i]->K5 == 1) bya[std::make_pair(maps[i]->nside, maps[i]->L)].push_back(i);
+        for (auto& kv : bya)
+            for (size_t at = 0; at + kAdjG <= kv.second.size(); at += kAdjG) {
+                const int gid = (int)(atable.size() / kAdjG);
+                for (int j = 0; j < kAdjG; ++j) {
+                    atable.push_back(kv.second[at + j]);
+                    agroup_of[kv.second[at + j]] = gid;
+                }
+            }
+    }
+    std::vector<ItemCost> wa;  // items of the shared adjoint, map = group, sidx = entry of subs
+    std::vector<MmaSub> subs;
     std::map<std::pair<int, int>, std::vector<ItemCost>> wg;  // (K, G) -> items
     for (int i = 0; i < nmaps; ++i) {
         cmbl_map* mp = maps[i];
@@ -1446,6 +1467,19 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
                                                   "entries would share one workspace)"));
         b->maps.push_back(mp);
         descs.push_back(mp->desc);
+        if (agroup_of[i] >= 0) {
+            // the shared adjoint writes one partial per kMmaWaves ring blocks
+            // (the last group may be partial), into the batch's own buffer
+            // (the map's is laid out per block)
+            const int nrb = (mp->npair_pad + 64 * kMmaWaves - 1) / (64 * kMmaWaves);
+            const size_t n = (size_t)nrb * mp->nalm * 2;
+            void* pb = nullptr;
+            if (hipMalloc(&pb, sizeof(double) * n) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
+            b->allocs.push_back(pb);
+            if (hipMemset(pb, 0, sizeof(double) * n) != hipSuccess) return cleanup(fail(CMBL_ERR_HIP, "batch"));
+            descs.back().upart = (double*)pb;
+            descs.back().nrb5 = nrb;
+        }
         int k1 = 0, k5 = 0;
         while ((1 << k1) < mp->K1) ++k1;
         while ((1 << k5) < mp->K5) ++k5;
@@ -1462,9 +1496,39 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
                 wg[gshape[gid]].push_back(ic);
             }
         }
-        for (ItemCost ic : mp->cost5) {
-            ic.it.map = i;
-            w5[k5].push_back(ic);
+        const int agid = agroup_of[i];
+        if (agid < 0) {
+            for (ItemCost ic : mp->cost5) {
+                ic.it.map = i;
+                w5[k5].push_back(ic);
+            }
+        } else if (atable[(size_t)kAdjG * agid] == i) {
+            // the group's first map brings its items, merged per (m, kMmaWaves
+            // consecutive ring blocks)
+            std::map<std::pair<int, int>, size_t> at;
+            for (const ItemCost& ic : mp->cost5) {
+                const std::pair<int, int> key(ic.it.m0, ic.it.rb / kMmaWaves);
+                auto f = at.find(key);
+                if (f == at.end()) {
+                    f = at.emplace(key, wa.size()).first;
+                    ItemCost g = ic;
+                    g.it.map = agid;
+                    g.it.rb = key.second;
+                    g.it.lend = mp->L;
+                    g.it.sidx = (int)subs.size();
+                    g.cost = 0.0;
+                    wa.push_back(g);
+                    MmaSub sb;
+                    for (int w = 0; w < kMmaWaves; ++w) sb.lf[w] = sb.sidx[w] = -1;
+                    subs.push_back(sb);
+                }
+                ItemCost& g = wa[f->second];
+                MmaSub& sb = subs[g.it.sidx];
+                sb.lf[ic.it.rb % kMmaWaves] = ic.it.lfirst;
+                sb.sidx[ic.it.rb % kMmaWaves] = ic.it.sidx;
+                g.it.lfirst = std::min(g.it.lfirst, ic.it.lfirst);
+                g.cost += ic.cost;
+            }
         }
         // the map's own two-ring workgroups are cut again for the batch's launches
         for (const RingItem& r : mp->ring_items) {
@@ -1510,6 +1574,25 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
         gl.n = (int)w.size();
         gl.d_items = (WorkItem*)q;
         b->syn_groups.push_back(gl);
+    }
+    if (!wa.empty()) {
+        const std::vector<WorkItem> w = order_items(wa);
+        if (hipMalloc(&q, sizeof(WorkItem) * w.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
+        b->allocs.push_back(q);
+        if (hipMemcpy(q, w.data(), sizeof(WorkItem) * w.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(CMBL_ERR_HIP, "batch upload"));
+        b->d_items_mma = (WorkItem*)q;
+        b->n_mma = (int)w.size();
+        if (hipMalloc(&q, sizeof(MmaSub) * subs.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
+        b->allocs.push_back(q);
+        if (hipMemcpy(q, subs.data(), sizeof(MmaSub) * subs.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(CMBL_ERR_HIP, "batch upload"));
+        b->d_subs_mma = (MmaSub*)q;
+        if (hipMalloc(&q, sizeof(int) * atable.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
+        b->allocs.push_back(q);
+        if (hipMemcpy(q, atable.data(), sizeof(int) * atable.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(CMBL_ERR_HIP, "batch upload"));
+        b->d_amaps = (int*)q;
     }
     if (!gtable.empty()) {
         // rows of four entries; a G = 2 group reads the first two of its row
@@ -1588,6 +1671,7 @@ int cmbl_batch_value_and_grad_device(cmbl_batch* b, const double* const* x, doub
     for (const cmbl_batch::SynGroupList& gl : b->syn_groups)
         CMBL_TRY(launch_synth_group(gl.K, gl.G, gl.d_items, gl.n, b->d_descs, b->d_gmaps, s));
     CMBL_TRY(run_ring(kRingLike, b->d_ring_items, b->ring_classes, b->d_descs, nullptr, nullptr, b->d_tw, s));
+    CMBL_TRY(launch_adj_mma(b->d_items_mma, b->d_subs_mma, b->n_mma, b->d_descs, b->d_amaps, s));
     for (int k = 0; k < 4; ++k) CMBL_TRY(launch_adj(1 << k, b->d_items5[k], b->n5[k], b->d_descs, s));
     for (size_t q = 0; q < groups.size(); ++q) {
         CMBL_TRY(launch_post_batch(b->d_descs + q * kBatchGroup, groups[q], kPostLike, s));

@@ -96,11 +96,10 @@ CONFIG1 = dict(nside=64, lmax=128, fwhm_deg=2.0, sigma=10.0, cut_deg=20.0)
 CONFIG2 = dict(nside=512, lmax=1024, fwhm_deg=20.0 / 60.0, sigma=30.0, cut_deg=20.0, holes=200, hole_deg=0.5)
 CONFIG4 = dict(nside=1024, lmax=2048, fwhm_deg=10.0 / 60.0, sigma=20.0, cut_deg=20.0)
 CONFIG5 = dict(nside=2048, lmax=4096, fwhm_deg=5.0 / 60.0, sigma=15.0, cut_deg=20.0)
-# The chain maps are evaluated 16 at a time (cmbl_batch), so their adjoint
-# takes 4 ring pairs per lane instead of the single-map table's 2: the batch
-# supplies the work items that K = 2 provides for a lone nside-256 map, and
-# the larger K halves the cross-lane reduction's share of the adjoint.
-CONFIG5_CHAIN = dict(nside=256, lmax=512, fwhm_deg=40.0 / 60.0, sigma=40.0, cut_deg=20.0, tuning=(2, 4, 4096))
+# The chain maps are evaluated 16 at a time (cmbl_batch) with one ring pair
+# per lane in the adjoint, so that the batch runs it on the matrix cores in
+# two groups of eight maps that share the recurrence.
+CONFIG5_CHAIN = dict(nside=256, lmax=512, fwhm_deg=40.0 / 60.0, sigma=40.0, cut_deg=20.0, tuning=(2, 1, 4096))
 
 
 def batch_configs(nmaps=64, seed=3):

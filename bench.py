@@ -59,6 +59,9 @@ def bench_batch(args):
         maps.append(mp)
         xs.append(torch.from_numpy(inp["x"]).to(dev))
         gs.append(torch.empty_like(xs[-1]))
+    # the per-map comparison runs each map with its single-map tuning (the
+    # tuned table) rather than the batch members' one ring pair per lane
+    singles = [build_case(dict(c, tuning=None), c["seed"])[0] for c in cfgs]
     f = torch.empty(len(maps), dtype=torch.float64, device=dev)
     batch = cm.Batch(maps)
     sp = torch.cuda.current_stream().cuda_stream
@@ -68,7 +71,7 @@ def bench_batch(args):
         batch.value_and_grad_device(xp, f.data_ptr(), gp, sp)
 
     def single():
-        for i, mp in enumerate(maps):
+        for i, mp in enumerate(singles):
             mp.value_and_grad_device(xp[i], f.data_ptr() + 8 * i, gp[i], sp)
 
     rates = {}
@@ -105,7 +108,7 @@ def bench_batch(args):
         "legendre_tflops_batched": round(flop * rates["batched"] / len(maps) / 1e12, 3),
     }
     batch.close()
-    for mp in maps:
+    for mp in maps + singles:
         mp.close()
     print(json.dumps(out), flush=True)
 

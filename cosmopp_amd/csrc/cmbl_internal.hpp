@@ -296,6 +296,7 @@ int launch_adj(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, 
 // first live l and start-state slot); item.rb counts blocks of kMmaWaves * 64
 // pairs, whose partials the maps' descriptors address (nrb5 of them).
 constexpr int kAdjG = 8;
+constexpr int kAdjMinG = 3;  // fewest maps of a group (the rest of its columns empty, entry -1)
 constexpr int kMmaWaves = 4;
 struct MmaSub {
     int lf[kMmaWaves];    // first live l of the wave's block, -1 if it has none

@@ -1829,13 +1829,15 @@ __global__ __launch_bounds__(64 * kMmaWaves, 2) void k_adj_mma(const WorkItem* _
     bool all = __all(sc == 0);
 
     // B operands: B[k][j] = Psi of ring pair rb * 64 + 4 b + k, column j =
-    // (map j / 2 of the group, re / im j % 2)
+    // (map j / 2 of the group, re / im j % 2); a group's empty columns
+    // (entry -1) hold zeros and are not stored
     const int j = lane & 15, kq = lane >> 4;
-    const MapDesc& mj = maps[gm[j >> 1]];
+    const int mapj = gm[j >> 1];
+    const MapDesc& mj = maps[mapj >= 0 ? mapj : gm[0]];
     double bE[16], bO[16];
 #pragma unroll
     for (int b = 0; b < 16; ++b) bE[b] = bO[b] = 0.0;
-    if (lfw >= 0) {
+    if (lfw >= 0 && mapj >= 0) {
         const long row = 4 * (long)(L + 1);
         // padding pairs (past npair) read the last ring pair's row and are
         // zeroed afterwards, so that all 32 loads are issued together
@@ -1920,8 +1922,8 @@ __global__ __launch_bounds__(64 * kMmaWaves, 2) void k_adj_mma(const WorkItem* _
             for (int r = 0; r < 4; ++r) {
                 const int lE = c0 + sE + 2 * (kq + 4 * r);
                 const int lO = c0 + (sE ^ 1) + 2 * (kq + 4 * r);
-                if (lE <= lend) dst[2 * (long)lE] = dE[r];
-                if (lO <= lend) dst[2 * (long)lO] = dO[r];
+                if (mapj >= 0 && lE <= lend) dst[2 * (long)lE] = dE[r];
+                if (mapj >= 0 && lO <= lend) dst[2 * (long)lO] = dO[r];
             }
         }
         if (t < kMmaSteps) {

@@ -1446,12 +1446,17 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
         std::map<std::pair<int, int>, std::vector<int>> bya;
         for (int i = 0; on && i < nmaps; ++i)
             if (maps[i] && maps[i]->K5 == 1) bya[std::make_pair(maps[i]->nside, maps[i]->L)].push_back(i);
+        // full groups, then the rest as one group with empty columns when it
+        // holds at least kAdjMinG maps (below that the one-pair-per-lane
+        // adjoint of each map is cheaper than the products' fixed cost)
         for (auto& kv : bya)
-            for (size_t at = 0; at + kAdjG <= kv.second.size(); at += kAdjG) {
+            for (size_t at = 0; at < kv.second.size(); at += kAdjG) {
+                const size_t n = std::min<size_t>(kAdjG, kv.second.size() - at);
+                if (n < (size_t)kAdjMinG) break;
                 const int gid = (int)(atable.size() / kAdjG);
-                for (int j = 0; j < kAdjG; ++j) {
-                    atable.push_back(kv.second[at + j]);
-                    agroup_of[kv.second[at + j]] = gid;
+                for (size_t j = 0; j < kAdjG; ++j) {
+                    atable.push_back(j < n ? kv.second[at + j] : -1);
+                    if (j < n) agroup_of[kv.second[at + j]] = gid;
                 }
             }
     }

@@ -105,14 +105,18 @@ CONFIG5_CHAIN = dict(nside=256, lmax=512, fwhm_deg=40.0 / 60.0, sigma=40.0, cut_
 def batch_configs(nmaps=64, seed=3):
     """64 maps of uneven size: lmax ~ U{256, 512, 768, 1024, 1536}, nside =
     2^round(log2(lmax/2)), FWHM = 5400/lmax arcmin, sigma ~ U[10, 50] uK,
-    cut ~ U[10, 30] deg, independent seeds."""
+    cut ~ U[10, 30] deg, independent seeds.  Built as batch members: the
+    tuned table's synthesis K (1 at lmax 256, else 2) and one ring pair per
+    lane in the adjoint, so that the batch runs the adjoint of each lmax
+    class on the matrix cores in groups of up to eight maps."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(nmaps):
         lmax = int(rng.choice([256, 512, 768, 1024, 1536]))
         nside = 1 << int(round(math.log2(lmax / 2)))
         out.append(dict(nside=nside, lmax=lmax, fwhm_deg=5400.0 / lmax / 60.0, sigma=float(rng.uniform(10, 50)),
-                        cut_deg=float(rng.uniform(10, 30)), seed=1000 + 10 * i))
+                        cut_deg=float(rng.uniform(10, 30)), seed=1000 + 10 * i,
+                        tuning=(1 if lmax <= 256 else 2, 1, 4096)))
     return out
 
 

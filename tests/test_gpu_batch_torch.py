@@ -69,6 +69,56 @@ def test_batch_several_launch_groups():
         mp.close()
 
 
+def test_batch_matrix_core_adjoint():
+    """Eleven maps of one geometry with one ring pair per lane in the adjoint:
+    a full group of eight and a partial group of three run the adjoint on the
+    matrix cores (nside 128: four 64-pair blocks per workgroup, polar blocks
+    starting at different l).  Each map equals its own evaluation, the batch
+    with the matrix-core adjoint switched off (CMBL_BATCH_MMA=0) and the
+    oracle."""
+    import os
+
+    import torch
+
+    print("case batch matrix-core adjoint nside 128 lmax 256", flush=True)
+    cases = [Case(128, 256, 20.0 / 60.0, 20.0 + i, 15.0 + i, seed=400 + i) for i in range(11)]
+    maps = []
+    for c in cases:
+        mp = cm.Map(c.nside, c.lmax, tuning=(1, 1, 4096))
+        mp.set_beam(c.beam)
+        mp.set_cl(c.cl)
+        mp.set_data(c.d, c.w)
+        maps.append(mp)
+    dev = torch.device("cuda:0")
+    xs = [torch.from_numpy(c.x).to(dev) for c in cases]
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for mma in ("1", "0"):
+        os.environ["CMBL_BATCH_MMA"] = mma
+        try:
+            b = cm.Batch(maps)
+        finally:
+            del os.environ["CMBL_BATCH_MMA"]
+        gs = [torch.empty_like(x) for x in xs]
+        f = torch.empty(len(cases), dtype=torch.float64, device=dev)
+        for _ in range(2):
+            b.value_and_grad_device([x.data_ptr() for x in xs], f.data_ptr(), [g.data_ptr() for g in gs], stream)
+        torch.cuda.synchronize()
+        out[mma] = (f.cpu().numpy(), [g.cpu().numpy() for g in gs])
+        b.close()
+    for i, (c, mp) in enumerate(zip(cases, maps)):
+        fs, gsn = mp.value_and_grad(c.x)
+        for mma in ("1", "0"):
+            fb, gb = out[mma]
+            assert abs(fb[i] - fs) <= 1e-13 * abs(fs)
+            assert rel(gb[i], gsn) <= 1e-13
+        fo, go = oracle.neg2lnL(c.nside, c.lmax, c.x, c.beam, c.cl, c.d, c.w)
+        assert abs(out["1"][0][i] - fo) / abs(fo) < 1e-10
+        assert rel(out["1"][1][i], go) < 1e-10
+    for mp in maps:
+        mp.close()
+
+
 def test_torch_autograd_pair():
     import torch
 

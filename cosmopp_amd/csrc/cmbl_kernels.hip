@@ -1711,8 +1711,9 @@ __global__ __launch_bounds__(64, K == 4 ? 3 : 1) void k_adj(const WorkItem* __re
 
 // -------------------------------------------------------------- shared adjoint on the matrix cores
 //
-// Maps of one geometry share lambda_lm(theta).  For a group of kAdjG = 8 such
-// maps the adjoint of one (m, block of 64 ring pairs) is a matrix product,
+// Maps of one geometry share lambda_lm(theta).  For a group of up to kAdjG = 8
+// such maps (a smaller group leaves columns empty) the adjoint of one
+// (m, block of 64 ring pairs) is a matrix product,
 //   U[l][c] = sum_p Lambda[l][p] Psi[p][c],  c = (map, re / im),
 // the ring-pair sum of 16 l-rows by 16 columns taking one FP64 matrix
 // instruction per 4 ring pairs (v_mfma_f64_16x16x4f64): the sum over ring

@@ -1472,6 +1472,17 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
                                                   "entries would share one workspace)"));
         b->maps.push_back(mp);
         descs.push_back(mp->desc);
+        // maps of one nside and split threshold have the same Bluestein
+        // chirps and filters (same rings, lengths and values): the batch's
+        // ring stage reads one copy, which stays in the caches
+        for (int j = 0; j < i; ++j)
+            if (maps[j]->nside == mp->nside && maps[j]->tuning.split_m == mp->tuning.split_m) {
+                descs.back().bs_filt = descs[j].bs_filt;
+                descs.back().bs_chirp = descs[j].bs_chirp;
+                descs.back().ring_bsofs = descs[j].ring_bsofs;
+                descs.back().ring_chofs = descs[j].ring_chofs;
+                break;
+            }
         if (agroup_of[i] >= 0) {
             // the shared adjoint writes one partial per kMmaWaves ring blocks
             // (the last group may be partial), into the batch's own buffer

@@ -48,8 +48,10 @@ def test_graph_replay_matches_eager(nside, lmax):
 
 def test_batch_graph_replay_matches_eager():
     """A batch of uneven maps (two of one geometry, so the shared-recurrence
-    synthesis runs too) captured in one HIP graph: every replay, with new
-    inputs and eager work in between, equals the eager batch bit for bit."""
+    synthesis runs too, and three of another with one ring pair per lane in
+    the adjoint, which run it on the matrix cores) captured in one HIP graph:
+    every replay, with new inputs and eager work in between, equals the eager
+    batch bit for bit."""
     import torch
 
     import cosmopp_amd as cm
@@ -58,6 +60,14 @@ def test_batch_graph_replay_matches_eager():
     cases = [Case(16, 40, 2.0, 10.0, 20.0, seed=31), Case(32, 64, 2.0, 10.0, 20.0, seed=32),
              Case(32, 64, 2.0, 10.0, 20.0, seed=33)]
     maps = [c.make_map() for c in cases]
+    for seed in (34, 35, 36):
+        c = Case(64, 96, 2.0, 10.0, 20.0, seed=seed)
+        mp = cm.Map(c.nside, c.lmax, tuning=(1, 1, 4096))
+        mp.set_beam(c.beam)
+        mp.set_cl(c.cl)
+        mp.set_data(c.d, c.w)
+        cases.append(c)
+        maps.append(mp)
     batch = cm.Batch(maps)
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(6)

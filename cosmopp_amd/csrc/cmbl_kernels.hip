@@ -138,23 +138,45 @@ __device__ __forceinline__ void pre_tile(const MapDesc& md, const double* __rest
     int tl, tm;
     tile_of(tile, tl, tm);
     const int l0 = tl * kLM, m0 = tm * kLM;
-    for (int i = threadIdx.x; i < kLM * 2 * kLM; i += 256) {
+    // all loads of a thread are issued before their first use (entries
+    // outside the triangle read entry 0 and are dropped)
+    constexpr int kX = kLM * 2 * kLM / 256, kE = kLM * kLM / 256;
+    double xv[kX];
+#pragma unroll
+    for (int e = 0; e < kX; ++e) {
+        const int i = threadIdx.x + 256 * e;
         const int lr = i / (2 * kLM), c = i % (2 * kLM);
         const int l = l0 + lr, m = m0 + (c >> 1);
         const long pi = (l <= L && m <= l) ? packed_index(l, m, c & 1) : -1;
-        T[lr][c] = pi >= 0 ? x[pi] : 0.0;
+        xv[e] = gview(x)[pi >= 0 ? pi : 0];
+        xv[e] = pi >= 0 ? xv[e] : 0.0;
+    }
+    double sc[kE], ic[kE];
+#pragma unroll
+    for (int e = 0; e < kE; ++e) {
+        const int i = threadIdx.x + 256 * e;
+        const int l = l0 + i % kLM, m = m0 + i / kLM;
+        const bool ok = l <= L && m <= l;
+        sc[e] = gview(md.beam)[ok ? l : 0] * gview(md.Nrm)[ok ? moff(L, m) + (l - m) : 0];
+        ic[e] = gview(md.invcl)[ok ? l : 0];
+    }
+#pragma unroll
+    for (int e = 0; e < kX; ++e) {
+        const int i = threadIdx.x + 256 * e;
+        T[i / (2 * kLM)][i % (2 * kLM)] = xv[e];
     }
     __syncthreads();
     double pr = 0.0;
-    for (int i = threadIdx.x; i < kLM * kLM; i += 256) {
+#pragma unroll
+    for (int e = 0; e < kE; ++e) {
+        const int i = threadIdx.x + 256 * e;
         const int mr = i / kLM, lr = i % kLM;
         const int l = l0 + lr, m = m0 + mr;
         if (l <= L && m <= l) {
             const double re = T[lr][2 * mr], im = T[lr][2 * mr + 1];
             const long k = moff(L, m) + (l - m);
-            const double sc = md.beam[l] * md.Nrm[k];
-            md.alm[k] = make_double2(sc * re, sc * im);
-            pr += md.invcl[l] * ((m == 0) ? re * re : 2.0 * (re * re + im * im));
+            md.alm[k] = make_double2(sc[e] * re, sc[e] * im);
+            pr += ic[e] * ((m == 0) ? re * re : 2.0 * (re * re + im * im));
         }
     }
     const double t = block_sum<256>(pr, sh);
@@ -192,37 +214,62 @@ __device__ __forceinline__ void post_tile(const MapDesc& md, const double* __res
     int tl, tm;
     tile_of(tile, tl, tm);
     const int l0 = tl * kLM, m0 = tm * kLM;
-    for (int i = threadIdx.x; i < kLM * kLM; i += 256) {
-        const int mr = i / kLM, lr = i % kLM;
-        const int l = l0 + lr, m = m0 + mr;
-        double ur = 0.0, ui = 0.0;
-        if (l <= L && m <= l) {
-            const long k = moff(L, m) + (l - m);
-            const double2* u = reinterpret_cast<const double2*>(md.upart) + k;
-            const long stride = md.nalm;  // double2 per ring block
-            int rb = 0;
-            // eight partials in flight per thread, added in ring-block order
-            for (; rb + 8 <= md.nrb5; rb += 8) {
-                double2 v[8];
+    constexpr int kE = kLM * kLM / 256;  // elements per thread
+    // every load of the thread's elements is issued before the sums (entries
+    // outside the triangle read entry 0 and are dropped)
+    long k[kE];
+    bool ok[kE];
+    double sc[kE];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) v[j] = u[(rb + j) * stride];
+    for (int e = 0; e < kE; ++e) {
+        const int i = threadIdx.x + 256 * e;
+        const int l = l0 + i % kLM, m = m0 + i / kLM;
+        ok[e] = l <= L && m <= l;
+        k[e] = ok[e] ? moff(L, m) + (l - m) : 0;
+        const double s = (mode == kPostAddAlm || mode == kPostQuad) ? c : gview(md.beam)[ok[e] ? l : 0];
+        sc[e] = gview(md.Nrm)[k[e]] * s;
+    }
+    const GCView2 u = gview(reinterpret_cast<const double2*>(md.upart));
+    const long stride = md.nalm;  // double2 per ring block
+    double ur[kE], ui[kE];
+#pragma unroll
+    for (int e = 0; e < kE; ++e) ur[e] = ui[e] = 0.0;
+    // eight partials of two elements in flight per thread, added in
+    // ring-block order
+#pragma unroll
+    for (int e0 = 0; e0 < kE; e0 += 2) {
+        int rb = 0;
+        for (; rb + 8 <= md.nrb5; rb += 8) {
+            double2 v[2][8];
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[e][j] = u[k[e0 + e] + (rb + j) * stride];
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                    ur += v[j].x;
-                    ui += v[j].y;
+                    ur[e0 + e] += v[e][j].x;
+                    ui[e0 + e] += v[e][j].y;
                 }
-            }
-            for (; rb < md.nrb5; ++rb) {
-                const double2 v = u[rb * stride];
-                ur += v.x;
-                ui += v.y;
-            }
-            const double sc = md.Nrm[k] * (mode == kPostAddAlm || mode == kPostQuad ? c : md.beam[l]);
-            ur *= sc;
-            ui *= sc;
         }
-        T[lr][2 * mr] = ur;
-        T[lr][2 * mr + 1] = ui;
+        for (; rb < md.nrb5; ++rb) {
+            double2 v[2];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) v[e] = u[k[e0 + e] + rb * stride];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                ur[e0 + e] += v[e].x;
+                ui[e0 + e] += v[e].y;
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < kE; ++e) {
+        const int i = threadIdx.x + 256 * e;
+        const int mr = i / kLM, lr = i % kLM;
+        T[lr][2 * mr] = ok[e] ? ur[e] * sc[e] : 0.0;
+        T[lr][2 * mr + 1] = ok[e] ? ui[e] * sc[e] : 0.0;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kLM * 2 * kLM; i += 256) {

@@ -75,6 +75,32 @@ __device__ double final_replay(const double* part, double* sh16)
 // elements before applying them in their grid-stride order, so more loads
 // are in flight while the accumulation order stays that of the plain dot
 // product.
+// The element loop of a pass from element i on (the thread's elements are
+// i, i + stride, ...), accumulating into a; then the block sums.
+template <class Op>
+__device__ __forceinline__ void pass_loop(Op& op, long i, double* a)
+{
+    const long stride = (long)gridDim.x * kPassThreads;
+    for (; i + 3 * stride < op.n; i += 4 * stride) {
+        typename Op::In v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = op.fetch(i + u * stride);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) op.apply(i + u * stride, v[u], a);
+    }
+    for (; i < op.n; i += stride) op.apply(i, op.fetch(i), a);
+}
+template <class Op>
+__device__ __forceinline__ void pass_store(const double* a, const PassSync& ps, double* sh)
+{
+    constexpr int ND = Op::kDots;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const double t = block_sum<kPassThreads>(a[d], sh);
+        if (threadIdx.x == 0) ps.part[d * kDotParts + blockIdx.x] = t;
+    }
+}
+
 template <class Op>
 __global__ __launch_bounds__(kPassThreads) void k_pass(Op op, PassSync ps)
 {
@@ -84,22 +110,47 @@ __global__ __launch_bounds__(kPassThreads) void k_pass(Op op, PassSync ps)
     double a[ND > 0 ? ND : 1];
 #pragma unroll
     for (int d = 0; d < (ND > 0 ? ND : 1); ++d) a[d] = 0.0;
+    pass_loop(op, blockIdx.x * (long)kPassThreads + threadIdx.x, a);
+    if (ND > 0) pass_store<Op>(a, ps, sh);
+}
+
+// A pass chained to its predecessor (a one-product pass of the two-loop
+// recursion): every block replays the predecessor's dot product from its
+// block sums `prev`, exactly as the predecessor's final stage would, while
+// its first elements' loads are in flight; Op::scalar turns the product into
+// the scalar this pass needs, the first block also storing what that final
+// stage stored.  The final stage's launch, and its wait between two passes,
+// are saved; `prev` and ps.part are different buffers.
+template <class Op>
+__global__ __launch_bounds__(kPassThreads) void k_pass_chained(Op op, PassSync ps, const double* __restrict__ prev)
+{
+    constexpr int ND = Op::kDots;
+    __shared__ double sh[kPassThreads / 64];
+    __shared__ double sh16[kDotParts / 64];
+    __shared__ double rprev;
+    op.load_pointers();
     const long stride = (long)gridDim.x * kPassThreads;
     long i = blockIdx.x * (long)kPassThreads + threadIdx.x;
-    for (; i + 3 * stride < op.n; i += 4 * stride) {
-        typename Op::In v[4];
+    const bool full = i + 3 * stride < op.n;
+    typename Op::In v[4];
+    if (full) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) v[u] = op.fetch(i + u * stride);
+    }
+    const double r = final_replay(prev, sh16);
+    if (threadIdx.x == 0) rprev = r;
+    __syncthreads();
+    op.scalar(rprev, blockIdx.x == 0 && threadIdx.x == 0);
+    double a[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) a[d] = 0.0;
+    if (full) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) op.apply(i + u * stride, v[u], a);
+        i += 4 * stride;
     }
-    for (; i < op.n; i += stride) op.apply(i, op.fetch(i), a);
-    if (ND == 0) return;
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-        const double t = block_sum<kPassThreads>(a[d], sh);
-        if (threadIdx.x == 0) ps.part[d * kDotParts + blockIdx.x] = t;
-    }
+    pass_loop(op, i, a);
+    pass_store<Op>(a, ps, sh);
 }
 
 // The pass's final stage, one block: the kDots results from the block sums,
@@ -123,6 +174,26 @@ static int run_pass(const Op& op, const PassSync& ps, hipStream_t s)
 {
     hipLaunchKernelGGL(k_pass<Op>, dim3(kDotParts), dim3(kPassThreads), 0, s, op, ps);
     if (Op::kDots > 0) hipLaunchKernelGGL(k_pass_final<Op>, dim3(1), dim3(kPassThreads), 0, s, op, ps);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+
+// The pieces of a chain of passes: the first pass without its final stage,
+// the chained passes, and the last pass's final stage.
+template <class Op>
+static int run_pass_open(const Op& op, const PassSync& ps, const double* prev, hipStream_t s)
+{
+    if (prev)
+        hipLaunchKernelGGL(k_pass_chained<Op>, dim3(kDotParts), dim3(kPassThreads), 0, s, op, ps, prev);
+    else
+        hipLaunchKernelGGL(k_pass<Op>, dim3(kDotParts), dim3(kPassThreads), 0, s, op, ps);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+template <class Op>
+static int run_pass_close(const Op& op, const PassSync& ps, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pass_final<Op>, dim3(1), dim3(kPassThreads), 0, s, op, ps);
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }
@@ -244,6 +315,23 @@ struct OpLoop1 {
         c = -d.alpha[ki];
         h = *d.h0k;
     }
+    // Chained to step i - 1 (i >= 1), whose final stage stores
+    // alpha_i = rho_i s_i . q (its product r).
+    __device__ void load_pointers()
+    {
+        ki = d.slot[i];
+        kn = i + 1 < mu ? d.slot[i + 1] : -1;
+        qin = d.q;
+        yi = d.Y + (long)ki * d.n;
+        sn = kn >= 0 ? d.S + (long)kn * d.n : nullptr;
+        h = *d.h0k;
+    }
+    __device__ void scalar(double r, bool writer)
+    {
+        const double a = d.rho[ki] * r;
+        if (writer) d.alpha[ki] = a;
+        c = -a;
+    }
     struct In {
         double y, q, s;
     };
@@ -281,14 +369,31 @@ struct OpLoop2 {
     int i;
     const double *si, *yp;
     double c;
-    int kp;
+    int ki, kp;
     __device__ void load()
     {
-        const int ki = d.slot[i];
+        ki = d.slot[i];
         kp = i > 0 ? d.slot[i - 1] : -1;
         si = d.S + (long)ki * d.n;
         yp = kp >= 0 ? d.Y + (long)kp * d.n : nullptr;
         c = d.coef[ki];
+    }
+    // Chained to the first loop's last step or to step i + 1 of this loop,
+    // whose final stage stores coef_i = alpha_i - rho_i r (r: its product
+    // y_i . z).
+    __device__ void load_pointers()
+    {
+        ki = d.slot[i];
+        kp = i > 0 ? d.slot[i - 1] : -1;
+        si = d.S + (long)ki * d.n;
+        yp = kp >= 0 ? d.Y + (long)kp * d.n : nullptr;
+    }
+    __device__ void scalar(double r, bool writer)
+    {
+        const double beta = d.rho[ki] * r;
+        const double cc = d.alpha[ki] - beta;
+        if (writer) d.coef[ki] = cc;
+        c = cc;
     }
     struct In {
         double s, z, o;  // o: y_{i-1}, or g at i = 0
@@ -508,20 +613,32 @@ static int lb_direction(cmbl_lbfgs* lb, bool pending, int mu, hipStream_t s)
     const LbDev& d = lb->d;
     if (pending) CMBL_TRY(run_pass(OpHistory{d, d.n, nullptr, nullptr}, lb->ps, s));
     if (mu == 0) return run_pass(OpDir0{d, d.n, 0.0}, lb->ps, s);
+    // the 2 mu passes form one chain: each pass replays its predecessor's
+    // product (block sums alternating between two buffers), and only the
+    // last one has a final stage of its own
+    const double* prev = nullptr;
+    int buf = 0;
     for (int i = 0; i < mu; ++i) {
         OpLoop1 op{};
         op.d = d;
         op.n = d.n;
         op.i = i;
         op.mu = mu;
-        CMBL_TRY(run_pass(op, lb->ps, s));
+        const PassSync ps{lb->ps.part + (long)buf * kMaxPassDots * kDotParts};
+        CMBL_TRY(run_pass_open(op, ps, prev, s));
+        prev = ps.part;
+        buf ^= 1;
     }
     for (int i = mu - 1; i >= 0; --i) {
         OpLoop2 op{};
         op.d = d;
         op.n = d.n;
         op.i = i;
-        CMBL_TRY(run_pass(op, lb->ps, s));
+        const PassSync ps{lb->ps.part + (long)buf * kMaxPassDots * kDotParts};
+        CMBL_TRY(run_pass_open(op, ps, prev, s));
+        prev = ps.part;
+        buf ^= 1;
+        if (i == 0) CMBL_TRY(run_pass_close(op, ps, s));
     }
     return CMBL_OK;
 }
@@ -592,7 +709,7 @@ int cmbl_lbfgs_create(cmbl_lbfgs** out, cmbl_map* map, const double* d_x0, int m
         (rc = lb->mem.dalloc(&d.slot, m)) || (rc = lb->mem.dalloc(&d.rho, m)) ||
         (rc = lb->mem.dalloc(&d.alpha, m)) || (rc = lb->mem.dalloc(&d.coef, m)) ||
         (rc = lb->mem.dalloc(&d.h0k, 1)) || (rc = lb->mem.dalloc(&lb->d_f, 1)) ||
-        (rc = lb->mem.dalloc(&lb->ps.part, (size_t)kMaxPassDots * kDotParts)))
+        (rc = lb->mem.dalloc(&lb->ps.part, (size_t)2 * kMaxPassDots * kDotParts)))
         return bail(rc);
     if ((rc = lb->mem.halloc(&lb->h, &d.h, 1))) return bail(rc);
     d.f = lb->d_f;

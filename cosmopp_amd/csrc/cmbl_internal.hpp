@@ -297,11 +297,17 @@ int launch_adj(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, 
 // pairs, whose partials the maps' descriptors address (nrb5 of them).
 constexpr int kAdjG = 8;
 constexpr int kAdjMinG = 3;  // fewest maps of a group (the rest of its columns empty, entry -1)
+constexpr int kSynMinG = 6;  // fewest maps of a matrix-core synthesis group (fewer share k_synth_g)
 constexpr int kMmaWaves = 4;
 struct MmaSub {
     int lf[kMmaWaves];    // first live l of the wave's block, -1 if it has none
     int sidx[kMmaWaves];  // its start-state slot
 };
+// Shared synthesis of groups of up to kAdjG maps of one geometry (ring
+// pairs per lane K = 1 or 2) on the FP64 matrix cores; d_groups: kAdjG map
+// indices per group (-1: empty).
+int launch_synth_mma(int K, const WorkItem* items, int nitems, const MapDesc* d_maps, const int* d_groups,
+                     hipStream_t s);
 int launch_adj_mma(const WorkItem* items, const MmaSub* subs, int nitems, const MapDesc* d_maps, const int* d_groups,
                    hipStream_t s);
 int lm_tiles(int L);  // blocks of k_pre / k_post (prior partial sums)
@@ -440,6 +446,10 @@ struct cmbl_batch {
     int ngroups = 0;
     // shared adjoint on the matrix cores: groups of kAdjG maps of one
     // geometry with one ring pair per lane, their items and group table
+    // matrix-core synthesis: groups of up to kAdjG maps of one geometry, items by K - 1
+    cmbl::WorkItem* d_items_smma[2] = {nullptr, nullptr};
+    int n_smma[2] = {0, 0};
+    int* d_smaps = nullptr;
     cmbl::WorkItem* d_items_mma = nullptr;
     cmbl::MmaSub* d_subs_mma = nullptr;
     int n_mma = 0;

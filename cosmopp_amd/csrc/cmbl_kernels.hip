@@ -1795,7 +1795,7 @@ __device__ __forceinline__ void mma_rescale(double& mu1, double& mu2, int& sc, d
 // kMmaSteps recurrence steps of one ring pair into tile column `lane`: row s
 // holds lambda at l = c0 + s.  SCALED: times the gate, rescale checks every 8
 // steps.  Coefficients are read from LDS eight steps ahead of their use.
-template <bool SCALED>
+template <bool SCALED, int RS = kMmaRow>
 __device__ __forceinline__ void mma_fill(double* __restrict__ T, const double* __restrict__ cA, int lane, double x,
                                          double& mu1, double& mu2, int& sc, double& g, bool& all)
 {
@@ -1813,7 +1813,7 @@ __device__ __forceinline__ void mma_fill(double* __restrict__ T, const double* _
             const double nu = fma(c[u] * x, mu1, -mu2);
             mu2 = mu1;
             mu1 = nu;
-            T[(s8 + u) * kMmaRow + lane] = SCALED ? mu1 * g : mu1;
+            T[(s8 + u) * RS + lane] = SCALED ? mu1 * g : mu1;
         }
         if (SCALED) mma_rescale(mu1, mu2, sc, g, all);
 #pragma unroll
@@ -1824,6 +1824,7 @@ __device__ __forceinline__ void mma_fill(double* __restrict__ T, const double* _
 // The tile holding a wave's first live l at row r0: rows below it are zero,
 // row r0 is the start state itself, later rows advance.  (Rescale checks
 // before r0 find nothing: start states are at most 2^900.)
+template <int RS = kMmaRow>
 __device__ __forceinline__ void mma_fill_first(double* __restrict__ T, const double* __restrict__ cA, int lane,
                                                int r0, double x, double& mu1, double& mu2, int& sc, double& g,
                                                bool& all)
@@ -1834,7 +1835,7 @@ __device__ __forceinline__ void mma_fill_first(double* __restrict__ T, const dou
         const double nu = fma(cA[s] * x, mu1, -mu2);
         mu2 = adv ? mu1 : mu2;
         mu1 = adv ? nu : mu1;
-        T[s * kMmaRow + lane] = s >= r0 ? mu1 * g : 0.0;
+        T[s * RS + lane] = s >= r0 ? mu1 * g : 0.0;
         if ((s & 7) == 7) mma_rescale(mu1, mu2, sc, g, all);
     }
 }
@@ -1980,6 +1981,131 @@ __global__ __launch_bounds__(64 * kMmaWaves, 2) void k_adj_mma(const WorkItem* _
         }
         __syncthreads();
         buf ^= 1;
+    }
+}
+
+// -------------------------------------------------------------- shared synthesis on the matrix cores
+//
+// The synthesis of a group of up to kAdjG = 8 maps of one geometry (a smaller
+// group leaves columns empty) as matrix products: for a block of 64 ring
+// pairs, P[p][c] = sum_l Lambda[l][p] a_l[c] per parity, c = (map, re / im),
+// accumulated over the item's l range in registers.  Each lane runs the
+// recurrence of one ring pair (once for the group) into an LDS tile of
+// kMmaSteps l-steps as in k_adj_mma; an instruction takes 16 ring pairs by 4
+// l-steps of one parity (A, read across the tile's rows) times those 4
+// l-steps' a_lm of the 8 maps (B, from global memory, a tile ahead).  A tile
+// is 2 parities x 4 groups of 4 l-steps x 4 blocks of 16 pairs = 32
+// instructions; the item's sums (E, O per pair and map) are stored once at
+// its end.  No sums cross waves, so a workgroup is one wave; an item of K
+// ring pairs per lane runs as K workgroups, all from the item's first l.
+constexpr int kMmaRowS = 72;  // tile row stride: rows two apart start 32 banks apart
+
+template <int K>
+__global__ __launch_bounds__(64, 2) void k_synth_mma(const WorkItem* __restrict__ items,
+                                                     const MapDesc* __restrict__ maps, const int* __restrict__ groups)
+{
+    __shared__ double T[kMmaSteps * kMmaRowS];
+    __shared__ double sA[2][kMmaSteps];
+    const WorkItem it = items[blockIdx.x / K];
+    if (it.mcnt == 0) return;
+    const int w = blockIdx.x % K;
+    const int* __restrict__ gm = groups + kAdjG * it.map;
+    const MapDesc& md = maps[gm[0]];
+    const int L = md.L;
+    const int m = it.m0;
+    const int l0 = it.lfirst;
+    const int lane = threadIdx.x;
+    const long base = moff(L, m) - m;
+    gcdouble* __restrict__ A = (gcdouble*)(md.A + base);
+    const int pbase = it.rb * 64 * K + 64 * w;  // this wave's 64 ring pairs
+
+    // the recurrence of ring pair pbase + lane, from the item's start state
+    const long q = (long)m * md.npair_pad + pbase + lane;
+    const double x = md.ring_x[pbase + lane];
+    double mu1 = md.s1_cur[q], mu2 = md.s1_prev[q];
+    int sc = md.s1_k[q];
+    double g = sc == 0 ? 1.0 : 0.0;
+    bool all = __all(sc == 0);
+
+    // B operands: B[k][j] = a_l of column j = (map j / 2, re / im j % 2) at
+    // the k-th l-step of a group; empty columns hold zeros and are not stored
+    const int j = lane & 15, kq = lane >> 4;
+    const int mapj = gm[j >> 1];
+    gcdouble* __restrict__ aj = (gcdouble*)(maps[mapj >= 0 ? mapj : gm[0]].alm + base) + (j & 1);
+    auto load_b = [&](int c0, double(&bE)[4], double(&bO)[4]) {
+        const int sE = (c0 - m) & 1;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int lE = c0 + sE + 2 * (4 * h + kq), lO = c0 + (sE ^ 1) + 2 * (4 * h + kq);
+            const double e = aj[2 * (long)min(lE, L)], o = aj[2 * (long)min(lO, L)];
+            bE[h] = (mapj >= 0 && lE <= L) ? e : 0.0;
+            bO[h] = (mapj >= 0 && lO <= L) ? o : 0.0;
+        }
+    };
+    double bE[4], bO[4], nE[4], nO[4];
+    load_b(l0, bE, bO);
+
+    dbl4 dE[4], dO[4];
+#pragma unroll
+    for (int pb = 0; pb < 4; ++pb) dE[pb] = dO[pb] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    // coefficients of the current tile in sA[buf], of the next one in tA
+    double tA = 0.0;
+    if (lane < kMmaSteps) {
+        sA[0][lane] = (l0 + lane <= L) ? A[l0 + lane] : 0.0;
+        tA = (l0 + kMmaSteps + lane <= L) ? A[l0 + kMmaSteps + lane] : 0.0;
+    }
+    wave_sync();
+    int buf = 0;
+    for (int c0 = l0; c0 <= L; c0 += kMmaSteps) {
+        const int n2 = c0 + 2 * kMmaSteps;
+        if (c0 + kMmaSteps <= L) load_b(c0 + kMmaSteps, nE, nO);
+        // tile rows past L hold finite values whose B operands are zero
+        if (c0 == l0)
+            mma_fill_first<kMmaRowS>(T, sA[buf], lane, 0, x, mu1, mu2, sc, g, all);
+        else if (all)
+            mma_fill<false, kMmaRowS>(T, sA[buf], lane, x, mu1, mu2, sc, g, all);
+        else
+            mma_fill<true, kMmaRowS>(T, sA[buf], lane, x, mu1, mu2, sc, g, all);
+        wave_sync();
+        const int sE = (c0 - m) & 1;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const double* tE = T + (sE + 2 * (4 * h + kq)) * kMmaRowS + j;
+            const double* tO = T + ((sE ^ 1) + 2 * (4 * h + kq)) * kMmaRowS + j;
+#pragma unroll
+            for (int pb = 0; pb < 4; ++pb) {
+                dE[pb] = __builtin_amdgcn_mfma_f64_16x16x4f64(tE[16 * pb], bE[h], dE[pb], 0, 0, 0);
+                dO[pb] = __builtin_amdgcn_mfma_f64_16x16x4f64(tO[16 * pb], bO[h], dO[pb], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            bE[h] = nE[h];
+            bO[h] = nO[h];
+        }
+        if (lane < kMmaSteps) {
+            sA[buf ^ 1][lane] = tA;
+            tA = (n2 + lane <= L) ? A[n2 + lane] : 0.0;
+        }
+        wave_sync();
+        buf ^= 1;
+    }
+    // D[i][j] lands in lane j + 16 (i % 4), register i / 4: ring pair
+    // pbase + 16 pb + kq + 4 r
+    if (mapj >= 0) {
+        gdouble* __restrict__ P = (gdouble*)maps[mapj].P;
+#pragma unroll
+        for (int pb = 0; pb < 4; ++pb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int pair = pbase + 16 * pb + kq + 4 * r;
+                if (pair < md.npair) {
+                    const long o = ((long)pair * (L + 1) + m) * 4 + (j & 1);
+                    P[o] = dE[pb][r];
+                    P[o + 2] = dO[pb][r];
+                }
+            }
     }
 }
 
@@ -3128,6 +3254,18 @@ int launch_adj_mma(const WorkItem* items, const MmaSub* subs, int n, const MapDe
 {
     if (n <= 0) return CMBL_OK;
     hipLaunchKernelGGL(k_adj_mma, dim3(n), dim3(64 * kMmaWaves), 0, s, items, subs, d_maps, d_groups);
+    CMBL_HIP(hipGetLastError());
+    return CMBL_OK;
+}
+int launch_synth_mma(int K, const WorkItem* items, int n, const MapDesc* d_maps, const int* d_groups, hipStream_t s)
+{
+    if (n <= 0) return CMBL_OK;
+    if (K == 1)
+        hipLaunchKernelGGL(k_synth_mma<1>, dim3(n), dim3(64), 0, s, items, d_maps, d_groups);
+    else if (K == 2)
+        hipLaunchKernelGGL(k_synth_mma<2>, dim3(2 * n), dim3(64), 0, s, items, d_maps, d_groups);
+    else
+        return fail(CMBL_ERR_ARG, "unsupported rings-per-lane K for the matrix-core synthesis");
     CMBL_HIP(hipGetLastError());
     return CMBL_OK;
 }

@@ -1411,9 +1411,16 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
     std::vector<RingItem> rings;
     // maps of one geometry (nside, lmax, synthesis ring pairs per lane) share
     // the synthesis recurrence in groups of four (then two)
+    // (groups of up to kAdjG such maps, at least kSynMinG, run the synthesis
+    // on the matrix cores first; CMBL_BATCH_MMA=0 turns that off)
+    const char* mma_env = getenv("CMBL_BATCH_MMA");
+    const bool mma_on = !(mma_env && mma_env[0] == '0');
     std::vector<int> group_of(nmaps, -1);
     std::vector<int> gtable;                   // G map indices per group
     std::vector<std::pair<int, int>> gshape;   // (K, G) per group
+    std::vector<int> sgroup_of(nmaps, -1);
+    std::vector<int> stable;                   // kAdjG map indices per matrix-core synthesis group (-1: empty)
+    std::vector<int> sK;                       // its ring pairs per lane
     {
         std::map<std::tuple<int, int, int>, std::vector<int>> byg;
         for (int i = 0; i < nmaps; ++i)
@@ -1422,6 +1429,16 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
             const std::vector<int>& v = kv.second;
             const int K = std::get<2>(kv.first);
             size_t at = 0;
+            while (mma_on && (K == 1 || K == 2) && v.size() - at >= (size_t)kSynMinG) {
+                const size_t n = std::min<size_t>(kAdjG, v.size() - at);
+                const int sid = (int)sK.size();
+                sK.push_back(K);
+                for (size_t j = 0; j < (size_t)kAdjG; ++j) {
+                    stable.push_back(j < n ? v[at + j] : -1);
+                    if (j < n) sgroup_of[v[at + j]] = sid;
+                }
+                at += n;
+            }
             for (int G : {4, 2}) {
                 const bool ok = (G == 4 && (K == 1 || K == 2)) || (G == 2 && K <= 4);
                 for (; ok && at + G <= v.size(); at += G) {
@@ -1441,10 +1458,8 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
     std::vector<int> agroup_of(nmaps, -1);
     std::vector<int> atable;  // kAdjG map indices per group
     {
-        const char* env = getenv("CMBL_BATCH_MMA");
-        const bool on = !(env && env[0] == '0');
         std::map<std::pair<int, int>, std::vector<int>> bya;
-        for (int i = 0; on && i < nmaps; ++i)
+        for (int i = 0; mma_on && i < nmaps; ++i)
             if (maps[i] && maps[i]->K5 == 1) bya[std::make_pair(maps[i]->nside, maps[i]->L)].push_back(i);
         // full groups, then the rest as one group with empty columns when it
         // holds at least kAdjMinG maps (below that the one-pair-per-lane
@@ -1460,6 +1475,7 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
                 }
             }
     }
+    std::vector<ItemCost> ws[2];  // items of the matrix-core synthesis by K - 1, map = its group
     std::vector<ItemCost> wa;  // items of the shared adjoint, map = group, sidx = entry of subs
     std::vector<MmaSub> subs;
     std::map<std::pair<int, int>, std::vector<ItemCost>> wg;  // (K, G) -> items
@@ -1500,7 +1516,14 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
         while ((1 << k1) < mp->K1) ++k1;
         while ((1 << k5) < mp->K5) ++k5;
         const int gid = group_of[i];
-        if (gid < 0) {
+        const int sid = sgroup_of[i];
+        if (sid >= 0) {
+            if (stable[(size_t)kAdjG * sid] == i)  // the group's first map brings the group's items
+                for (ItemCost ic : mp->cost1) {
+                    ic.it.map = sid;
+                    ws[mp->K1 - 1].push_back(ic);
+                }
+        } else if (gid < 0) {
             for (ItemCost ic : mp->cost1) {
                 ic.it.map = i;
                 w1[k1].push_back(ic);
@@ -1590,6 +1613,23 @@ int cmbl_batch_create(cmbl_batch** out, cmbl_map* const* maps, int nmaps)
         gl.n = (int)w.size();
         gl.d_items = (WorkItem*)q;
         b->syn_groups.push_back(gl);
+    }
+    for (int k = 0; k < 2; ++k) {
+        if (ws[k].empty()) continue;
+        const std::vector<WorkItem> w = order_items(ws[k]);
+        if (hipMalloc(&q, sizeof(WorkItem) * w.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
+        b->allocs.push_back(q);
+        if (hipMemcpy(q, w.data(), sizeof(WorkItem) * w.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(CMBL_ERR_HIP, "batch upload"));
+        b->d_items_smma[k] = (WorkItem*)q;
+        b->n_smma[k] = (int)w.size();
+    }
+    if (!stable.empty()) {
+        if (hipMalloc(&q, sizeof(int) * stable.size()) != hipSuccess) return cleanup(fail(CMBL_ERR_ALLOC, "batch"));
+        b->allocs.push_back(q);
+        if (hipMemcpy(q, stable.data(), sizeof(int) * stable.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(CMBL_ERR_HIP, "batch upload"));
+        b->d_smaps = (int*)q;
     }
     if (!wa.empty()) {
         const std::vector<WorkItem> w = order_items(wa);
@@ -1686,6 +1726,7 @@ int cmbl_batch_value_and_grad_device(cmbl_batch* b, const double* const* x, doub
     for (int k = 0; k < 4; ++k) CMBL_TRY(launch_synth(1 << k, b->d_items1[k], b->n1[k], b->d_descs, s));
     for (const cmbl_batch::SynGroupList& gl : b->syn_groups)
         CMBL_TRY(launch_synth_group(gl.K, gl.G, gl.d_items, gl.n, b->d_descs, b->d_gmaps, s));
+    for (int k = 0; k < 2; ++k) CMBL_TRY(launch_synth_mma(k + 1, b->d_items_smma[k], b->n_smma[k], b->d_descs, b->d_smaps, s));
     CMBL_TRY(run_ring(kRingLike, b->d_ring_items, b->ring_classes, b->d_descs, nullptr, nullptr, b->d_tw, s));
     CMBL_TRY(launch_adj_mma(b->d_items_mma, b->d_subs_mma, b->n_mma, b->d_descs, b->d_amaps, s));
     for (int k = 0; k < 4; ++k) CMBL_TRY(launch_adj(1 << k, b->d_items5[k], b->n5[k], b->d_descs, s));
